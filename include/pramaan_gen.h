@@ -1,0 +1,61 @@
+/*
+ * pramaan_gen.h -- synthetic LP workloads with a known optimum.
+ *
+ * No reference counterpart: the reference builds its benchmark LPs inside
+ * its harnesses (benchmark/rolling_horizon_bench.cpp:131-233).  These
+ * generators produce the workloads the engine is measured on, each with a
+ * primal-dual pair (x*, y*) that satisfies the KKT conditions exactly, so
+ * the optimum p* = c^T x* is known in closed form.
+ *
+ * Construction (minimisation, y in the textbook sign):
+ *   A: for each column j, k_j distinct rows drawn uniformly without
+ *      replacement (k_j uniform in [k_min, k_max], capped at m), values
+ *      N(0,1) * (1 + U(-noise, noise)) rounded to FP32.
+ *   x*: a random floor(support * n) columns get U(0,1), the rest 0;
+ *      bounds 0 <= x <= var_upper.
+ *   rows: a random round(eq_frac * m) rows are equalities b_i = (A x*)_i
+ *      with y*_i ~ N(0,1); the others are "<=" rows with
+ *      b_i = (A x*)_i + U(0,1) and y*_i = 0 (inactive).
+ *   c = A^T y* + z with z_j = 0 where x*_j > 0 and U(0,1) elsewhere.
+ * The engine's dual (reference sign) converges to -y*.
+ */
+#ifndef PRAMAAN_GEN_H
+#define PRAMAAN_GEN_H
+
+#include <stdint.h>
+
+#include "pramaan_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pm_gen_params {
+    int32_t m, n;
+    int32_t k_min, k_max;   /* nonzeros per column                        */
+    double eq_frac;         /* fraction of equality rows (1.0: all)        */
+    double noise;           /* relative value noise (0.05: +-5 %)          */
+    double support;         /* fraction of columns with x*_j > 0 (0.3)     */
+    double var_upper;       /* upper bound of every x_j (10)               */
+    uint64_t seed;
+} pm_gen_params;
+
+typedef struct pm_gen_lp pm_gen_lp;
+
+/* Parameters of the named workloads:
+ *  0: m=2,000 n=5,000, 8 nnz/col, all equality rows (config 0)
+ *  1: m=500,000 n=1,000,000, 12 nnz/col, 60 % equality, 5 % noise
+ *  2: 1/10 of workload 1: m=50,000 n=100,000
+ *  3: one instance of the batch workload: m log-uniform in [50, 5,000]
+ *     drawn from the seed, n = 2.5 m, 4-16 nnz/col, all equality rows */
+int pm_gen_params_named(int32_t which, uint64_t seed, pm_gen_params* out);
+int pm_gen_synthetic(const pm_gen_params* p, pm_gen_lp** out);
+/* Views into the generated arrays (valid until pm_gen_free). */
+int pm_gen_view(const pm_gen_lp* g, pm_lp* lp, const double** x_star,
+                const double** y_star, double* p_star);
+void pm_gen_free(pm_gen_lp* g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

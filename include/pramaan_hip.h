@@ -1,0 +1,210 @@
+/*
+ * pramaan_hip.h -- C-ABI of the MI355X (gfx950) PDHG LP engine.
+ *
+ * This is the drop-in boundary for PRAMAAN's GPU PDHG path.  Every entry
+ * point takes plain pointers and sizes (no C++ or torch types), returns an
+ * int status (PM_OK on success) and never throws; the message of the last
+ * failure on the calling thread is available from pm_last_error().
+ *
+ * The C++ solver interface (pramaan::gpu::PdhgSolver, pramaan::Backend) in
+ * include/pramaan/ is a thin header-level wrapper over these functions.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   pm_pdhg_solve            <- gpu::PdhgSolver::solve
+ *                               include/pramaan/gpu/pdhg_solver.hpp:15-24,
+ *                               src/gpu/pdhg_solver.cu:169-411
+ *   pm_pdhg_options_default  <- gpu::PdhgOptions defaults
+ *                               include/pramaan/gpu/pdhg_solver.hpp:9-13
+ *   pm_launch_spmv           <- gpu::launch_spmv   src/gpu/pdhg.cu:156-196
+ *   pm_launch_spmv_t         <- gpu::launch_spmv_t src/gpu/pdhg.cu:198-241
+ *   pm_launch_pdhg_iteration <- gpu::launch_pdhg_iteration src/gpu/pdhg.cu:77-146
+ *   pm_last_error            <- the message CUDA_CHECK puts in the
+ *                               std::runtime_error it throws
+ *                               src/gpu/cuda_utils.cuh:8-15
+ * The device-resident context (pm_ctx_*) and the batched solver
+ * (pm_batch_solve) have no reference counterpart: the reference solves one
+ * LP per call and re-uploads it every time (pdhg_solver.cu:252-297).
+ */
+#ifndef PRAMAAN_HIP_H
+#define PRAMAAN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Infinite bound sentinel, ModelIR's kInfinity (include/pramaan/ir.hpp:47). */
+#define PM_INFINITY_VALUE 1.0e30
+
+/* ---- return codes ------------------------------------------------------ */
+#define PM_OK                 0
+#define PM_ERR_INVALID_ARG    1 /* bad sizes, null pointers, inconsistent CSR */
+#define PM_ERR_HIP            2 /* a HIP runtime call failed               */
+#define PM_ERR_NO_DEVICE      3 /* no usable gfx950 device                  */
+#define PM_ERR_OUT_OF_MEMORY  4
+#define PM_ERR_INTERNAL       5
+
+/* ---- SolveStatus (include/pramaan/simplex.hpp:26-34), same order ------- */
+#define PM_SOLVE_OPTIMAL              0
+#define PM_SOLVE_INFEASIBLE           1
+#define PM_SOLVE_UNBOUNDED            2
+#define PM_SOLVE_ITERATION_LIMIT      3
+#define PM_SOLVE_NUMERICAL_FAILURE    4
+#define PM_SOLVE_WARM_START_REJECTED  5
+
+/* ---- algorithm --------------------------------------------------------- */
+/* PM_ALG_REFERENCE: the reference PDHG (pdhg_solver.cu:169-411): diagonal
+ *   Pock-Chambolle steps 0.95/sum|A| (237-250), FP32 iterations, FP64
+ *   residual check every check_frequency iterations, stagnation restarts
+ *   (307-381).  Same statuses, iteration count and residual definitions.
+ * PM_ALG_PDLP: restarted PDHG on the Ruiz + Pock-Chambolle scaled LP with a
+ *   power-iteration step size, primal-weight updates, KKT-driven adaptive
+ *   restarts and an FP64 iterative-refinement outer loop.  Terminates on
+ *   the FP64 relative KKT error of the original LP. */
+#define PM_ALG_REFERENCE 0
+#define PM_ALG_PDLP      1
+
+/* ---- termination test --------------------------------------------------- */
+#define PM_TERM_REFERENCE    0 /* pv <= tol && dv <= tol (pdhg_solver.cu:356) */
+#define PM_TERM_RELATIVE_KKT 1 /* max(rel primal, rel dual, rel gap) <= tol   */
+
+/* ---- SpMV variants ------------------------------------------------------ */
+#define PM_SPMV_AUTO        0  /* autotuned per matrix                        */
+#define PM_SPMV_SEQUENTIAL  1  /* one lane per row, sequential order, no FMA:
+                                  bitwise equal to a sequential CPU loop     */
+#define PM_SPMV_SCALAR      2  /* one lane per row, FMA                       */
+#define PM_SPMV_VECTOR2     3  /* 2 lanes per row, wavefront-shuffle reduce  */
+#define PM_SPMV_VECTOR4     4
+#define PM_SPMV_VECTOR8     5
+#define PM_SPMV_VECTOR16    6
+#define PM_SPMV_VECTOR32    7
+#define PM_SPMV_VECTOR64    8
+#define PM_SPMV_STREAM      9  /* LDS-staged row segments (CSR-stream)        */
+#define PM_SPMV_NUM_VARIANTS 10
+
+/* ---- the LP, in the layout of ModelIR (include/pramaan/ir.hpp:62-119) ---
+ * optimize sense: c^T x + offset  s.t.  row_lower <= A x <= row_upper,
+ *                                       var_lower <= x   <= var_upper.
+ * Infinite bounds are +-1e30 (ir.hpp:47).  A is CSR with int32 indices,
+ * column indices sorted and unique within a row (sparse_matrix.hpp:28-35). */
+typedef struct pm_lp {
+    int32_t m, n, nnz;
+    int32_t sense;            /* 0 = minimize, 1 = maximize                  */
+    double offset;
+    const double* c;          /* [n] */
+    const int32_t* row_ptr;   /* [m+1] */
+    const int32_t* col_idx;   /* [nnz] */
+    const double* values;     /* [nnz] */
+    const double* row_lower;  /* [m] */
+    const double* row_upper;  /* [m] */
+    const double* var_lower;  /* [n] */
+    const double* var_upper;  /* [n] */
+} pm_lp;
+
+/* ---- options: PdhgOptions (pdhg_solver.hpp:9-13) plus extensions -------
+ * Defaults (pm_pdhg_options_default) equal the reference's defaults, so a
+ * caller that only sets the three reference fields gets the reference
+ * algorithm. */
+typedef struct pm_pdhg_options {
+    double  tolerance;         /* 1e-6                                       */
+    int32_t max_iterations;    /* 100000                                     */
+    int32_t check_frequency;   /* 100                                        */
+    int32_t algorithm;         /* PM_ALG_REFERENCE                           */
+    int32_t termination;       /* PM_TERM_REFERENCE                          */
+    int32_t spmv_variant;      /* PM_SPMV_AUTO                               */
+    int32_t ruiz_iterations;   /* PDLP: Ruiz passes (10)                     */
+    int32_t max_refinements;   /* PDLP: FP64 refinement rounds (8)           */
+    int32_t device;            /* HIP device ordinal (0)                     */
+    int32_t verbose;           /* 0: silent; 1: one line per check to stderr */
+    double  primal_weight;     /* PDLP: initial primal weight, <= 0 = auto   */
+    const double* x0;          /* optional warm start [n], original space    */
+    const double* y0;          /* optional warm start [m], reference sign    */
+} pm_pdhg_options;
+
+/* ---- result: SolveResult (simplex.hpp:46-76) plus PDHG diagnostics ------ */
+typedef struct pm_pdhg_result {
+    int32_t status;            /* PM_SOLVE_*                                 */
+    int32_t iterations;        /* iter+1 when optimal, else max_iterations   */
+    int32_t restarts;
+    int32_t refinements;
+    double  objective;         /* c^T x + offset in the model's sense        */
+    double  primal_residual;   /* reference pv (pdhg_solver.cu:124-134)      */
+    double  dual_residual;     /* reference dv (pdhg_solver.cu:136-157)      */
+    double  rel_kkt;           /* max of the three relative errors below     */
+    double  rel_primal;        /* ||Ax - proj(Ax)||_2 / (1 + ||b||_2)        */
+    double  rel_dual;          /* ||c + A^T y - lambda||_2 / (1 + ||c||_2)   */
+    double  rel_gap;           /* |p - d| / (1 + |p| + |d|)                  */
+    double  primal_objective;  /* minimization form, without offset          */
+    double  dual_objective;
+    double  primal_weight;
+    double  device_seconds;    /* wall time of the device solve loop         */
+    double* x;                 /* caller-owned [n] or NULL                   */
+    double* y;                 /* caller-owned [m] or NULL (reference sign:
+                                  reduced cost is c + A^T y)                 */
+    double* row_activity;      /* caller-owned [m] or NULL: A x in FP64      */
+} pm_pdhg_result;
+
+typedef struct pm_ctx pm_ctx;
+
+/* ---- library ------------------------------------------------------------ */
+const char* pm_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* pm_last_error(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int pm_device_count(int32_t* count);
+
+/* ---- the solver --------------------------------------------------------- */
+void pm_pdhg_options_default(pm_pdhg_options* opts);
+/* Solves one LP.  The LP is copied to the device; res->x/y/row_activity are
+ * filled when non-NULL.  Replaces PdhgSolver::solve (pdhg_solver.cu:169). */
+int pm_pdhg_solve(const pm_lp* lp, const pm_pdhg_options* opts,
+                  pm_pdhg_result* res);
+
+/* ---- test-facing kernels (host vectors in, host vectors out) ----------- */
+/* y[m] = A x, A in CSR (row_ptr[m+1], col_idx/values[nnz]), x[n].
+ * variant: PM_SPMV_*; PM_SPMV_AUTO picks by average row length. */
+int pm_launch_spmv(const int32_t* row_ptr, const int32_t* col_idx,
+                   const float* values, int32_t nnz, const float* x,
+                   float* y, int32_t m, int32_t n, int32_t variant);
+/* y[n] = A^T x, x[m]; computed as a row-parallel SpMV over the transpose
+ * CSR (deterministic, no atomics; the reference scatters with atomicAdd). */
+int pm_launch_spmv_t(const int32_t* row_ptr, const int32_t* col_idx,
+                     const float* values, int32_t nnz, const float* x,
+                     float* y, int32_t m, int32_t n, int32_t variant);
+/* One simplified PDHG step on dense vectors (pdhg.cu:42-66):
+ * x <- max(0, x - tau*gradient) (n), y <- y + sigma*(ax_bar - b) (m). */
+int pm_launch_pdhg_iteration(float* x, float* y, const float* gradient,
+                             const float* ax_bar, const float* b, int32_t n,
+                             int32_t m, float tau, float sigma);
+
+/* ---- device-resident engine -------------------------------------------- */
+/* A context owns one device stream and the device copy of one LP (both CSR
+ * orientations, scaled FP32 working data and FP64 originals).  A context is
+ * used by one thread at a time. */
+int pm_ctx_create(int32_t device, pm_ctx** out);
+int pm_ctx_destroy(pm_ctx* ctx);
+/* Uploads and prepares the LP (canonicalisation, scaling, transpose,
+ * step sizes, SpMV autotuning). */
+int pm_ctx_load(pm_ctx* ctx, const pm_lp* lp, const pm_pdhg_options* opts);
+/* Solves the loaded LP from a cold start (or opts->x0/y0 given at load). */
+int pm_ctx_solve(pm_ctx* ctx, pm_pdhg_result* res);
+
+/* Kernel ids for pm_ctx_time_kernel. */
+#define PM_KERNEL_SPMV_A      0 /* A x-bar with the fused dual update       */
+#define PM_KERNEL_SPMV_AT     1 /* A^T y with the fused primal update       */
+#define PM_KERNEL_ITERATION   2 /* both, i.e. one PDHG iteration            */
+/* Launches kernel `which` `reps` times back to back on the context stream
+ * between two HIP events and reports the mean duration per launch and the
+ * algorithmic bytes one launch moves (each array touched once). */
+int pm_ctx_time_kernel(pm_ctx* ctx, int32_t which, int32_t reps,
+                       double* avg_ms, double* alg_bytes);
+/* SpMV variant the context chose for A and A^T (PM_SPMV_*). */
+int pm_ctx_spmv_variants(pm_ctx* ctx, int32_t* variant_a,
+                         int32_t* variant_at);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRAMAAN_HIP_H */

@@ -1,0 +1,48 @@
+// ref_driver.cpp -- command-line driver around the reference's own CPU
+// solver, compiled from the reference sources by oracle/Makefile into
+// oracle/_ref/ref_driver.  TEST INFRASTRUCTURE: it gives tests FP64
+// objectives and infeasible/unbounded verdicts of the reference's
+// RevisedSimplex (src/simplex/revised_simplex.cpp:484-680) on small LPs.
+//
+//   ref_driver simplex <model.mps>   -> "status objective iterations"
+//   ref_driver ruiz <model.mps> <passes> -> "sum_row_scale sum_col_scale sum|A|"
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "pramaan/mps_parser.hpp"
+#include "pramaan/presolve.hpp"
+#include "pramaan/simplex.hpp"
+#include "pramaan/transformation_ledger.hpp"
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: ref_driver simplex|ruiz <model.mps> [passes]\n");
+        return 2;
+    }
+    try {
+        pramaan::ModelIR model = pramaan::parse_mps(argv[2]);
+        if (std::strcmp(argv[1], "simplex") == 0) {
+            pramaan::RevisedSimplex solver;
+            const pramaan::SolveResult r = solver.solve(model);
+            std::printf("%d %.17g %d\n", static_cast<int>(r.status), r.objective_value, r.iterations);
+            return 0;
+        }
+        if (std::strcmp(argv[1], "ruiz") == 0) {
+            const int passes = argc > 3 ? std::atoi(argv[3]) : 10;
+            pramaan::TransformationLedger ledger;
+            const pramaan::ScalingFactors f = pramaan::ruiz_scale(model, ledger, passes);
+            double sr = 0, sc = 0, sa = 0;
+            for (double v : f.row_scale) sr += v;
+            for (double v : f.col_scale) sc += v;
+            for (double v : model.A.values()) sa += std::fabs(v);
+            std::printf("%.17g %.17g %.17g\n", sr, sc, sa);
+            return 0;
+        }
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 2;
+}

@@ -1,0 +1,377 @@
+"""ctypes binding of the engine's C-ABI (include/pramaan_hip.h,
+include/pramaan_gen.h).
+
+The library is loaded from pramaan_amd/lib/libpramaan_hip.so (built in-tree
+by pramaan_amd.build).  Loading fails loudly when it is missing: there is no
+CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libpramaan_hip.so"
+
+# ---- constants (pramaan_hip.h) ------------------------------------------
+PM_OK = 0
+PM_ERR_INVALID_ARG, PM_ERR_HIP, PM_ERR_NO_DEVICE, PM_ERR_OUT_OF_MEMORY, PM_ERR_INTERNAL = 1, 2, 3, 4, 5
+SOLVE_STATUS = {0: "OPTIMAL", 1: "INFEASIBLE", 2: "UNBOUNDED", 3: "ITERATION_LIMIT",
+                4: "NUMERICAL_FAILURE", 5: "WARM_START_REJECTED"}
+PM_ALG_REFERENCE, PM_ALG_PDLP = 0, 1
+PM_TERM_REFERENCE, PM_TERM_RELATIVE_KKT = 0, 1
+SPMV = {"auto": 0, "sequential": 1, "scalar": 2, "vector2": 3, "vector4": 4, "vector8": 5,
+        "vector16": 6, "vector32": 7, "vector64": 8, "stream": 9}
+SPMV_NAMES = {v: k for k, v in SPMV.items()}
+PM_KERNEL_SPMV_A, PM_KERNEL_SPMV_AT, PM_KERNEL_ITERATION = 0, 1, 2
+INF = 1.0e30
+
+_d = C.POINTER(C.c_double)
+_i = C.POINTER(C.c_int32)
+_f = C.POINTER(C.c_float)
+
+
+class pm_lp(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("nnz", C.c_int32), ("sense", C.c_int32),
+                ("offset", C.c_double), ("c", _d), ("row_ptr", _i), ("col_idx", _i), ("values", _d),
+                ("row_lower", _d), ("row_upper", _d), ("var_lower", _d), ("var_upper", _d)]
+
+
+class pm_pdhg_options(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int32), ("check_frequency", C.c_int32),
+                ("algorithm", C.c_int32), ("termination", C.c_int32), ("spmv_variant", C.c_int32),
+                ("ruiz_iterations", C.c_int32), ("max_refinements", C.c_int32), ("device", C.c_int32),
+                ("verbose", C.c_int32), ("primal_weight", C.c_double), ("x0", _d), ("y0", _d)]
+
+
+class pm_pdhg_result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("restarts", C.c_int32),
+                ("refinements", C.c_int32), ("objective", C.c_double), ("primal_residual", C.c_double),
+                ("dual_residual", C.c_double), ("rel_kkt", C.c_double), ("rel_primal", C.c_double),
+                ("rel_dual", C.c_double), ("rel_gap", C.c_double), ("primal_objective", C.c_double),
+                ("dual_objective", C.c_double), ("primal_weight", C.c_double), ("device_seconds", C.c_double),
+                ("x", _d), ("y", _d), ("row_activity", _d)]
+
+
+class pm_gen_params(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("k_min", C.c_int32), ("k_max", C.c_int32),
+                ("eq_frac", C.c_double), ("noise", C.c_double), ("support", C.c_double),
+                ("var_upper", C.c_double), ("seed", C.c_uint64)]
+
+
+# Every symbol the headers declare, with (restype, argtypes).
+SIGNATURES = {
+    "pm_version": (C.c_char_p, []),
+    "pm_last_error": (C.c_char_p, []),
+    "pm_device_count": (C.c_int, [_i]),
+    "pm_pdhg_options_default": (None, [C.POINTER(pm_pdhg_options)]),
+    "pm_pdhg_solve": (C.c_int, [C.POINTER(pm_lp), C.POINTER(pm_pdhg_options), C.POINTER(pm_pdhg_result)]),
+    "pm_launch_spmv": (C.c_int, [_i, _i, _f, C.c_int32, _f, _f, C.c_int32, C.c_int32, C.c_int32]),
+    "pm_launch_spmv_t": (C.c_int, [_i, _i, _f, C.c_int32, _f, _f, C.c_int32, C.c_int32, C.c_int32]),
+    "pm_launch_pdhg_iteration": (C.c_int, [_f, _f, _f, _f, _f, C.c_int32, C.c_int32, C.c_float, C.c_float]),
+    "pm_ctx_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
+    "pm_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "pm_ctx_load": (C.c_int, [C.c_void_p, C.POINTER(pm_lp), C.POINTER(pm_pdhg_options)]),
+    "pm_ctx_solve": (C.c_int, [C.c_void_p, C.POINTER(pm_pdhg_result)]),
+    "pm_ctx_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _d, _d]),
+    "pm_ctx_spmv_variants": (C.c_int, [C.c_void_p, _i, _i]),
+    "pm_gen_params_named": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(pm_gen_params)]),
+    "pm_gen_synthetic": (C.c_int, [C.POINTER(pm_gen_params), C.POINTER(C.c_void_p)]),
+    "pm_gen_view": (C.c_int, [C.c_void_p, C.POINTER(pm_lp), C.POINTER(_d), C.POINTER(_d), _d]),
+    "pm_gen_free": (None, [C.c_void_p]),
+}
+
+_lib = None
+
+
+class PramaanError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pramaan error {code}: {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """Loads the engine library (once).  Raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m pramaan_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != PM_OK:
+        raise PramaanError(rc, lib().pm_last_error().decode())
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    lib().pm_device_count(C.byref(n))
+    return n.value
+
+
+def _p(a: np.ndarray | None, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+@dataclass
+class LP:
+    """An LP in ModelIR layout (include/pramaan/ir.hpp:62-119), FP64."""
+    c: np.ndarray
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+    values: np.ndarray
+    row_lower: np.ndarray
+    row_upper: np.ndarray
+    var_lower: np.ndarray
+    var_upper: np.ndarray
+    sense: int = 0  # 0 minimize, 1 maximize
+    offset: float = 0.0
+    x_star: np.ndarray | None = None
+    y_star: np.ndarray | None = None
+    p_star: float | None = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    def __post_init__(self):
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        self.c, self.values = f64(self.c), f64(self.values)
+        self.row_lower, self.row_upper = f64(self.row_lower), f64(self.row_upper)
+        self.var_lower, self.var_upper = f64(self.var_lower), f64(self.var_upper)
+        self.row_ptr, self.col_idx = i32(self.row_ptr), i32(self.col_idx)
+
+    @property
+    def m(self) -> int:
+        return len(self.row_ptr) - 1
+
+    @property
+    def n(self) -> int:
+        return len(self.c)
+
+    @property
+    def nnz(self) -> int:
+        return len(self.values)
+
+    def struct(self) -> pm_lp:
+        return pm_lp(self.m, self.n, self.nnz, self.sense, self.offset, _p(self.c, C.c_double),
+                     _p(self.row_ptr, C.c_int32), _p(self.col_idx, C.c_int32), _p(self.values, C.c_double),
+                     _p(self.row_lower, C.c_double), _p(self.row_upper, C.c_double),
+                     _p(self.var_lower, C.c_double), _p(self.var_upper, C.c_double))
+
+    @staticmethod
+    def from_dense(A, c, row_lower, row_upper, var_lower, var_upper, sense=0, offset=0.0) -> "LP":
+        A = np.asarray(A, dtype=np.float64).reshape(len(row_lower), len(c))
+        rp, ci, vals = [0], [], []
+        for row in A:
+            for j, a in enumerate(row):
+                if a != 0.0:
+                    ci.append(j)
+                    vals.append(a)
+            rp.append(len(ci))
+        return LP(np.asarray(c, float), np.asarray(rp), np.asarray(ci, dtype=np.int32), np.asarray(vals, float),
+                  np.asarray(row_lower, float), np.asarray(row_upper, float), np.asarray(var_lower, float),
+                  np.asarray(var_upper, float), sense, offset)
+
+
+def options(**kw) -> pm_pdhg_options:
+    o = pm_pdhg_options()
+    lib().pm_pdhg_options_default(C.byref(o))
+    alg = kw.pop("algorithm", None)
+    if alg is not None:
+        o.algorithm = {"reference": PM_ALG_REFERENCE, "pdlp": PM_ALG_PDLP}.get(alg, alg)
+    term = kw.pop("termination", None)
+    if term is not None:
+        o.termination = {"reference": PM_TERM_REFERENCE, "relative_kkt": PM_TERM_RELATIVE_KKT}.get(term, term)
+    sv = kw.pop("spmv_variant", None)
+    if sv is not None:
+        o.spmv_variant = SPMV.get(sv, sv)
+    for k, v in kw.items():
+        if k in ("x0", "y0"):
+            continue
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class Result:
+    status: str
+    iterations: int
+    restarts: int
+    refinements: int
+    objective: float
+    primal_residual: float
+    dual_residual: float
+    rel_kkt: float
+    rel_primal: float
+    rel_dual: float
+    rel_gap: float
+    primal_objective: float
+    dual_objective: float
+    primal_weight: float
+    device_seconds: float
+    x: np.ndarray
+    y: np.ndarray
+    row_activity: np.ndarray
+
+
+def _result(r: pm_pdhg_result, x, y, act) -> Result:
+    return Result(SOLVE_STATUS[r.status], r.iterations, r.restarts, r.refinements, r.objective,
+                  r.primal_residual, r.dual_residual, r.rel_kkt, r.rel_primal, r.rel_dual, r.rel_gap,
+                  r.primal_objective, r.dual_objective, r.primal_weight, r.device_seconds, x, y, act)
+
+
+def _opts_with_warm(lp: LP, kw: dict):
+    x0 = kw.get("x0")
+    y0 = kw.get("y0")
+    o = options(**kw)
+    keep = []
+    if x0 is not None:
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        keep.append(x0)
+        o.x0 = _p(x0, C.c_double)
+    if y0 is not None:
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        keep.append(y0)
+        o.y0 = _p(y0, C.c_double)
+    return o, keep
+
+
+def solve(lp: LP, **kw) -> Result:
+    """pm_pdhg_solve: one LP, uploaded, solved and freed."""
+    o, keep = _opts_with_warm(lp, kw)
+    x = np.zeros(lp.n)
+    y = np.zeros(lp.m)
+    act = np.zeros(lp.m)
+    r = pm_pdhg_result()
+    r.x, r.y, r.row_activity = _p(x, C.c_double), _p(y, C.c_double), _p(act, C.c_double)
+    s = lp.struct()
+    _check(lib().pm_pdhg_solve(C.byref(s), C.byref(o), C.byref(r)))
+    del keep
+    return _result(r, x, y, act)
+
+
+class Context:
+    """pm_ctx_*: an LP resident on the device."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().pm_ctx_create(device, C.byref(self._h)))
+        self.lp = None
+
+    def load(self, lp: LP, **kw) -> None:
+        o, keep = _opts_with_warm(lp, kw)
+        s = lp.struct()
+        _check(lib().pm_ctx_load(self._h, C.byref(s), C.byref(o)))
+        self.lp = lp
+
+    def solve(self, want_vectors: bool = True) -> Result:
+        lp = self.lp
+        x = np.zeros(lp.n) if want_vectors else None
+        y = np.zeros(lp.m) if want_vectors else None
+        r = pm_pdhg_result()
+        if want_vectors:
+            r.x, r.y = _p(x, C.c_double), _p(y, C.c_double)
+        _check(lib().pm_ctx_solve(self._h, C.byref(r)))
+        return _result(r, x, y, None)
+
+    def time_kernel(self, which: int, reps: int) -> tuple[float, float]:
+        ms, b = C.c_double(), C.c_double()
+        _check(lib().pm_ctx_time_kernel(self._h, which, reps, C.byref(ms), C.byref(b)))
+        return ms.value, b.value
+
+    def spmv_variants(self) -> tuple[str, str]:
+        a, t = C.c_int32(), C.c_int32()
+        _check(lib().pm_ctx_spmv_variants(self._h, C.byref(a), C.byref(t)))
+        return SPMV_NAMES[a.value], SPMV_NAMES[t.value]
+
+    def close(self) -> None:
+        if self._h:
+            lib().pm_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _csr32(row_ptr, col_idx, values):
+    return (np.ascontiguousarray(row_ptr, np.int32), np.ascontiguousarray(col_idx, np.int32),
+            np.ascontiguousarray(values, np.float32))
+
+
+def launch_spmv(row_ptr, col_idx, values, x, m, n, variant="auto") -> np.ndarray:
+    """pm_launch_spmv: y = A x in FP32 on the GPU."""
+    rp, ci, v = _csr32(row_ptr, col_idx, values)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros(m, np.float32)
+    _check(lib().pm_launch_spmv(_p(rp, C.c_int32), _p(ci, C.c_int32), _p(v, C.c_float), len(v),
+                                _p(x, C.c_float), _p(y, C.c_float), m, n, SPMV.get(variant, variant)))
+    return y
+
+
+def launch_spmv_t(row_ptr, col_idx, values, x, m, n, variant="auto") -> np.ndarray:
+    """pm_launch_spmv_t: y = A^T x in FP32 on the GPU."""
+    rp, ci, v = _csr32(row_ptr, col_idx, values)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros(n, np.float32)
+    _check(lib().pm_launch_spmv_t(_p(rp, C.c_int32), _p(ci, C.c_int32), _p(v, C.c_float), len(v),
+                                  _p(x, C.c_float), _p(y, C.c_float), m, n, SPMV.get(variant, variant)))
+    return y
+
+
+def launch_pdhg_iteration(x, y, gradient, ax_bar, b, tau, sigma):
+    """pm_launch_pdhg_iteration; returns updated copies of x and y."""
+    x = np.array(x, np.float32)
+    y = np.array(y, np.float32)
+    g = np.ascontiguousarray(gradient, np.float32)
+    a = np.ascontiguousarray(ax_bar, np.float32)
+    bb = np.ascontiguousarray(b, np.float32)
+    _check(lib().pm_launch_pdhg_iteration(_p(x, C.c_float), _p(y, C.c_float), _p(g, C.c_float),
+                                          _p(a, C.c_float), _p(bb, C.c_float), len(x), len(y),
+                                          C.c_float(tau), C.c_float(sigma)))
+    return x, y
+
+
+def generate(which: int | None = None, seed: int = 1, **params) -> LP:
+    """Synthetic LP with known optimum (pramaan_gen.h).  `which` names a
+    workload (0: 2k x 5k, 1: 500k x 1M, 2: 50k x 100k, 3: batch instance);
+    keyword parameters override the fields of pm_gen_params."""
+    p = pm_gen_params()
+    if which is not None:
+        _check(lib().pm_gen_params_named(which, seed, C.byref(p)))
+    else:
+        p.support, p.var_upper, p.eq_frac, p.seed = 0.3, 10.0, 1.0, seed
+    for k, v in params.items():
+        setattr(p, k, v)
+    h = C.c_void_p()
+    _check(lib().pm_gen_synthetic(C.byref(p), C.byref(h)))
+    try:
+        s = pm_lp()
+        xs, ys = _d(), _d()
+        ps = C.c_double()
+        _check(lib().pm_gen_view(h, C.byref(s), C.byref(xs), C.byref(ys), C.byref(ps)))
+        m, n, nnz = s.m, s.n, s.nnz
+        arr = lambda ptr, k: np.ctypeslib.as_array(ptr, (k,)).copy() if k else np.zeros(0)
+        lp = LP(arr(s.c, n), np.ctypeslib.as_array(s.row_ptr, (m + 1,)).copy(),
+                np.ctypeslib.as_array(s.col_idx, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
+                arr(s.values, nnz), arr(s.row_lower, m), arr(s.row_upper, m), arr(s.var_lower, n),
+                arr(s.var_upper, n), 0, 0.0, arr(xs, n), arr(ys, m), ps.value)
+        return lp
+    finally:
+        lib().pm_gen_free(h)

@@ -1,0 +1,112 @@
+// dispatch.hpp -- host-side launch of the templated row kernels by SpMV
+// variant (PM_SPMV_*), and the small elementwise/reduction launchers.
+#pragma once
+
+#include <type_traits>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pm {
+
+template <int V>
+using IntC = std::integral_constant<int, V>;
+template <bool V>
+using BoolC = std::integral_constant<bool, V>;
+
+// Lanes per row of a variant (the LDS-stream id falls back to 8 lanes).
+inline int variant_lanes(int variant) {
+    switch (variant) {
+        case PM_SPMV_SEQUENTIAL:
+        case PM_SPMV_SCALAR: return 1;
+        case PM_SPMV_VECTOR2: return 2;
+        case PM_SPMV_VECTOR4: return 4;
+        case PM_SPMV_VECTOR8: return 8;
+        case PM_SPMV_VECTOR16: return 16;
+        case PM_SPMV_VECTOR32: return 32;
+        case PM_SPMV_VECTOR64: return 64;
+        default: return 8;
+    }
+}
+
+inline int grid_for(int nrows, int lanes) {
+    const long long threads = static_cast<long long>(nrows) * lanes;
+    return static_cast<int>((threads + kBlock - 1) / kBlock);
+}
+
+template <class F>
+void with_variant(int variant, F&& f) {
+    switch (variant) {
+        case PM_SPMV_SEQUENTIAL: f(IntC<1>{}, BoolC<true>{}); break;
+        case PM_SPMV_SCALAR: f(IntC<1>{}, BoolC<false>{}); break;
+        case PM_SPMV_VECTOR2: f(IntC<2>{}, BoolC<false>{}); break;
+        case PM_SPMV_VECTOR4: f(IntC<4>{}, BoolC<false>{}); break;
+        case PM_SPMV_VECTOR16: f(IntC<16>{}, BoolC<false>{}); break;
+        case PM_SPMV_VECTOR32: f(IntC<32>{}, BoolC<false>{}); break;
+        case PM_SPMV_VECTOR64: f(IntC<64>{}, BoolC<false>{}); break;
+        case PM_SPMV_VECTOR8:
+        default: f(IntC<8>{}, BoolC<false>{}); break;
+    }
+}
+
+// y-side row kernel over `nrows` rows.
+template <class AT, class VT, class XT, class Epi>
+void launch_rows(int variant, int nrows, const int* rp, const int* ci, const VT* v, const XT* x,
+                 const Epi& epi, hipStream_t s) {
+    if (nrows <= 0) return;
+    with_variant(variant, [&](auto Lc, auto Ec) {
+        constexpr int L = decltype(Lc)::value;
+        constexpr bool E = decltype(Ec)::value;
+        hipLaunchKernelGGL((k_rows<L, E, AT, VT, XT, Epi>), dim3(grid_for(nrows, L)), dim3(kBlock), 0, s,
+                           nrows, rp, ci, v, x, epi);
+    });
+    PM_LAUNCHED();
+}
+
+// Row kernel with block-partial reduction, followed by the final reduce of
+// its partials into out[0 .. NS+NM).  `partials` must hold
+// grid_for(nrows, lanes) * (NS+NM) doubles.
+template <class AT, class VT, class XT, class Epi>
+void launch_rows_reduce(int variant, int nrows, const int* rp, const int* ci, const VT* v,
+                        const XT* x, const Epi& epi, double* partials, double* out, hipStream_t s) {
+    constexpr int NA = Epi::NS + Epi::NM;
+    int grid = 0;
+    if (nrows > 0) {
+        with_variant(variant, [&](auto Lc, auto Ec) {
+            constexpr int L = decltype(Lc)::value;
+            constexpr bool E = decltype(Ec)::value;
+            grid = grid_for(nrows, L);
+            hipLaunchKernelGGL((k_rows_reduce<L, E, AT, VT, XT, Epi>), dim3(grid), dim3(kBlock), 0, s,
+                               nrows, rp, ci, v, x, epi, partials);
+        });
+        PM_LAUNCHED();
+    }
+    hipLaunchKernelGGL(k_final_reduce, dim3(1), dim3(kBlock), 0, s, partials, grid, Epi::NS, Epi::NM, out);
+    PM_LAUNCHED();
+    (void)NA;
+}
+
+template <class F>
+void launch_elem(int n, const F& f, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_elem<F>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, f);
+    PM_LAUNCHED();
+}
+
+template <class F>
+void launch_elem_reduce(int n, const F& f, double* partials, double* out, hipStream_t s) {
+    const int grid = n > 0 ? (n + kBlock - 1) / kBlock : 0;
+    if (grid > 0) {
+        hipLaunchKernelGGL((k_elem_reduce<F>), dim3(grid), dim3(kBlock), 0, s, n, f, partials);
+        PM_LAUNCHED();
+    }
+    hipLaunchKernelGGL(k_final_reduce, dim3(1), dim3(kBlock), 0, s, partials, grid, F::NS, F::NM, out);
+    PM_LAUNCHED();
+}
+
+// Partial-buffer size (doubles) needed by a reduce over `nrows` rows.
+inline size_t partials_needed(int nrows, int width) {
+    return static_cast<size_t>(grid_for(nrows > 0 ? nrows : 1, 64)) * static_cast<size_t>(width) + 64;
+}
+
+}  // namespace pm

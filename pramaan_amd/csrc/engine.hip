@@ -1,0 +1,1152 @@
+// engine.hip -- the device-resident PDHG engine behind the C-ABI.
+//
+// pm_pdhg_solve replaces gpu::PdhgSolver::solve (reference
+// src/gpu/pdhg_solver.cu:169-411).  Two algorithms share the kernels:
+//
+//  * PM_ALG_REFERENCE restates the reference solver: canonicalisation to
+//    FP32 (196-229), diagonal steps 0.95/sum|A| (231-250), x0 = clamp(0)
+//    (288-297), dual-then-primal FP32 iterations (319-338), FP64 residual
+//    check every check_frequency iterations (340-360), stagnation restarts
+//    (362-381), final residuals and FP64 objective (385-410).  The check
+//    runs on the device in FP64 and only two numbers cross PCIe.
+//
+//  * PM_ALG_PDLP: reflected Halpern PDHG on the Ruiz (10 inf-norm passes) +
+//    Pock-Chambolle (alpha = 1) scaled LP with constant step 0.99/||A||_2
+//    (power iteration), KKT-driven adaptive restarts (0.2 / 0.8 / 0.36), a
+//    primal-weight update at each restart, and an FP64 iterative-refinement
+//    outer loop: each round solves, in FP32, the correction LP of the FP64
+//    residuals of the current point, amplified by 1/KKT.  Termination and
+//    every decision use the FP64 relative KKT error of the original LP.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <tuple>
+
+#include "dispatch.hpp"
+
+namespace pm {
+
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+void clear_last_error() { g_last_error.clear(); }
+
+// ---------------------------------------------------------------------------
+// Elementwise functors
+// ---------------------------------------------------------------------------
+template <class T>
+struct FnFill {
+    T* p;
+    T v;
+    __device__ void operator()(int i) const { p[i] = v; }
+};
+template <class T>
+struct FnCopy {
+    T* dst;
+    const T* src;
+    __device__ void operator()(int i) const { dst[i] = src[i]; }
+};
+// x = clamp(0, lo, hi) into up to three buffers (pdhg_solver.cu:288-297).
+struct FnInitX {
+    const float* lo;
+    const float* hi;
+    float* a;
+    float* b;
+    float* c;
+    __device__ void operator()(int i) const {
+        const float v = fmaxf(lo[i], fminf(0.0f, hi[i]));
+        a[i] = v;
+        if (b) b[i] = v;
+        if (c) c[i] = v;
+    }
+};
+// x_tot = clamp(x_base + D_c x_hat / DP, l, u) in FP64.
+struct FnTotalX {
+    const double* base;
+    const double* dc;
+    const float* xh;
+    double inv;
+    const double* lo;
+    const double* hi;
+    double* out;
+    __device__ void operator()(int j) const {
+        double v = base[j] + dc[j] * static_cast<double>(xh[j]) * inv;
+        if (v < lo[j]) v = lo[j];
+        if (v > hi[j]) v = hi[j];
+        out[j] = v;
+    }
+};
+// y_tot = y_base + D_r y_hat / DD, projected on the sign its row allows
+// (reference sign: no upper bound -> y <= 0, no lower bound -> y >= 0).
+struct FnTotalY {
+    const double* base;
+    const double* dr;
+    const float* yh;
+    double inv;
+    const double* lo;
+    const double* hi;
+    double* out;
+    __device__ void operator()(int i) const {
+        double v = base[i] + dr[i] * static_cast<double>(yh[i]) * inv;
+        if (!(hi[i] < kInf) && v > 0.0) v = 0.0;
+        if (!(lo[i] > -kInf) && v < 0.0) v = 0.0;
+        out[i] = v;
+    }
+};
+struct FnNormDiff {  // sums: [0] |a - b|^2
+    static constexpr int NS = 1, NM = 0;
+    const float* a;
+    const float* b;
+    __device__ void operator()(int i, double* acc) const {
+        const double d = static_cast<double>(a[i]) - static_cast<double>(b[i]);
+        acc[0] = d * d;
+    }
+};
+struct FnNormSq {
+    static constexpr int NS = 1, NM = 0;
+    const float* a;
+    __device__ void operator()(int i, double* acc) const {
+        const double d = static_cast<double>(a[i]);
+        acc[0] = d * d;
+    }
+};
+// v = w / sqrt(ss) with ss read on the device (power iteration).
+struct FnNormalize {
+    const float* w;
+    const double* ss;
+    float* v;
+    __device__ void operator()(int i) const {
+        const double s = ss[0];
+        v[i] = s > 0.0 ? static_cast<float>(static_cast<double>(w[i]) / sqrt(s)) : 0.0f;
+    }
+};
+
+__device__ inline float to_f32_bound(double v) {
+    if (v >= 1e29) return kFltInf;
+    if (v <= -1e29) return -kFltInf;
+    return static_cast<float>(v);
+}
+
+// Correction-LP column data of a refinement round from FP64 A^T y_base:
+// c_hat = D_c DD (c + A^T y_base), bounds DP (l - x_base) / D_c.
+struct EpiRoundCol {
+    const double* c;
+    const double* lo;
+    const double* hi;
+    const double* xb;
+    const double* dc;
+    double dp, dd;
+    float* c32;
+    float* lo32;
+    float* hi32;
+    __device__ void operator()(int j, double aty) const {
+        c32[j] = to_f32_bound(dc[j] * dd * (c[j] + aty));
+        lo32[j] = lo[j] > -kInf ? to_f32_bound(dp * (lo[j] - xb[j]) / dc[j]) : -kFltInf;
+        hi32[j] = hi[j] < kInf ? to_f32_bound(dp * (hi[j] - xb[j]) / dc[j]) : kFltInf;
+    }
+};
+// Row data: bounds D_r DP (l - A x_base), shift g = -DD y_base / D_r.
+struct EpiRoundRow {
+    const double* lo;
+    const double* hi;
+    const double* yb;
+    const double* dr;
+    double dp, dd;
+    float* lo32;
+    float* hi32;
+    float* g32;
+    __device__ void operator()(int i, double ax) const {
+        lo32[i] = lo[i] > -kInf ? to_f32_bound(dr[i] * dp * (lo[i] - ax)) : -kFltInf;
+        hi32[i] = hi[i] < kInf ? to_f32_bound(dr[i] * dp * (hi[i] - ax)) : kFltInf;
+        g32[i] = to_f32_bound(-dd * yb[i] / dr[i]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// SpMV autotuning cache: (rows, cols, nnz) -> variant
+// ---------------------------------------------------------------------------
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tune_cache;
+
+int heuristic_variant(int nrows, int nnz) {
+    const double avg = nrows > 0 ? static_cast<double>(nnz) / nrows : 0.0;
+    const int want = static_cast<int>(std::ceil(avg / 3.0));
+    int l = 1;
+    while (l < want && l < 32) l <<= 1;
+    switch (l) {
+        case 1: return PM_SPMV_SCALAR;
+        case 2: return PM_SPMV_VECTOR2;
+        case 4: return PM_SPMV_VECTOR4;
+        case 8: return PM_SPMV_VECTOR8;
+        case 16: return PM_SPMV_VECTOR16;
+        default: return PM_SPMV_VECTOR32;
+    }
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+// ---------------------------------------------------------------------------
+// The context
+// ---------------------------------------------------------------------------
+struct pm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    pm_pdhg_options opts{};
+    bool loaded = false;
+
+    int m = 0, n = 0, nnz = 0;
+    int sense = 0;
+    double offset = 0.0;
+    bool nan_cost = false;
+    bool exact_f32 = false;
+    double c_inf32 = 1.0;  // max(1, ||c32||_inf) for the reference dv
+
+    // Host FP64 model in minimisation form (c negated for a maximisation).
+    std::vector<double> c_model, cmin, lr, ur, lx, ux;
+    std::vector<double> x0_h, y0_h;
+
+    // Device structure: A (rp, ci) and A^T (cp, ri).
+    DevBuf<int> rp, ci, cp, ri;
+    DevBuf<float> va, vat;          // FP32 working values (reference: A, PDLP: scaled A)
+    DevBuf<float> oa32, oat32;      // original values, when exactly FP32 (PDLP only)
+    DevBuf<double> oa64, oat64;     // original values otherwise
+    DevBuf<double> d_cmin, d_lr, d_ur, d_lx, d_ux, d_dr, d_dc;
+
+    // FP32 working vectors.
+    DevBuf<float> c32, lr32, ur32, lx32, ux32, sig32, tau32, g32;
+    DevBuf<float> x, y, xe, x0, y0, xt, yt, xl, yl, tmp_n, tmp_m;
+    // FP64 vectors.
+    DevBuf<double> xtot, ytot, xbase, ybase, act64;
+    DevBuf<double> partials, red;
+    PinnedBuf<double> hred;
+
+    int var_a = PM_SPMV_VECTOR8, var_at = PM_SPMV_VECTOR8;
+    double eta = 1.0, omega0 = 1.0, sigma_max = 1.0;
+
+    const void* orig_a() const { return exact_f32 ? static_cast<const void*>(oa32.get()) : oa64.get(); }
+};
+
+namespace pm {
+namespace {
+
+void check_lp(const pm_lp* lp) {
+    if (!lp) fail(PM_ERR_INVALID_ARG, "lp is NULL");
+    if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) fail(PM_ERR_INVALID_ARG, "negative LP dimension");
+    if (lp->n > 0 && (!lp->c || !lp->var_lower || !lp->var_upper))
+        fail(PM_ERR_INVALID_ARG, "c / var bounds are NULL");
+    if (lp->m > 0 && (!lp->row_lower || !lp->row_upper)) fail(PM_ERR_INVALID_ARG, "row bounds are NULL");
+    if (!lp->row_ptr) fail(PM_ERR_INVALID_ARG, "row_ptr is NULL");
+    if (lp->nnz > 0 && (!lp->col_idx || !lp->values)) fail(PM_ERR_INVALID_ARG, "col_idx / values are NULL");
+    if (lp->row_ptr[0] != 0 || lp->row_ptr[lp->m] != lp->nnz)
+        fail(PM_ERR_INVALID_ARG, "row_ptr[0] must be 0 and row_ptr[m] must equal nnz");
+    for (int i = 0; i < lp->m; ++i)
+        if (lp->row_ptr[i] > lp->row_ptr[i + 1]) fail(PM_ERR_INVALID_ARG, "row_ptr must be non-decreasing");
+    for (int k = 0; k < lp->nnz; ++k)
+        if (lp->col_idx[k] < 0 || lp->col_idx[k] >= lp->n) fail(PM_ERR_INVALID_ARG, "col_idx entry out of bounds");
+    for (int j = 0; j < lp->n; ++j)
+        if (lp->var_lower[j] > lp->var_upper[j]) fail(PM_ERR_INVALID_ARG, "var_lower > var_upper");
+    for (int i = 0; i < lp->m; ++i)
+        if (lp->row_lower[i] > lp->row_upper[i]) fail(PM_ERR_INVALID_ARG, "row_lower > row_upper");
+}
+
+void sync(pm_ctx* c) { PM_HIP(hipStreamSynchronize(c->stream)); }
+
+// Transposes (rp, ci, v) of an m x n CSR into (cp, ri, vt): counting sort
+// over rows in increasing order, so each transposed row keeps its entries
+// sorted (the reference's CSRMatrix::transpose, sparse_matrix.cpp:13-57).
+template <class T>
+void host_transpose(int m, int n, const int* rp, const int* ci, const T* v, std::vector<int>& cp,
+                    std::vector<int>& ri, std::vector<T>& vt, std::vector<int64_t>* perm) {
+    const int nnz = rp[m];
+    cp.assign(static_cast<size_t>(n) + 1, 0);
+    for (int k = 0; k < nnz; ++k) ++cp[static_cast<size_t>(ci[k]) + 1];
+    for (int j = 0; j < n; ++j) cp[j + 1] += cp[j];
+    std::vector<int> cur(cp.begin(), cp.end() - 1);
+    ri.resize(nnz);
+    if (v) vt.resize(nnz);
+    if (perm) perm->resize(nnz);
+    for (int i = 0; i < m; ++i)
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int d = cur[ci[k]]++;
+            ri[d] = i;
+            if (v) vt[d] = v[k];
+            if (perm) (*perm)[d] = k;
+        }
+}
+
+// Times one SpMV variant (mean of 5 launches after 2 warm-ups).
+float time_variant(pm_ctx* c, int variant, int nrows, const int* rp, const int* ci, const float* v,
+                   const float* x, float* y) {
+    EpiStore<float> epi{y};
+    for (int r = 0; r < 2; ++r) launch_rows<float>(variant, nrows, rp, ci, v, x, epi, c->stream);
+    PM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < 5; ++r) launch_rows<float>(variant, nrows, rp, ci, v, x, epi, c->stream);
+    PM_HIP(hipEventRecord(c->ev1, c->stream));
+    PM_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    return ms / 5.0f;
+}
+
+int autotune(pm_ctx* c, int nrows, int ncols, int nnz, const int* rp, const int* ci, const float* v,
+             const float* x, float* y) {
+    if (c->opts.spmv_variant != PM_SPMV_AUTO) return c->opts.spmv_variant;
+    if (nnz < (1 << 18) || nrows == 0) return heuristic_variant(nrows, nnz);
+    const auto key = std::make_tuple(nrows, ncols, nnz);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) return it->second;
+    }
+    const double avg = static_cast<double>(nnz) / nrows;
+    int best = heuristic_variant(nrows, nnz);
+    float best_ms = 1e30f;
+    const int cands[] = {PM_SPMV_SCALAR,   PM_SPMV_VECTOR2,  PM_SPMV_VECTOR4, PM_SPMV_VECTOR8,
+                         PM_SPMV_VECTOR16, PM_SPMV_VECTOR32, PM_SPMV_VECTOR64};
+    for (int v_ : cands) {
+        const int lanes = variant_lanes(v_);
+        if (lanes > 2 * avg + 2 && lanes > 2) continue;  // mostly idle lanes
+        const float ms = time_variant(c, v_, nrows, rp, ci, v, x, y);
+        if (ms < best_ms) { best_ms = ms; best = v_; }
+    }
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_cache[key] = best;
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// Load
+// ---------------------------------------------------------------------------
+void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
+    check_lp(lp);
+    pm_pdhg_options opts;
+    pm_pdhg_options_default(&opts);
+    if (o) opts = *o;
+    if (opts.check_frequency <= 0) fail(PM_ERR_INVALID_ARG, "check_frequency must be positive");
+    if (opts.max_iterations < 0) fail(PM_ERR_INVALID_ARG, "max_iterations must be non-negative");
+    if (opts.spmv_variant < 0 || opts.spmv_variant >= PM_SPMV_NUM_VARIANTS)
+        fail(PM_ERR_INVALID_ARG, "unknown spmv_variant");
+    if (opts.algorithm != PM_ALG_REFERENCE && opts.algorithm != PM_ALG_PDLP)
+        fail(PM_ERR_INVALID_ARG, "unknown algorithm");
+    c->opts = opts;
+    c->loaded = false;
+    const int m = lp->m, n = lp->n, nnz = lp->nnz;
+    c->m = m; c->n = n; c->nnz = nnz;
+    c->sense = lp->sense;
+    c->offset = lp->offset;
+    hipStream_t s = c->stream;
+
+    c->c_model.assign(lp->c, lp->c + n);
+    c->nan_cost = false;
+    for (int j = 0; j < n; ++j) if (!std::isfinite(lp->c[j])) c->nan_cost = true;
+    const double sgn = lp->sense == 1 ? -1.0 : 1.0;
+    c->cmin.resize(n);
+    for (int j = 0; j < n; ++j) c->cmin[j] = sgn * lp->c[j];
+    c->lr.assign(lp->row_lower, lp->row_lower + m);
+    c->ur.assign(lp->row_upper, lp->row_upper + m);
+    c->lx.assign(lp->var_lower, lp->var_lower + n);
+    c->ux.assign(lp->var_upper, lp->var_upper + n);
+    c->x0_h.clear(); c->y0_h.clear();
+    if (opts.x0) c->x0_h.assign(opts.x0, opts.x0 + n);
+    if (opts.y0) c->y0_h.assign(opts.y0, opts.y0 + m);
+    c->opts.x0 = nullptr;
+    c->opts.y0 = nullptr;
+    if (n == 0 || c->nan_cost) { c->loaded = true; return; }
+
+    for (auto* b : {&c->oa32, &c->oat32, &c->va, &c->vat}) b->reset();
+    for (auto* b : {&c->oa64, &c->oat64, &c->d_dr, &c->d_dc}) b->reset();
+    c->exact_f32 = true;
+    for (int k = 0; k < nnz; ++k)
+        if (static_cast<double>(static_cast<float>(lp->values[k])) != lp->values[k]) { c->exact_f32 = false; break; }
+
+    std::vector<int> cp, ri;
+    std::vector<double> vt64;
+    host_transpose<double>(m, n, lp->row_ptr, lp->col_idx, lp->values, cp, ri, vt64, nullptr);
+    c->rp.upload(lp->row_ptr, static_cast<size_t>(m) + 1, s);
+    c->ci.upload(lp->col_idx, nnz, s);
+    c->cp.upload(cp, s);
+    c->ri.upload(ri, s);
+
+    auto up64 = [&](DevBuf<double>& d, const std::vector<double>& h) { d.upload(h, s); };
+    up64(c->d_cmin, c->cmin);
+    up64(c->d_lr, c->lr);
+    up64(c->d_ur, c->ur);
+    up64(c->d_lx, c->lx);
+    up64(c->d_ux, c->ux);
+
+    // FP32 copies of bounds with the +-1e30f sentinel (pdhg_solver.cu:198-220).
+    std::vector<float> lr32(m), ur32(m), lx32(n), ux32(n);
+    for (int i = 0; i < m; ++i) {
+        lr32[i] = c->lr[i] <= -1e29 ? -kFltInf : static_cast<float>(c->lr[i]);
+        ur32[i] = c->ur[i] >= 1e29 ? kFltInf : static_cast<float>(c->ur[i]);
+    }
+    for (int j = 0; j < n; ++j) {
+        lx32[j] = c->lx[j] <= -1e29 ? -kFltInf : static_cast<float>(c->lx[j]);
+        ux32[j] = c->ux[j] >= 1e29 ? kFltInf : static_cast<float>(c->ux[j]);
+    }
+
+    const size_t pn = partials_needed(std::max(m, n), 16);
+    c->partials.resize(pn);
+    c->red.resize(64);
+    c->hred.resize(64);
+    for (auto* b : {&c->x, &c->xe, &c->x0, &c->xt, &c->xl, &c->tmp_n}) b->resize(n);
+    for (auto* b : {&c->y, &c->y0, &c->yt, &c->yl, &c->tmp_m}) b->resize(std::max(m, 1));
+    for (auto* b : {&c->xtot, &c->xbase}) b->resize(n);
+    for (auto* b : {&c->ytot, &c->ybase, &c->act64}) b->resize(std::max(m, 1));
+
+    if (opts.algorithm == PM_ALG_REFERENCE) {
+        // Canonicalisation and diagonal steps as pdhg_solver.cu:196-250.
+        const float obj_sign = lp->sense == 1 ? -1.0f : 1.0f;
+        std::vector<float> c32(n), v32(nnz), vt32(nnz), sig(m, 0.95f), tau(n, 0.95f);
+        for (int j = 0; j < n; ++j) c32[j] = static_cast<float>(static_cast<double>(obj_sign) * lp->c[j]);
+        for (int k = 0; k < nnz; ++k) v32[k] = static_cast<float>(lp->values[k]);
+        for (int k = 0; k < nnz; ++k) vt32[k] = static_cast<float>(vt64[k]);
+        std::vector<double> rn(m, 0.0), cn(n, 0.0);
+        for (int i = 0; i < m; ++i)
+            for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) {
+                const double av = std::abs(static_cast<double>(v32[k]));
+                rn[i] += av;
+                cn[lp->col_idx[k]] += av;
+            }
+        const float alpha = 0.95f;
+        for (int i = 0; i < m; ++i) if (rn[i] > 1e-12) sig[i] = alpha / static_cast<float>(rn[i]);
+        for (int j = 0; j < n; ++j) if (cn[j] > 1e-12) tau[j] = alpha / static_cast<float>(cn[j]);
+        c->c_inf32 = 1.0;
+        for (int j = 0; j < n; ++j) c->c_inf32 = std::max(c->c_inf32, static_cast<double>(std::abs(c32[j])));
+        c->va.upload(v32, s);
+        c->vat.upload(vt32, s);
+        if (!c->exact_f32) {
+            c->oa64.upload(lp->values, nnz, s);
+            c->oat64.upload(vt64, s);
+        }
+        up64(c->d_dr, std::vector<double>(std::max(m, 1), 1.0));
+        up64(c->d_dc, std::vector<double>(n, 1.0));
+        c->c32.upload(c32, s);
+        c->sig32.upload(sig, s);
+        c->tau32.upload(tau, s);
+        c->lr32.upload(lr32, s);
+        c->ur32.upload(ur32, s);
+        c->lx32.upload(lx32, s);
+        c->ux32.upload(ux32, s);
+    } else {
+        // Ruiz equilibration of A (inf-norm, opts.ruiz_iterations passes)
+        // followed by Pock-Chambolle alpha = 1 (1-norms).
+        std::vector<double> dr(m, 1.0), dc(n, 1.0), vs(lp->values, lp->values + nnz);
+        std::vector<double> rnorm(m), cnorm(n);
+        for (int pass = 0; pass < opts.ruiz_iterations; ++pass) {
+            std::fill(rnorm.begin(), rnorm.end(), 0.0);
+            std::fill(cnorm.begin(), cnorm.end(), 0.0);
+            for (int i = 0; i < m; ++i)
+                for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) {
+                    const double a = std::abs(vs[k]);
+                    rnorm[i] = std::max(rnorm[i], a);
+                    cnorm[lp->col_idx[k]] = std::max(cnorm[lp->col_idx[k]], a);
+                }
+            for (int i = 0; i < m; ++i) rnorm[i] = rnorm[i] > 0.0 ? 1.0 / std::sqrt(rnorm[i]) : 1.0;
+            for (int j = 0; j < n; ++j) cnorm[j] = cnorm[j] > 0.0 ? 1.0 / std::sqrt(cnorm[j]) : 1.0;
+            for (int i = 0; i < m; ++i)
+                for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) vs[k] *= rnorm[i] * cnorm[lp->col_idx[k]];
+            for (int i = 0; i < m; ++i) dr[i] *= rnorm[i];
+            for (int j = 0; j < n; ++j) dc[j] *= cnorm[j];
+        }
+        std::fill(rnorm.begin(), rnorm.end(), 0.0);
+        std::fill(cnorm.begin(), cnorm.end(), 0.0);
+        for (int i = 0; i < m; ++i)
+            for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) {
+                const double a = std::abs(vs[k]);
+                rnorm[i] += a;
+                cnorm[lp->col_idx[k]] += a;
+            }
+        for (int i = 0; i < m; ++i) rnorm[i] = rnorm[i] > 0.0 ? 1.0 / std::sqrt(rnorm[i]) : 1.0;
+        for (int j = 0; j < n; ++j) cnorm[j] = cnorm[j] > 0.0 ? 1.0 / std::sqrt(cnorm[j]) : 1.0;
+        for (int i = 0; i < m; ++i)
+            for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) vs[k] *= rnorm[i] * cnorm[lp->col_idx[k]];
+        for (int i = 0; i < m; ++i) dr[i] *= rnorm[i];
+        for (int j = 0; j < n; ++j) dc[j] *= cnorm[j];
+
+        std::vector<float> v32(nnz), vt32(nnz);
+        for (int k = 0; k < nnz; ++k) v32[k] = static_cast<float>(vs[k]);
+        {
+            std::vector<int> cp2, ri2;
+            std::vector<float> vt;
+            host_transpose<float>(m, n, lp->row_ptr, lp->col_idx, v32.data(), cp2, ri2, vt, nullptr);
+            vt32.swap(vt);
+        }
+        c->va.upload(v32, s);
+        c->vat.upload(vt32, s);
+        if (c->exact_f32) {
+            std::vector<float> o(nnz), ot(nnz);
+            for (int k = 0; k < nnz; ++k) o[k] = static_cast<float>(lp->values[k]);
+            for (int k = 0; k < nnz; ++k) ot[k] = static_cast<float>(vt64[k]);
+            c->oa32.upload(o, s);
+            c->oat32.upload(ot, s);
+        } else {
+            c->oa64.upload(lp->values, nnz, s);
+            c->oat64.upload(vt64, s);
+        }
+        up64(c->d_dr, dr);
+        up64(c->d_dc, dc);
+        for (auto* b : {&c->c32, &c->lx32, &c->ux32}) b->resize(n);
+        for (auto* b : {&c->lr32, &c->ur32, &c->g32}) b->resize(std::max(m, 1));
+
+        // Initial primal weight ||c_hat||_2 / ||b_hat||_2 (PDLP).
+        double cn2 = 0.0, bn2 = 0.0;
+        for (int j = 0; j < n; ++j) cn2 += (dc[j] * c->cmin[j]) * (dc[j] * c->cmin[j]);
+        for (int i = 0; i < m; ++i) {
+            double bh = 0.0;
+            if (c->lr[i] > -kInf) bh = std::max(bh, std::abs(c->lr[i]));
+            if (c->ur[i] < kInf) bh = std::max(bh, std::abs(c->ur[i]));
+            bn2 += (dr[i] * bh) * (dr[i] * bh);
+        }
+        c->omega0 = (cn2 > 1e-20 && bn2 > 1e-20) ? std::sqrt(cn2) / std::sqrt(bn2) : 1.0;
+        if (opts.primal_weight > 0.0) c->omega0 = opts.primal_weight;
+    }
+
+    // SpMV variant per orientation.
+    {
+        float* xs = c->x.get();
+        launch_elem(n, FnFill<float>{xs, 1.0f}, s);
+        launch_elem(m, FnFill<float>{c->y.get(), 1.0f}, s);
+        c->var_a = autotune(c, m, n, nnz, c->rp.get(), c->ci.get(), c->va.get(), xs, c->tmp_m.get());
+        c->var_at = autotune(c, n, m, nnz, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), c->tmp_n.get());
+    }
+
+    if (opts.algorithm == PM_ALG_PDLP && m > 0 && nnz > 0) {
+        // Power iteration for ||A_hat||_2 on A^T A.
+        float* v = c->x.get();
+        float* w = c->tmp_n.get();
+        float* u = c->tmp_m.get();
+        double* ss = c->red.get();
+        launch_elem(n, FnFill<float>{v, 1.0f / std::sqrt(static_cast<float>(n))}, s);
+        for (int it = 0; it < 64; ++it) {
+            launch_rows<float>(c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), v, EpiStore<float>{u}, s);
+            launch_rows<float>(c->var_at, n, c->cp.get(), c->ri.get(), c->vat.get(), u, EpiStore<float>{w}, s);
+            launch_elem_reduce(n, FnNormSq{w}, c->partials.get(), ss, s);
+            launch_elem(n, FnNormalize{w, ss, v}, s);
+        }
+        PM_HIP(hipMemcpyAsync(c->hred.get(), ss, sizeof(double), hipMemcpyDeviceToHost, s));
+        sync(c);
+        const double lam = c->hred[0];  // ||A^T A v|| with ||v|| = 1 -> sigma_max^2
+        c->sigma_max = lam > 0.0 ? std::sqrt(std::sqrt(lam)) : 1.0;
+        c->eta = 0.99 / c->sigma_max;
+    }
+    sync(c);
+    c->loaded = true;
+}
+
+// ---------------------------------------------------------------------------
+// Reference solve
+// ---------------------------------------------------------------------------
+void ref_residuals(pm_ctx* c, double tol, double& pv, double& dv) {
+    hipStream_t s = c->stream;
+    double* red = c->red.get();
+    const int m = c->m, n = c->n;
+    const int vrow = c->opts.spmv_variant == PM_SPMV_SEQUENTIAL ? PM_SPMV_SEQUENTIAL : c->var_a;
+    const int vcol = c->opts.spmv_variant == PM_SPMV_SEQUENTIAL ? PM_SPMV_SEQUENTIAL : c->var_at;
+    EpiRefRowViol er{c->d_lr.get(), c->d_ur.get()};
+    if (c->exact_f32)
+        launch_rows_reduce<double>(vrow, m, c->rp.get(), c->ci.get(), c->va.get(), c->x.get(), er,
+                                   c->partials.get(), red, s);
+    else
+        launch_rows_reduce<double>(vrow, m, c->rp.get(), c->ci.get(), c->oa64.get(), c->x.get(), er,
+                                   c->partials.get(), red, s);
+    EpiRefColDual<float> ec{c->x.get(), c->c32.get(), c->d_lx.get(), c->d_ux.get(), tol};
+    launch_rows_reduce<double>(vcol, n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ec,
+                               c->partials.get(), red + 1, s);
+    PM_HIP(hipMemcpyAsync(c->hred.get(), red, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    pv = std::max(c->hred[0], c->hred[1]);
+    dv = c->hred[2] / c->c_inf32;
+}
+
+void ref_iteration(pm_ctx* c) {
+    hipStream_t s = c->stream;
+    const int va = c->var_a, vt = c->var_at;
+    if (c->m > 0)
+        launch_rows<float>(va, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(),
+                           EpiDualRef{c->sig32.get(), c->lr32.get(), c->ur32.get(), c->y.get()}, s);
+    launch_rows<float>(vt, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(),
+                       EpiPrimalRef{c->tau32.get(), c->c32.get(), c->lx32.get(), c->ux32.get(), c->x.get(),
+                                    c->xe.get()},
+                       s);
+}
+
+void finish_result(pm_ctx* c, pm_pdhg_result* res, const double* xd_dev, const double* yd_dev,
+                   std::vector<double>& xh, std::vector<double>& yh);
+
+void ref_solve(pm_ctx* c, pm_pdhg_result* res) {
+    hipStream_t s = c->stream;
+    const int m = c->m, n = c->n;
+    const auto& o = c->opts;
+    if (c->var_a == PM_SPMV_SEQUENTIAL || o.spmv_variant == PM_SPMV_SEQUENTIAL) {
+        c->var_a = PM_SPMV_SEQUENTIAL;
+        c->var_at = PM_SPMV_SEQUENTIAL;
+    }
+    // x0 = clamp(0, l, u), x-bar = x0, y = 0 (pdhg_solver.cu:288-297).
+    launch_elem(n, FnInitX{c->lx32.get(), c->ux32.get(), c->x.get(), c->xe.get(), nullptr}, s);
+    launch_elem(m, FnFill<float>{c->y.get(), 0.0f}, s);
+    double best = INFINITY;
+    int stagnation = 0, restarts = 0;
+    const int kRestP = 8, kMaxR = 10;
+    res->status = PM_SOLVE_ITERATION_LIMIT;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int iter = 0; iter < o.max_iterations; ++iter) {
+        ref_iteration(c);
+        if ((iter + 1) % o.check_frequency == 0) {
+            double pv = 0, dv = 0;
+            ref_residuals(c, o.tolerance, pv, dv);
+            res->primal_residual = pv;
+            res->dual_residual = dv;
+            if (o.verbose)
+                fprintf(stderr, "[pramaan] iter %d pv %.3e dv %.3e restarts %d\n", iter + 1, pv, dv, restarts);
+            if (pv <= o.tolerance && dv <= o.tolerance) {
+                res->status = PM_SOLVE_OPTIMAL;
+                res->iterations = iter + 1;
+                break;
+            }
+            const double combined = pv + dv;
+            if (combined > o.tolerance * 10.0) {
+                if (combined < best * 0.99) { best = combined; stagnation = 0; }
+                else ++stagnation;
+                if (stagnation >= kRestP && restarts < kMaxR) {
+                    launch_elem(m, FnFill<float>{c->y.get(), 0.0f}, s);
+                    launch_elem(n, FnCopy<float>{c->xe.get(), c->x.get()}, s);
+                    stagnation = 0;
+                    best = INFINITY;
+                    ++restarts;
+                }
+            }
+        }
+    }
+    sync(c);
+    res->device_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (res->status != PM_SOLVE_OPTIMAL) res->iterations = o.max_iterations;
+    double pv = 0, dv = 0;
+    ref_residuals(c, o.tolerance, pv, dv);
+    res->primal_residual = pv;
+    res->dual_residual = dv;
+    res->restarts = restarts;
+    // FP64 copies of the FP32 iterates for the KKT evaluation and outputs.
+    std::vector<float> xf(n), yf(m);
+    c->x.download(xf.data(), n, s);
+    if (m > 0) c->y.download(yf.data(), m, s);
+    sync(c);
+    std::vector<double> xh(n), yh(m);
+    for (int j = 0; j < n; ++j) xh[j] = static_cast<double>(xf[j]);
+    for (int i = 0; i < m; ++i) yh[i] = static_cast<double>(yf[i]);
+    c->xtot.upload(xh, s);
+    if (m > 0) c->ytot.upload(yh, s);
+    finish_result(c, res, c->xtot.get(), c->ytot.get(), xh, yh);
+}
+
+// Relative KKT of (xtot, ytot) on the device.  out: see EpiKktRow/Col.
+struct Kkt {
+    double rel, rel_p, rel_d, rel_g, p, d, rp_inf, rd_inf, ps, ds;
+};
+
+Kkt eval_kkt(pm_ctx* c, const double* xd, const double* yd) {
+    hipStream_t s = c->stream;
+    double* red = c->red.get();
+    const int m = c->m, n = c->n;
+    const int vrow = c->opts.spmv_variant == PM_SPMV_SEQUENTIAL ? PM_SPMV_SEQUENTIAL : c->var_a;
+    const int vcol = c->opts.spmv_variant == PM_SPMV_SEQUENTIAL ? PM_SPMV_SEQUENTIAL : c->var_at;
+    const double* dr = c->d_dr.get();   // unit scales in reference mode
+    const double* dcp = c->d_dc.get();
+    EpiKktRow er{c->d_lr.get(), c->d_ur.get(), yd, dr};
+    EpiKktCol ec{c->d_cmin.get(), c->d_lx.get(), c->d_ux.get(), xd, dcp};
+    const bool ref = c->opts.algorithm == PM_ALG_REFERENCE;
+    if (ref && c->exact_f32) {
+        launch_rows_reduce<double>(vrow, m, c->rp.get(), c->ci.get(), c->va.get(), xd, er, c->partials.get(), red, s);
+        launch_rows_reduce<double>(vcol, n, c->cp.get(), c->ri.get(), c->vat.get(), yd, ec, c->partials.get(),
+                                   red + 8, s);
+    } else if (ref) {
+        launch_rows_reduce<double>(vrow, m, c->rp.get(), c->ci.get(), c->oa64.get(), xd, er, c->partials.get(), red,
+                                   s);
+        launch_rows_reduce<double>(vcol, n, c->cp.get(), c->ri.get(), c->oat64.get(), yd, ec, c->partials.get(),
+                                   red + 8, s);
+    } else if (c->exact_f32) {
+        launch_rows_reduce<double>(vrow, m, c->rp.get(), c->ci.get(), c->oa32.get(), xd, er, c->partials.get(), red,
+                                   s);
+        launch_rows_reduce<double>(vcol, n, c->cp.get(), c->ri.get(), c->oat32.get(), yd, ec, c->partials.get(),
+                                   red + 8, s);
+    } else {
+        launch_rows_reduce<double>(vrow, m, c->rp.get(), c->ci.get(), c->oa64.get(), xd, er, c->partials.get(), red,
+                                   s);
+        launch_rows_reduce<double>(vcol, n, c->cp.get(), c->ri.get(), c->oat64.get(), yd, ec, c->partials.get(),
+                                   red + 8, s);
+    }
+    PM_HIP(hipMemcpyAsync(c->hred.get(), red, 14 * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    const double* h = c->hred.get();
+    // row: 0 r^2, 1 (Dr r)^2, 2 bhat^2, 3 e^2, 4 (e/Dr)^2, 5 d_row, 6 |r|max
+    // col: 8 e^2, 9 (Dc e)^2, 10 c^2, 11 d_col, 12 c^T x, 13 |e|max
+    Kkt k{};
+    k.p = h[12];
+    k.d = h[5] + h[11];
+    k.rel_p = std::sqrt(h[0]) / (1.0 + std::sqrt(h[2]));
+    k.rel_d = std::sqrt(h[3] + h[8]) / (1.0 + std::sqrt(h[10]));
+    k.rel_g = std::fabs(k.p - k.d) / (1.0 + std::fabs(k.p) + std::fabs(k.d));
+    k.rel = std::max(k.rel_p, std::max(k.rel_d, k.rel_g));
+    if (!std::isfinite(k.rel)) k.rel = INFINITY;
+    k.rp_inf = h[6];
+    k.rd_inf = h[13];
+    k.ps = std::sqrt(h[1]);
+    k.ds = std::sqrt(h[4] + h[9]);
+    return k;
+}
+
+// Reference-style pv/dv (pdhg_solver.cu:107-161) of FP64 iterates.
+void ref_style_residuals(pm_ctx* c, const double* xd, const double* yd, double tol, double& pv, double& dv) {
+    hipStream_t s = c->stream;
+    double* red = c->red.get() + 32;
+    const int m = c->m, n = c->n;
+    EpiRefRowViol er{c->d_lr.get(), c->d_ur.get()};
+    if (c->exact_f32)
+        launch_rows_reduce<double>(c->var_a, m, c->rp.get(), c->ci.get(),
+                                   c->oa32.get() ? c->oa32.get() : c->va.get(), xd, er, c->partials.get(), red, s);
+    else
+        launch_rows_reduce<double>(c->var_a, m, c->rp.get(), c->ci.get(), c->oa64.get(), xd, er, c->partials.get(),
+                                   red, s);
+    // c in FP32 as the reference keeps it.
+    std::vector<float> c32(n);
+    for (int j = 0; j < n; ++j) c32[j] = static_cast<float>(c->cmin[j]);
+    c->tmp_n.upload(c32, s);
+    EpiRefColDual<double> ec{xd, c->tmp_n.get(), c->d_lx.get(), c->d_ux.get(), tol};
+    if (c->exact_f32)
+        launch_rows_reduce<double>(c->var_at, n, c->cp.get(), c->ri.get(),
+                                   c->oat32.get() ? c->oat32.get() : c->vat.get(), yd, ec, c->partials.get(), red + 1,
+                                   s);
+    else
+        launch_rows_reduce<double>(c->var_at, n, c->cp.get(), c->ri.get(), c->oat64.get(), yd, ec,
+                                   c->partials.get(), red + 1, s);
+    PM_HIP(hipMemcpyAsync(c->hred.get() + 32, red, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    double cinf = 1.0;
+    for (int j = 0; j < n; ++j) cinf = std::max(cinf, static_cast<double>(std::abs(c32[j])));
+    pv = std::max(c->hred[32], c->hred[33]);
+    dv = c->hred[34] / cinf;
+}
+
+void finish_result(pm_ctx* c, pm_pdhg_result* res, const double* xd_dev, const double* yd_dev,
+                   std::vector<double>& xh, std::vector<double>& yh) {
+    const int m = c->m, n = c->n;
+    hipStream_t s = c->stream;
+    // Objective in the model's sense with the offset, from x (pdhg_solver.cu:402-408).
+    double obj = c->offset;
+    for (int j = 0; j < n; ++j) obj += c->c_model[j] * xh[j];
+    res->objective = obj;
+    {
+        const Kkt k = eval_kkt(c, xd_dev, yd_dev);
+        res->rel_kkt = k.rel;
+        res->rel_primal = k.rel_p;
+        res->rel_dual = k.rel_d;
+        res->rel_gap = k.rel_g;
+        res->primal_objective = k.p;
+        res->dual_objective = k.d;
+    }
+    if (res->x) std::copy(xh.begin(), xh.end(), res->x);
+    if (res->y && m > 0) std::copy(yh.begin(), yh.end(), res->y);
+    if (res->row_activity && m > 0) {
+        double* act = c->act64.get();
+        if (c->exact_f32) {
+            const float* v = (c->opts.algorithm == PM_ALG_PDLP) ? c->oa32.get() : c->va.get();
+            launch_rows<double>(c->var_a, m, c->rp.get(), c->ci.get(), v, xd_dev, EpiStore<double>{act}, s);
+        } else {
+            launch_rows<double>(c->var_a, m, c->rp.get(), c->ci.get(), c->oa64.get(), xd_dev, EpiStore<double>{act},
+                                s);
+        }
+        c->act64.download(res->row_activity, m, s);
+        sync(c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PDLP (reflected Halpern) solve
+// ---------------------------------------------------------------------------
+void round_data(pm_ctx* c, double dp, double dd) {
+    hipStream_t s = c->stream;
+    const int m = c->m, n = c->n;
+    EpiRoundCol ec{c->d_cmin.get(), c->d_lx.get(), c->d_ux.get(), c->xbase.get(), c->d_dc.get(), dp, dd,
+                   c->c32.get(), c->lx32.get(), c->ux32.get()};
+    if (c->exact_f32)
+        launch_rows<double>(c->var_at, n, c->cp.get(), c->ri.get(), c->oat32.get(), c->ybase.get(), ec, s);
+    else
+        launch_rows<double>(c->var_at, n, c->cp.get(), c->ri.get(), c->oat64.get(), c->ybase.get(), ec, s);
+    if (m > 0) {
+        EpiRoundRow er{c->d_lr.get(), c->d_ur.get(), c->ybase.get(), c->d_dr.get(), dp, dd,
+                       c->lr32.get(), c->ur32.get(), c->g32.get()};
+        if (c->exact_f32)
+            launch_rows<double>(c->var_a, m, c->rp.get(), c->ci.get(), c->oa32.get(), c->xbase.get(), er, s);
+        else
+            launch_rows<double>(c->var_a, m, c->rp.get(), c->ci.get(), c->oa64.get(), c->xbase.get(), er, s);
+    }
+    // Iterates restart at x = clamp(0, l_hat, u_hat), y = 0.
+    launch_elem(n, FnInitX{c->lx32.get(), c->ux32.get(), c->x.get(), c->x0.get(), c->xl.get()}, s);
+    launch_elem(m, FnFill<float>{c->y.get(), 0.0f}, s);
+    launch_elem(m, FnFill<float>{c->y0.get(), 0.0f}, s);
+    launch_elem(m, FnFill<float>{c->yl.get(), 0.0f}, s);
+}
+
+void hal_iteration(pm_ctx* c, int k, double omega, bool keep_t, bool use_g) {
+    hipStream_t s = c->stream;
+    const float tau = static_cast<float>(c->eta / omega);
+    const float sig = static_cast<float>(c->eta * omega);
+    const float lam = static_cast<float>((k + 1.0) / (k + 2.0));
+    const float rho = 1.0f;
+    EpiPrimalHal ep{tau, lam, rho, c->c32.get(), c->lx32.get(), c->ux32.get(), c->x0.get(), c->x.get(), c->xe.get(),
+                    keep_t ? c->xt.get() : nullptr};
+    launch_rows<float>(c->var_at, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep, s);
+    if (c->m > 0) {
+        EpiDualHal ed{sig, lam, rho, c->lr32.get(), c->ur32.get(), use_g ? c->g32.get() : nullptr, c->y0.get(),
+                      c->y.get(), keep_t ? c->yt.get() : nullptr};
+        launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed, s);
+    }
+}
+
+void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
+    hipStream_t s = c->stream;
+    const int m = c->m, n = c->n;
+    const auto& o = c->opts;
+    const int K = std::max(1, o.check_frequency);
+    const double tol = o.tolerance;
+
+    // Round 0 base point: the warm start, or zero.
+    if (!c->x0_h.empty()) {
+        std::vector<double> xb(n);
+        for (int j = 0; j < n; ++j) xb[j] = std::min(std::max(c->x0_h[j], c->lx[j]), c->ux[j]);
+        c->xbase.upload(xb, s);
+    } else {
+        launch_elem(n, FnFill<double>{c->xbase.get(), 0.0}, s);
+        // zero is projected by the round data (bounds shift), so x_tot = clamp(0).
+    }
+    if (!c->y0_h.empty() && m > 0) c->ybase.upload(c->y0_h, s);
+    else launch_elem(std::max(m, 1), FnFill<double>{c->ybase.get(), 0.0}, s);
+    const bool warm = !c->x0_h.empty() || !c->y0_h.empty();
+
+    double DP = 1.0, DD = 1.0;
+    double omega = c->omega0;
+    int total_it = 0, restarts = 0, rounds = 0;
+    res->status = PM_SOLVE_ITERATION_LIMIT;
+    const auto t0 = std::chrono::steady_clock::now();
+    Kkt last{};
+    last.rel = INFINITY;
+    bool have_last = false;
+    bool done = false;
+    const double b_suff = 0.2, b_nec = 0.8, b_art = 0.36, theta = 0.5, round_gain = 1e-3;
+    const int stall_n = 20;
+    bool use_g = warm && !c->y0_h.empty();
+
+    while (!done) {
+        round_data(c, DP, DD);
+        int k = 0;
+        double k_round_start = -1.0, k_last = -1.0, k_prev = INFINITY;
+        int it_since = 0;
+        std::vector<double> hist;
+        while (true) {
+            const int chunk = std::min(K, o.max_iterations - total_it);
+            if (chunk <= 0) { done = true; break; }
+            for (int i = 0; i < chunk; ++i) hal_iteration(c, k + i, omega, i == chunk - 1, use_g);
+            k += chunk;
+            total_it += chunk;
+            it_since += chunk;
+            // Candidate = T(z) of the last iteration, as a total point in FP64.
+            launch_elem(n, FnTotalX{c->xbase.get(), c->d_dc.get(), c->xt.get(), 1.0 / DP, c->d_lx.get(),
+                                    c->d_ux.get(), c->xtot.get()},
+                        s);
+            if (m > 0)
+                launch_elem(m, FnTotalY{c->ybase.get(), c->d_dr.get(), c->yt.get(), 1.0 / DD, c->d_lr.get(),
+                                        c->d_ur.get(), c->ytot.get()},
+                            s);
+            const Kkt kk = eval_kkt(c, c->xtot.get(), c->ytot.get());
+            last = kk;
+            have_last = true;
+            if (o.verbose)
+                fprintf(stderr, "[pramaan] it %d round %d kkt %.3e (p %.2e d %.2e g %.2e) omega %.3g restarts %d\n",
+                        total_it, rounds, kk.rel, kk.rel_p, kk.rel_d, kk.rel_g, omega, restarts);
+            if (!std::isfinite(kk.rel)) {
+                res->status = PM_SOLVE_NUMERICAL_FAILURE;
+                done = true;
+                break;
+            }
+            if (kk.rel <= tol) {
+                res->status = PM_SOLVE_OPTIMAL;
+                done = true;
+                break;
+            }
+            if (total_it >= o.max_iterations) { done = true; break; }
+            if (k_round_start < 0) k_round_start = kk.rel;
+            hist.push_back(kk.rel);
+            bool stalled = false;
+            if (static_cast<int>(hist.size()) > stall_n) {
+                double recent = INFINITY, older = INFINITY;
+                for (size_t q = 0; q < hist.size(); ++q) {
+                    if (q + stall_n >= hist.size()) recent = std::min(recent, hist[q]);
+                    else older = std::min(older, hist[q]);
+                }
+                stalled = recent > 0.5 * older;
+            }
+            if (rounds < o.max_refinements && (kk.rel <= k_round_start * round_gain || stalled)) {
+                // New refinement round around the candidate.
+                std::swap(c->xbase, c->xtot);
+                std::swap(c->ybase, c->ytot);
+                const double f = std::min(1.0 / std::max(kk.rel, 1e-300), std::max(DP, 1.0) * 1e4);
+                DP = DD = std::max(f, 1.0);
+                use_g = true;
+                ++rounds;
+                break;
+            }
+            if (k_last < 0) k_last = kk.rel;
+            const bool restart = kk.rel <= b_suff * k_last || (kk.rel <= b_nec * k_last && kk.rel > k_prev) ||
+                                 it_since >= b_art * total_it;
+            k_prev = kk.rel;
+            if (restart) {
+                // z <- T(z); anchor z0 <- z; primal weight from the distances moved.
+                double* red = c->red.get();
+                launch_elem_reduce(n, FnNormDiff{c->xt.get(), c->xl.get()}, c->partials.get(), red + 48, s);
+                launch_elem_reduce(m, FnNormDiff{c->yt.get(), c->yl.get()}, c->partials.get(), red + 49, s);
+                PM_HIP(hipMemcpyAsync(c->hred.get() + 48, red + 48, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+                launch_elem(n, FnCopy<float>{c->x.get(), c->xt.get()}, s);
+                launch_elem(n, FnCopy<float>{c->x0.get(), c->xt.get()}, s);
+                launch_elem(n, FnCopy<float>{c->xl.get(), c->xt.get()}, s);
+                launch_elem(m, FnCopy<float>{c->y.get(), c->yt.get()}, s);
+                launch_elem(m, FnCopy<float>{c->y0.get(), c->yt.get()}, s);
+                launch_elem(m, FnCopy<float>{c->yl.get(), c->yt.get()}, s);
+                sync(c);
+                const double dx = std::sqrt(c->hred[48]), dy = std::sqrt(c->hred[49]);
+                if (dx > 1e-10 && dy > 1e-10)
+                    omega = std::exp(theta * std::log(dy / dx) + (1.0 - theta) * std::log(omega));
+                k = 0;
+                k_last = kk.rel;
+                k_prev = INFINITY;
+                it_since = 0;
+                ++restarts;
+            }
+        }
+    }
+    sync(c);
+    res->device_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->iterations = total_it;
+    res->restarts = restarts;
+    res->refinements = rounds;
+    res->primal_weight = omega;
+    if (!have_last) {
+        // No iteration ran (max_iterations == 0): report the base point.
+        launch_elem(n, FnTotalX{c->xbase.get(), c->d_dc.get(), c->x.get(), 0.0, c->d_lx.get(), c->d_ux.get(),
+                                c->xtot.get()},
+                    s);
+        if (m > 0) launch_elem(m, FnCopy<double>{c->ytot.get(), c->ybase.get()}, s);
+    }
+    std::vector<double> xh(n), yh(m);
+    c->xtot.download(xh.data(), n, s);
+    if (m > 0) c->ytot.download(yh.data(), m, s);
+    sync(c);
+    double pv = 0, dv = 0;
+    ref_style_residuals(c, c->xtot.get(), c->ytot.get(), tol, pv, dv);
+    res->primal_residual = pv;
+    res->dual_residual = dv;
+    finish_result(c, res, c->xtot.get(), c->ytot.get(), xh, yh);
+}
+
+void ctx_solve(pm_ctx* c, pm_pdhg_result* res) {
+    if (!res) fail(PM_ERR_INVALID_ARG, "result is NULL");
+    if (!c->loaded) fail(PM_ERR_INVALID_ARG, "no LP loaded");
+    res->status = PM_SOLVE_ITERATION_LIMIT;
+    res->iterations = 0;
+    res->restarts = 0;
+    res->refinements = 0;
+    res->objective = 0.0;
+    res->primal_residual = INFINITY;
+    res->dual_residual = INFINITY;
+    res->rel_kkt = res->rel_primal = res->rel_dual = res->rel_gap = INFINITY;
+    res->primal_objective = res->dual_objective = 0.0;
+    res->primal_weight = 0.0;
+    res->device_seconds = 0.0;
+    const int m = c->m, n = c->n;
+    if (n == 0) {  // pdhg_solver.cu:178-184
+        res->status = PM_SOLVE_OPTIMAL;
+        res->objective = c->offset;
+        if (res->row_activity) std::fill(res->row_activity, res->row_activity + m, 0.0);
+        if (res->y) std::fill(res->y, res->y + m, 0.0);
+        return;
+    }
+    if (c->nan_cost) {  // pdhg_solver.cu:186-192
+        res->status = PM_SOLVE_NUMERICAL_FAILURE;
+        return;
+    }
+    if (c->opts.algorithm == PM_ALG_REFERENCE) ref_solve(c, res);
+    else pdlp_solve(c, res);
+}
+
+}  // namespace
+}  // namespace pm
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* pm_version(void) { return "pramaan-mi355x 0.1 (gfx950)"; }
+
+const char* pm_last_error(void) { return pm::g_last_error.c_str(); }
+
+int pm_device_count(int32_t* count) {
+    if (!count) return PM_ERR_INVALID_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    (void)hipGetLastError();
+    *count = c;
+    return PM_OK;
+}
+
+void pm_pdhg_options_default(pm_pdhg_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->tolerance = 1e-6;
+    o->max_iterations = 100000;
+    o->check_frequency = 100;
+    o->algorithm = PM_ALG_REFERENCE;
+    o->termination = PM_TERM_REFERENCE;
+    o->spmv_variant = PM_SPMV_AUTO;
+    o->ruiz_iterations = 10;
+    o->max_refinements = 8;
+    o->device = 0;
+    o->verbose = 0;
+    o->primal_weight = 0.0;
+    o->x0 = nullptr;
+    o->y0 = nullptr;
+}
+
+int pm_ctx_create(int32_t device, pm_ctx** out) {
+    return pm::guarded([&] {
+        if (!out) pm::fail(PM_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            (void)hipGetLastError();
+            pm::fail(PM_ERR_NO_DEVICE, "no HIP device visible");
+        }
+        if (device < 0 || device >= count) pm::fail(PM_ERR_NO_DEVICE, "device ordinal out of range");
+        PM_HIP(hipSetDevice(device));
+        auto c = std::make_unique<pm_ctx>();
+        c->device = device;
+        PM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        PM_HIP(hipEventCreate(&c->ev0));
+        PM_HIP(hipEventCreate(&c->ev1));
+        *out = c.release();
+    });
+}
+
+int pm_ctx_destroy(pm_ctx* c) {
+    return pm::guarded([&] {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
+        hipStream_t s = c->stream;
+        delete c;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int pm_ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* opts) {
+    return pm::guarded([&] {
+        if (!c) pm::fail(PM_ERR_INVALID_ARG, "ctx is NULL");
+        PM_HIP(hipSetDevice(c->device));
+        pm::ctx_load(c, lp, opts);
+    });
+}
+
+int pm_ctx_solve(pm_ctx* c, pm_pdhg_result* res) {
+    return pm::guarded([&] {
+        if (!c) pm::fail(PM_ERR_INVALID_ARG, "ctx is NULL");
+        PM_HIP(hipSetDevice(c->device));
+        pm::ctx_solve(c, res);
+    });
+}
+
+int pm_pdhg_solve(const pm_lp* lp, const pm_pdhg_options* opts, pm_pdhg_result* res) {
+    pm_ctx* c = nullptr;
+    const int dev = opts ? opts->device : 0;
+    int rc = pm_ctx_create(dev, &c);
+    if (rc != PM_OK) return rc;
+    rc = pm_ctx_load(c, lp, opts);
+    if (rc == PM_OK) rc = pm_ctx_solve(c, res);
+    const std::string err = pm::g_last_error;
+    pm_ctx_destroy(c);
+    if (rc != PM_OK) pm::set_last_error(err);
+    return rc;
+}
+
+int pm_ctx_spmv_variants(pm_ctx* c, int32_t* va, int32_t* vat) {
+    return pm::guarded([&] {
+        if (!c || !va || !vat) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
+        *va = c->var_a;
+        *vat = c->var_at;
+    });
+}
+
+int pm_ctx_time_kernel(pm_ctx* c, int32_t which, int32_t reps, double* avg_ms, double* alg_bytes) {
+    return pm::guarded([&] {
+        using namespace pm;
+        if (!c || !avg_ms || !alg_bytes) fail(PM_ERR_INVALID_ARG, "NULL argument");
+        if (!c->loaded || c->n == 0 || c->nan_cost) fail(PM_ERR_INVALID_ARG, "no LP loaded");
+        if (reps <= 0) fail(PM_ERR_INVALID_ARG, "reps must be positive");
+        PM_HIP(hipSetDevice(c->device));
+        const double m = c->m, n = c->n, nnz = c->nnz;
+        const double bytes_a = 8.0 * nnz + 4.0 * (m + 1) + 4.0 * n + 20.0 * m;
+        const double bytes_at = 8.0 * nnz + 4.0 * (n + 1) + 4.0 * m + 28.0 * n;
+        const bool pdlp = c->opts.algorithm == PM_ALG_PDLP;
+        const double omega = c->omega0;
+        auto launch = [&](int w) {
+            if (w == PM_KERNEL_SPMV_A || w == PM_KERNEL_ITERATION) {
+                if (pdlp) {
+                    EpiDualHal ed{static_cast<float>(c->eta * omega), 0.5f, 1.0f, c->lr32.get(), c->ur32.get(),
+                                  nullptr, c->y0.get(), c->y.get(), nullptr};
+                    launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed,
+                                       c->stream);
+                } else {
+                    launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(),
+                                       EpiDualRef{c->sig32.get(), c->lr32.get(), c->ur32.get(), c->y.get()}, c->stream);
+                }
+            }
+            if (w == PM_KERNEL_SPMV_AT || w == PM_KERNEL_ITERATION) {
+                if (pdlp) {
+                    EpiPrimalHal ep{static_cast<float>(c->eta / omega), 0.5f, 1.0f, c->c32.get(), c->lx32.get(),
+                                    c->ux32.get(), c->x0.get(), c->x.get(), c->xe.get(), nullptr};
+                    launch_rows<float>(c->var_at, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep,
+                                       c->stream);
+                } else {
+                    launch_rows<float>(c->var_at, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(),
+                                       EpiPrimalRef{c->tau32.get(), c->c32.get(), c->lx32.get(), c->ux32.get(),
+                                                    c->x.get(), c->xe.get()},
+                                       c->stream);
+                }
+            }
+        };
+        if (which < 0 || which > 2) fail(PM_ERR_INVALID_ARG, "unknown kernel id");
+        // Inputs of the timed kernels must be finite: give the working
+        // vectors a neutral state when no solve ran yet.
+        launch(which);
+        launch(which);
+        PM_HIP(hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < reps; ++r) launch(which);
+        PM_HIP(hipEventRecord(c->ev1, c->stream));
+        PM_HIP(hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *avg_ms = ms / reps;
+        *alg_bytes = which == PM_KERNEL_SPMV_A ? bytes_a : which == PM_KERNEL_SPMV_AT ? bytes_at : bytes_a + bytes_at;
+    });
+}
+
+}  // extern "C"

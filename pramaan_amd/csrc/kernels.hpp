@@ -1,0 +1,404 @@
+// kernels.hpp -- gfx950 device code of the PDHG engine.
+//
+// Layout in HBM (one LP): A as CSR (row_ptr int32[m+1], col int32[nnz],
+// val float[nnz]) and A^T as a second CSR built once at load (col_ptr
+// int32[n+1], row int32[nnz], val float[nnz]), so both products are
+// row-parallel gathers with no atomics.  The reference computes A^T y by a
+// per-row atomicAdd scatter (src/gpu/pdhg_kernels.cuh:42-54); the
+// transpose makes the result deterministic and the stream coalesced.
+//
+// Every SpMV is one templated row kernel: L lanes of a 64-wide wavefront
+// share a row (L in 1..64), walk it with stride L (four loads in flight per
+// lane), reduce with a butterfly of wavefront shuffles and hand the row sum
+// to an epilogue functor.  The PDHG projections, extrapolation and Halpern
+// averaging are those epilogues, so one iteration is two launches that each
+// stream the matrix once.  All kernels are built with -ffp-contract=off:
+// fused multiply-adds are written explicitly (fma), everything else rounds
+// exactly as written, which lets the sequential variant reproduce a CPU
+// loop bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pm {
+
+constexpr int kBlock = 256;
+constexpr float kFltInf = 1e30f;   // FP32 infinity sentinel (pdhg_solver.cu:200)
+constexpr double kInf = 1e30;      // ModelIR kInfinity (ir.hpp:47)
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8).  Remap so each XCD owns one contiguous range of rows: rows
+// that share gathered vector entries then share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    if (nb < 16) return b;
+    const int x = b & 7, q = b >> 3, per = nb >> 3, rem = nb & 7;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
+template <int L, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, L);
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T fma_t(T a, T b, T c);
+template <>
+__device__ __forceinline__ float fma_t<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double fma_t<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Row sum of row `row` by the L lanes of its group (result valid on every
+// lane of the group).  EXACT (L == 1 only): sequential, unfused, in CSR
+// order -- the reference's csr_spmv_kernel loop (pdhg_kernels.cuh:31-35).
+template <int L, bool EXACT, class AT, class VT, class XT>
+__device__ __forceinline__ AT row_dot(int beg, int end, int lane, const int* __restrict__ ci,
+                                      const VT* __restrict__ v, const XT* __restrict__ x) {
+    AT s = AT(0);
+    if constexpr (EXACT) {
+        for (int k = beg; k < end; ++k) s = s + static_cast<AT>(v[k]) * static_cast<AT>(x[ci[k]]);
+    } else {
+        int k = beg + lane;
+        for (; k + 3 * L < end; k += 4 * L) {
+            const int c0 = ci[k], c1 = ci[k + L], c2 = ci[k + 2 * L], c3 = ci[k + 3 * L];
+            const AT v0 = static_cast<AT>(v[k]), v1 = static_cast<AT>(v[k + L]);
+            const AT v2 = static_cast<AT>(v[k + 2 * L]), v3 = static_cast<AT>(v[k + 3 * L]);
+            const AT x0 = static_cast<AT>(x[c0]), x1 = static_cast<AT>(x[c1]);
+            const AT x2 = static_cast<AT>(x[c2]), x3 = static_cast<AT>(x[c3]);
+            s = fma_t<AT>(v0, x0, s);
+            s = fma_t<AT>(v1, x1, s);
+            s = fma_t<AT>(v2, x2, s);
+            s = fma_t<AT>(v3, x3, s);
+        }
+        for (; k < end; k += L) s = fma_t<AT>(static_cast<AT>(v[k]), static_cast<AT>(x[ci[k]]), s);
+        s = group_sum<L>(s);
+    }
+    return s;
+}
+
+// y-side row kernel: epi(row, sum) on lane 0 of each row group.
+template <int L, bool EXACT, class AT, class VT, class XT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_rows(int nrows, const int* __restrict__ rp,
+                                                 const int* __restrict__ ci,
+                                                 const VT* __restrict__ v,
+                                                 const XT* __restrict__ x, Epi epi) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int row = (b * kBlock + static_cast<int>(threadIdx.x)) / L;
+    const int lane = static_cast<int>(threadIdx.x) & (L - 1);
+    if (row >= nrows) return;  // whole row groups leave together
+    const AT s = row_dot<L, EXACT, AT>(rp[row], rp[row + 1], lane, ci, v, x);
+    if (lane == 0) epi(row, s);
+}
+
+// Deterministic block reduction of NS sums followed by NM maxima.  Every
+// thread of the block must call it.  Writes one record per block.
+template <int NS, int NM>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[NS + NM], double* __restrict__ out,
+                                                   int block_slot) {
+    __shared__ double red[kBlock / 64][NS + NM];
+#pragma unroll
+    for (int j = 0; j < NS + NM; ++j) {
+        double a = acc[j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double t = __shfl_xor(a, o, 64);
+            if (j < NS) a += t;
+            else a = (t > a) ? t : a;
+        }
+        acc[j] = a;
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < NS + NM; ++j) red[wave][j] = acc[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < NS + NM) {
+        const int j = threadIdx.x;
+        double a = red[0][j];
+        for (int w = 1; w < kBlock / 64; ++w) {
+            if (j < NS) a += red[w][j];
+            else a = (red[w][j] > a) ? red[w][j] : a;
+        }
+        out[static_cast<size_t>(block_slot) * (NS + NM) + j] = a;
+    }
+}
+
+// Row kernel whose epilogue contributes to block partial sums/maxima.
+// Epi::operator()(row, sum, acc) adds its terms into acc[NS + NM].
+template <int L, bool EXACT, class AT, class VT, class XT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_rows_reduce(int nrows, const int* __restrict__ rp,
+                                                        const int* __restrict__ ci,
+                                                        const VT* __restrict__ v,
+                                                        const XT* __restrict__ x, Epi epi,
+                                                        double* __restrict__ partials) {
+    constexpr int NA = Epi::NS + Epi::NM;
+    double acc[NA];
+#pragma unroll
+    for (int j = 0; j < NA; ++j) acc[j] = 0.0;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int row = (b * kBlock + static_cast<int>(threadIdx.x)) / L;
+    const int lane = static_cast<int>(threadIdx.x) & (L - 1);
+    if (row < nrows) {
+        const AT s = row_dot<L, EXACT, AT>(rp[row], rp[row + 1], lane, ci, v, x);
+        if (lane == 0) epi(row, s, acc);
+    }
+    block_reduce_store<Epi::NS, Epi::NM>(acc, partials, b);
+}
+
+// Elementwise kernel: f(i, acc) for i < n, then the same block reduction.
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_elem_reduce(int n, F f, double* __restrict__ partials) {
+    constexpr int NA = F::NS + F::NM;
+    double acc[NA];
+#pragma unroll
+    for (int j = 0; j < NA; ++j) acc[j] = 0.0;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) f(i, acc);
+    block_reduce_store<F::NS, F::NM>(acc, partials, blockIdx.x);
+}
+
+// Sums (first ns) and maxima (next nm) of `nblocks` records of width
+// ns+nm, in a fixed order: one block, thread j owns blocks j, j+256, ...
+static __global__ __launch_bounds__(kBlock) void k_final_reduce(const double* __restrict__ partials,
+                                                         int nblocks, int ns, int nm,
+                                                         double* __restrict__ out) {
+    __shared__ double red[kBlock];
+    const int w = ns + nm;
+    for (int j = 0; j < w; ++j) {
+        const bool is_sum = j < ns;
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+            const double t = partials[static_cast<size_t>(b) * w + j];
+            if (is_sum) a += t;
+            else a = (t > a) ? t : a;
+        }
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if (static_cast<int>(threadIdx.x) < s) {
+                const double t = red[threadIdx.x + s];
+                if (is_sum) red[threadIdx.x] += t;
+                else red[threadIdx.x] = (t > red[threadIdx.x]) ? t : red[threadIdx.x];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[j] = red[0];
+        __syncthreads();
+    }
+}
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_elem(int n, F f) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) f(i);
+}
+
+// ---------------------------------------------------------------------------
+// Epilogues
+// ---------------------------------------------------------------------------
+
+template <class T>
+struct EpiStore {
+    T* y;
+    template <class AT>
+    __device__ void operator()(int i, AT s) const { y[i] = static_cast<T>(s); }
+};
+
+// Reference dual update fused into A x-bar (dual_update_kernel,
+// pdhg_kernels.cuh:72-89): w = y/sigma + Ax; y = sigma (w - clamp(w,l,u)),
+// rows with sigma <= 0 untouched.
+struct EpiDualRef {
+    const float* sigma;
+    const float* lo;
+    const float* hi;
+    float* y;
+    __device__ void operator()(int i, float ax) const {
+        const float sig = sigma[i];
+        if (sig <= 0.0f) return;
+        const float w = y[i] / sig + ax;
+        const float wp = fminf(fmaxf(w, lo[i]), hi[i]);
+        y[i] = sig * (w - wp);
+    }
+};
+
+// Reference primal update fused into A^T y (primal_update_kernel,
+// pdhg_kernels.cuh:95-109).
+struct EpiPrimalRef {
+    const float* tau;
+    const float* c;
+    const float* lo;
+    const float* hi;
+    float* x;
+    float* xbar;
+    __device__ void operator()(int j, float aty) const {
+        const float x_prev = x[j];
+        const float x_new = fminf(fmaxf(x_prev - tau[j] * (c[j] + aty), lo[j]), hi[j]);
+        xbar[j] = 2.0f * x_new - x_prev;
+        x[j] = x_new;
+    }
+};
+
+// Reflected Halpern PDHG, primal half (fused into A^T y):
+//   xt = clamp(x - tau (c + A^T y), l, u)           (the PDHG step T)
+//   xe = 2 xt - x                                    (extrapolation, feeds A xe)
+//   x  = lam ((1+rho) xt - rho x) + (1-lam) x0       (reflected Halpern step)
+// xt_out (may be null) keeps T(z) for the KKT check.
+struct EpiPrimalHal {
+    float tau, lam, rho;
+    const float* c;
+    const float* lo;
+    const float* hi;
+    const float* x0;
+    float* x;
+    float* xe;
+    float* xt_out;
+    __device__ void operator()(int j, float aty) const {
+        const float xk = x[j];
+        const float xt = fminf(fmaxf(xk - tau * (c[j] + aty), lo[j]), hi[j]);
+        xe[j] = 2.0f * xt - xk;
+        x[j] = lam * ((1.0f + rho) * xt - rho * xk) + (1.0f - lam) * x0[j];
+        if (xt_out) xt_out[j] = xt;
+    }
+};
+
+// Reflected Halpern PDHG, dual half (fused into A xe).  The row prox is
+// that of f(s) = indicator_[l,u](s) + g s: with w = y/sigma + A xe,
+//   yt = sigma (w - clamp(w - g/sigma, l, u)),
+// which for g = 0 is the reference's dual update; g != 0 carries the dual
+// base point of an iterative-refinement round.
+struct EpiDualHal {
+    float sigma, lam, rho;
+    const float* lo;
+    const float* hi;
+    const float* g;  // may be null
+    const float* y0;
+    float* y;
+    float* yt_out;
+    __device__ void operator()(int i, float ax) const {
+        const float yk = y[i];
+        const float w = yk / sigma + ax;
+        const float sh = g ? w - g[i] / sigma : w;
+        const float yt = sigma * (w - fminf(fmaxf(sh, lo[i]), hi[i]));
+        y[i] = lam * ((1.0f + rho) * yt - rho * yk) + (1.0f - lam) * y0[i];
+        if (yt_out) yt_out[i] = yt;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// FP64 checks
+// ---------------------------------------------------------------------------
+
+// Reference primal violation of the rows (ComputeFp64Residuals,
+// pdhg_solver.cu:124-130): max over rows of the bound violation of A x.
+struct EpiRefRowViol {
+    static constexpr int NS = 0, NM = 1;
+    const double* lo;
+    const double* hi;
+    __device__ void operator()(int i, double act, double* acc) const {
+        double pv = 0.0;
+        if (act < lo[i] && lo[i] - act > pv) pv = lo[i] - act;
+        if (act > hi[i] && act - hi[i] > pv) pv = act - hi[i];
+        acc[0] = pv;
+    }
+};
+
+// Reference bound violation of x and projected reduced-cost violation
+// (pdhg_solver.cu:131-157), fused into the FP64 A^T y over the FP32 values.
+// acc[0] = max bound violation, acc[1] = max dual violation (before the
+// division by max(1, ||c||_inf), which the host applies).
+template <class XT>
+struct EpiRefColDual {
+    static constexpr int NS = 0, NM = 2;
+    const XT* x;
+    const float* c32;
+    const double* lo;
+    const double* hi;
+    double tol;
+    __device__ void operator()(int j, double aty, double* acc) const {
+        const double xd = static_cast<double>(x[j]);
+        double bv = 0.0;
+        if (xd < lo[j] && lo[j] - xd > bv) bv = lo[j] - xd;
+        if (xd > hi[j] && xd - hi[j] > bv) bv = xd - hi[j];
+        const double rc = static_cast<double>(c32[j]) + aty;
+        const bool at_lb = xd <= lo[j] + tol;
+        const bool at_ub = xd >= hi[j] - tol;
+        double viol = 0.0;
+        if (!at_lb && !at_ub) viol = fabs(rc);
+        else if (at_lb && !at_ub) viol = (-rc > 0.0) ? -rc : 0.0;
+        else if (at_ub && !at_lb) viol = (rc > 0.0) ? rc : 0.0;
+        acc[0] = bv;
+        acc[1] = (viol > 0.0) ? viol : 0.0;  // NaN never wins a max
+    }
+};
+
+// Relative-KKT row terms of a total iterate (x_tot gathered, y_tot[i]):
+//  sums: [0] |r|^2, [1] |D_r r|^2, [2] bhat^2, [3] row dual-sign residual^2,
+//        [4] its scaled form, [5] row part of the dual objective
+//  max:  [6] |r|
+struct EpiKktRow {
+    static constexpr int NS = 6, NM = 1;
+    const double* lo;
+    const double* hi;
+    const double* y;
+    const double* dr;
+    __device__ void operator()(int i, double act, double* acc) const {
+        const double l = lo[i], u = hi[i];
+        const bool lf = l > -kInf, uf = u < kInf;
+        double p = act;
+        if (lf && p < l) p = l;
+        if (uf && p > u) p = u;
+        const double r = act - p;
+        const double d = dr[i];
+        double bh = 0.0;
+        if (lf && fabs(l) > bh) bh = fabs(l);
+        if (uf && fabs(u) > bh) bh = fabs(u);
+        const double t = -y[i];
+        double tp = t;
+        if (!lf && tp > 0.0) tp = 0.0;
+        if (!uf && tp < 0.0) tp = 0.0;
+        const double e = t - tp;
+        acc[0] = r * r;
+        acc[1] = (d * r) * (d * r);
+        acc[2] = bh * bh;
+        acc[3] = e * e;
+        acc[4] = (e / d) * (e / d);
+        acc[5] = tp > 0.0 ? tp * l : (tp < 0.0 ? tp * u : 0.0);
+        acc[6] = fabs(r);
+    }
+};
+
+// Relative-KKT column terms: rc = c + A^T y_tot, lambda = rc projected on
+// the signs x's finite bounds allow.
+//  sums: [0] |rc - lambda|^2, [1] |D_c (rc - lambda)|^2, [2] c^2,
+//        [3] column part of the dual objective, [4] c^T x
+//  max:  [5] |rc - lambda|
+struct EpiKktCol {
+    static constexpr int NS = 5, NM = 1;
+    const double* c;
+    const double* lo;
+    const double* hi;
+    const double* x;
+    const double* dc;
+    __device__ void operator()(int j, double aty, double* acc) const {
+        const double l = lo[j], u = hi[j];
+        const bool lf = l > -kInf, uf = u < kInf;
+        const double cj = c[j];
+        const double rc = cj + aty;
+        double lam = rc;
+        if (!lf && lam > 0.0) lam = 0.0;
+        if (!uf && lam < 0.0) lam = 0.0;
+        const double e = rc - lam;
+        acc[0] = e * e;
+        acc[1] = (dc[j] * e) * (dc[j] * e);
+        acc[2] = cj * cj;
+        acc[3] = lam > 0.0 ? lam * l : (lam < 0.0 ? lam * u : 0.0);
+        acc[4] = cj * x[j];
+        acc[5] = fabs(e);
+    }
+};
+
+}  // namespace pm
