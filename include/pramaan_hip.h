@@ -203,6 +203,34 @@ int pm_ctx_time_kernel(pm_ctx* ctx, int32_t which, int32_t reps,
 int pm_ctx_spmv_variants(pm_ctx* ctx, int32_t* variant_a,
                          int32_t* variant_at);
 
+/* ---- batched solver ------------------------------------------------------
+ * Many independent LPs solved by one kernel launch: one workgroup per LP
+ * (largest first), each running the PM_ALG_PDLP algorithm (power
+ * iteration, refinement rounds, reflected Halpern iterations, FP64 KKT
+ * checks, restarts) entirely on the device.  The launch is captured once in
+ * a HIP graph and replayed by every pm_batch_run.  Only opts->tolerance,
+ * max_iterations, check_frequency and max_refinements are used; the
+ * algorithm is always PM_ALG_PDLP with PM_TERM_RELATIVE_KKT. */
+typedef struct pm_batch pm_batch;
+typedef struct pm_batch_stats {
+    double device_seconds;      /* wall time of the launch                   */
+    double* instance_seconds;   /* caller-owned [count] or NULL: per-LP solve
+                                   time measured on the device clock         */
+} pm_batch_stats;
+int pm_batch_create(int32_t device, int32_t count, const pm_lp* lps,
+                    const pm_pdhg_options* opts, pm_batch** out);
+/* Warm start of LP i for the following runs (NULL clears it); original
+ * space, y in the reference sign.  Copied at the call. */
+int pm_batch_set_warm_start(pm_batch* b, int32_t i, const double* x0,
+                            const double* y0);
+/* Solves every LP of the batch; results[i].x/.y filled when non-NULL. */
+int pm_batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats);
+int pm_batch_destroy(pm_batch* b);
+/* create + run + destroy; x0/y0 may be NULL or arrays of count pointers. */
+int pm_batch_solve(int32_t count, const pm_lp* lps, const double* const* x0,
+                   const double* const* y0, const pm_pdhg_options* opts,
+                   pm_pdhg_result* results, pm_batch_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,10 @@ class pm_gen_params(C.Structure):
                 ("var_upper", C.c_double), ("seed", C.c_uint64)]
 
 
+class pm_batch_stats(C.Structure):
+    _fields_ = [("device_seconds", C.c_double), ("instance_seconds", _d)]
+
+
 # Every symbol the headers declare, with (restype, argtypes).
 SIGNATURES = {
     "pm_version": (C.c_char_p, []),
@@ -77,6 +81,13 @@ SIGNATURES = {
     "pm_ctx_solve": (C.c_int, [C.c_void_p, C.POINTER(pm_pdhg_result)]),
     "pm_ctx_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _d, _d]),
     "pm_ctx_spmv_variants": (C.c_int, [C.c_void_p, _i, _i]),
+    "pm_batch_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(pm_lp), C.POINTER(pm_pdhg_options),
+                                  C.POINTER(C.c_void_p)]),
+    "pm_batch_set_warm_start": (C.c_int, [C.c_void_p, C.c_int32, _d, _d]),
+    "pm_batch_run": (C.c_int, [C.c_void_p, C.POINTER(pm_pdhg_result), C.POINTER(pm_batch_stats)]),
+    "pm_batch_destroy": (C.c_int, [C.c_void_p]),
+    "pm_batch_solve": (C.c_int, [C.c_int32, C.POINTER(pm_lp), C.POINTER(_d), C.POINTER(_d),
+                                 C.POINTER(pm_pdhg_options), C.POINTER(pm_pdhg_result), C.POINTER(pm_batch_stats)]),
     "pm_gen_params_named": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(pm_gen_params)]),
     "pm_gen_synthetic": (C.c_int, [C.POINTER(pm_gen_params), C.POINTER(C.c_void_p)]),
     "pm_gen_view": (C.c_int, [C.c_void_p, C.POINTER(pm_lp), C.POINTER(_d), C.POINTER(_d), _d]),
@@ -295,6 +306,56 @@ class Context:
     def close(self) -> None:
         if self._h:
             lib().pm_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """pm_batch_*: many LPs resident on the device, solved by one graph launch."""
+
+    def __init__(self, lps: list[LP], device: int = 0, **kw):
+        self.lps = list(lps)
+        o = options(**kw)
+        arr = (pm_lp * len(self.lps))(*[lp.struct() for lp in self.lps])
+        self._h = C.c_void_p()
+        _check(lib().pm_batch_create(device, len(self.lps), arr, C.byref(o), C.byref(self._h)))
+
+    def set_warm_start(self, i: int, x0=None, y0=None) -> None:
+        x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        y0 = None if y0 is None else np.ascontiguousarray(y0, np.float64)
+        _check(lib().pm_batch_set_warm_start(self._h, i, _p(x0, C.c_double), _p(y0, C.c_double)))
+
+    def run(self, want_vectors: bool = False):
+        """Returns (results, device_seconds, per-instance seconds)."""
+        k = len(self.lps)
+        res = (pm_pdhg_result * k)()
+        xs, ys = [], []
+        if want_vectors:
+            for i, lp in enumerate(self.lps):
+                xs.append(np.zeros(lp.n))
+                ys.append(np.zeros(lp.m))
+                res[i].x, res[i].y = _p(xs[-1], C.c_double), _p(ys[-1], C.c_double)
+        inst = np.zeros(k)
+        st = pm_batch_stats(0.0, _p(inst, C.c_double))
+        _check(lib().pm_batch_run(self._h, res, C.byref(st)))
+        out = [_result(res[i], xs[i] if want_vectors else None, ys[i] if want_vectors else None, None)
+               for i in range(k)]
+        return out, st.device_seconds, inst
+
+    def close(self) -> None:
+        if self._h:
+            lib().pm_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __enter__(self):

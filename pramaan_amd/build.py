@@ -27,8 +27,10 @@ ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 ARCH = os.environ.get("PRAMAAN_ARCH", "gfx950")
 
-HIP_SOURCES = ["engine.hip", "launchers.hip"]
-HOST_SOURCES = ["host/gen.cpp", "host/pdhg_solver.cpp"]
+HIP_SOURCES = ["engine.hip", "launchers.hip", "batch.hip"]
+HOST_SOURCES = ["host/gen.cpp"]
+CXX_SOURCES = ["cxx/model_ir.cpp", "cxx/pdhg_solver.cpp", "cxx/backend_hip.cpp", "cxx/test_abi.cpp"]
+CXX_LIB = PKG / "lib" / "libpramaan_cxx.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -74,6 +76,16 @@ def build_library(jobs: int = 8) -> Path:
     return LIB
 
 
+def build_cxx() -> Path:
+    """The C++ interface layer (include/pramaan/...) over the C-ABI."""
+    srcs = [str(CSRC / s) for s in CXX_SOURCES]
+    tmp = CXX_LIB.with_suffix(".so.tmp")
+    _run([CXX, *CXX_FLAGS, "-shared", *srcs, "-o", str(tmp), f"-L{LIB.parent}", "-lpramaan_hip",
+          "-Wl,-rpath,$ORIGIN"], "libpramaan_cxx")
+    os.replace(tmp, CXX_LIB)
+    return CXX_LIB
+
+
 def build_oracle() -> Path:
     _run([CC, "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
           str(ORACLE_DIR / "pdhg_oracle.c"), "-o", str(ORACLE_LIB), "-lm"], "oracle")
@@ -85,12 +97,13 @@ def build_reference_oracle() -> bool:
     mk = ORACLE_DIR / "Makefile"
     if not Path("/root/reference").is_dir() or not mk.exists() or shutil.which("make") is None:
         return False
-    _run(["make", "-s", "-C", str(ORACLE_DIR), "-j8", "ref"], "oracle/_ref")
+    _run(["make", "-s", "-C", str(ORACLE_DIR), "-j8", "ref", "reftests"], "oracle/_ref")
     return True
 
 
 def build_all(jobs: int = 8) -> None:
     build_library(jobs)
+    build_cxx()
     build_oracle()
     build_reference_oracle()
 

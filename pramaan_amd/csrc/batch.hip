@@ -1,0 +1,733 @@
+// batch.hip -- many independent LPs solved by one kernel launch.
+//
+// Layout: every LP's A and A^T (CSR, local indices) and vectors are packed
+// back to back in shared device arrays; a per-LP descriptor holds the
+// offsets.  The kernel runs one 256-thread workgroup per LP, largest LP
+// first (blockIdx -> LP through `order`), so the hardware dispatcher hands
+// the next LP to whichever CU frees up.  Inside a workgroup the LP is solved
+// end to end: power iteration for ||A_hat||_2, initial primal weight,
+// refinement rounds, reflected Halpern iterations (the same epilogue
+// functors as the single-LP kernels, run as block-strided row loops with
+// L lanes per row), FP64 KKT checks with deterministic block reductions,
+// restarts and primal-weight updates.  Decisions come from pm::Control, the
+// same code the single-LP host loop runs.  Nothing is allocated or cleared
+// by the host between runs, so the launch is captured once in a HIP graph.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "control.hpp"
+#include "dispatch.hpp"
+#include "prep.hpp"
+
+namespace pm {
+namespace {
+
+struct BatchDesc {
+    int32_t m, n, nnz;
+    int32_t lanes_a, lanes_at;
+    int32_t warm_x, warm_y;
+    int32_t pad;
+    int64_t rp_off, cp_off, nz_off, row_off, col_off;
+    // results
+    int32_t status, iterations, restarts, refinements;
+    double rel_kkt, rel_p, rel_d, rel_g, pobj, dobj, omega;
+    uint64_t t_start, t_end;
+};
+
+template <class VT>
+struct BatchArrays {
+    const int* rp;
+    const int* ci;
+    const float* va;
+    const VT* oa;
+    const int* cp;
+    const int* ri;
+    const float* vat;
+    const VT* oat;
+    const double *cmin, *lr, *ur, *lx, *ux, *dr, *dc;
+    const double *wx, *wy;  // warm-start base points (original space)
+    float *c32, *lr32, *ur32, *lx32, *ux32, *g32;
+    float *x, *xe, *x0, *xt, *xl, *y, *y0, *yt, *yl;
+    double *xtot, *ytot, *xbase, *ybase;
+    BatchDesc* desc;
+    const int* order;
+    double tol;
+    int max_iterations, check_frequency, max_refinements;
+};
+
+// ---- block helpers -------------------------------------------------------
+
+template <int NS, int NM>
+__device__ void block_reduce_all(double (&acc)[NS + NM], double* out) {
+    __shared__ double tmp[kBlock / 64][16];
+#pragma unroll
+    for (int j = 0; j < NS + NM; ++j) {
+        double a = acc[j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double t = __shfl_xor(a, o, 64);
+            if (j < NS) a += t;
+            else a = (t > a) ? t : a;
+        }
+        acc[j] = a;
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < NS + NM; ++j) tmp[threadIdx.x >> 6][j] = acc[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < NS + NM) {
+        const int j = threadIdx.x;
+        double a = tmp[0][j];
+        for (int w = 1; w < kBlock / 64; ++w) a = (j < NS) ? a + tmp[w][j] : (tmp[w][j] > a ? tmp[w][j] : a);
+        out[j] = a;
+    }
+    __syncthreads();
+}
+
+template <int L, class AT, class VT, class XT, class Epi>
+__device__ __forceinline__ void brows(int nrows, const int* rp, const int* ci, const VT* v, const XT* x,
+                                      const Epi& epi) {
+    constexpr int G = kBlock / L;
+    const int g = threadIdx.x / L, lane = threadIdx.x & (L - 1);
+    for (int r0 = 0; r0 < nrows; r0 += G) {
+        const int row = r0 + g;
+        if (row < nrows) {
+            const AT s = row_dot<L, false, AT>(rp[row], rp[row + 1], lane, ci, v, x);
+            if (lane == 0) epi(row, s);
+        }
+    }
+}
+
+template <class AT, class VT, class XT, class Epi>
+__device__ void brows_any(int L, int nrows, const int* rp, const int* ci, const VT* v, const XT* x,
+                          const Epi& epi) {
+    switch (L) {
+        case 1: brows<1, AT>(nrows, rp, ci, v, x, epi); break;
+        case 2: brows<2, AT>(nrows, rp, ci, v, x, epi); break;
+        case 4: brows<4, AT>(nrows, rp, ci, v, x, epi); break;
+        case 8: brows<8, AT>(nrows, rp, ci, v, x, epi); break;
+        case 16: brows<16, AT>(nrows, rp, ci, v, x, epi); break;
+        default: brows<32, AT>(nrows, rp, ci, v, x, epi); break;
+    }
+}
+
+// Reduction row pass: per-row terms (assigned by the epilogue into t[])
+// accumulated per thread (sums, maxima), then reduced over the block into
+// out[] (shared, visible to all threads on return).
+template <int L, class VT, class XT, class Epi>
+__device__ __forceinline__ void brows_red(int nrows, const int* rp, const int* ci, const VT* v, const XT* x,
+                                          const Epi& epi, double (&acc)[Epi::NS + Epi::NM]) {
+    constexpr int G = kBlock / L;
+    const int g = threadIdx.x / L, lane = threadIdx.x & (L - 1);
+    for (int r0 = 0; r0 < nrows; r0 += G) {
+        const int row = r0 + g;
+        if (row < nrows) {
+            const double s = row_dot<L, false, double>(rp[row], rp[row + 1], lane, ci, v, x);
+            if (lane == 0) {
+                double t[Epi::NS + Epi::NM];
+                epi(row, s, t);
+#pragma unroll
+                for (int j = 0; j < Epi::NS + Epi::NM; ++j)
+                    acc[j] = j < Epi::NS ? acc[j] + t[j] : (t[j] > acc[j] ? t[j] : acc[j]);
+            }
+        }
+    }
+}
+
+template <class VT, class XT, class Epi>
+__device__ void brows_red_any(int L, int nrows, const int* rp, const int* ci, const VT* v, const XT* x,
+                              const Epi& epi, double* out) {
+    double acc[Epi::NS + Epi::NM];
+#pragma unroll
+    for (int j = 0; j < Epi::NS + Epi::NM; ++j) acc[j] = 0.0;
+    switch (L) {
+        case 1: brows_red<1>(nrows, rp, ci, v, x, epi, acc); break;
+        case 2: brows_red<2>(nrows, rp, ci, v, x, epi, acc); break;
+        case 4: brows_red<4>(nrows, rp, ci, v, x, epi, acc); break;
+        case 8: brows_red<8>(nrows, rp, ci, v, x, epi, acc); break;
+        case 16: brows_red<16>(nrows, rp, ci, v, x, epi, acc); break;
+        default: brows_red<32>(nrows, rp, ci, v, x, epi, acc); break;
+    }
+    block_reduce_all<Epi::NS, Epi::NM>(acc, out);
+}
+
+// Per-LP pointers.
+template <class VT>
+struct View {
+    int m, n, la, lat;
+    const int *rp, *ci, *cp, *ri;
+    const float *va, *vat;
+    const VT *oa, *oat;
+    const double *cmin, *lr, *ur, *lx, *ux, *dr, *dc, *wx, *wy;
+    float *c32, *lr32, *ur32, *lx32, *ux32, *g32, *x, *xe, *x0, *xt, *xl, *y, *y0, *yt, *yl;
+    double *xtot, *ytot, *xbase, *ybase;
+};
+
+struct SqDiff {
+    static constexpr int NS = 1, NM = 0;
+};
+
+template <class VT>
+__global__ __launch_bounds__(kBlock) void k_batch(BatchArrays<VT> A) {
+    const int lp = A.order[blockIdx.x];
+    BatchDesc& d = A.desc[lp];
+    const uint64_t t_start = wall_clock64();
+    View<VT> v;
+    v.m = d.m;
+    v.n = d.n;
+    v.la = d.lanes_a;
+    v.lat = d.lanes_at;
+    v.rp = A.rp + d.rp_off;
+    v.cp = A.cp + d.cp_off;
+    v.ci = A.ci + d.nz_off;
+    v.ri = A.ri + d.nz_off;
+    v.va = A.va + d.nz_off;
+    v.vat = A.vat + d.nz_off;
+    v.oa = A.oa + d.nz_off;
+    v.oat = A.oat + d.nz_off;
+    const int64_t ro = d.row_off, co = d.col_off;
+    v.cmin = A.cmin + co; v.lx = A.lx + co; v.ux = A.ux + co; v.dc = A.dc + co; v.wx = A.wx + co;
+    v.lr = A.lr + ro; v.ur = A.ur + ro; v.dr = A.dr + ro; v.wy = A.wy + ro;
+    v.c32 = A.c32 + co; v.lx32 = A.lx32 + co; v.ux32 = A.ux32 + co;
+    v.x = A.x + co; v.xe = A.xe + co; v.x0 = A.x0 + co; v.xt = A.xt + co; v.xl = A.xl + co;
+    v.xtot = A.xtot + co; v.xbase = A.xbase + co;
+    v.lr32 = A.lr32 + ro; v.ur32 = A.ur32 + ro; v.g32 = A.g32 + ro;
+    v.y = A.y + ro; v.y0 = A.y0 + ro; v.yt = A.yt + ro; v.yl = A.yl + ro;
+    v.ytot = A.ytot + ro; v.ybase = A.ybase + ro;
+    const int m = v.m, n = v.n, tid = threadIdx.x;
+
+    __shared__ Control ctl;
+    __shared__ double red[16];
+    __shared__ double eta_s, kkt_s[8];
+    __shared__ int action_s;
+
+    // ---- power iteration for ||A_hat||_2 (v in xt, u in yt, w in xe) ----
+    for (int j = tid; j < n; j += kBlock) v.xt[j] = rsqrtf(static_cast<float>(n));
+    __syncthreads();
+    double lam = 0.0;
+    if (m > 0 && d.nnz > 0) {
+        for (int it = 0; it < 48; ++it) {
+            brows_any<float>(v.la, m, v.rp, v.ci, v.va, v.xt, EpiStore<float>{v.yt});
+            __syncthreads();
+            brows_any<float>(v.lat, n, v.cp, v.ri, v.vat, v.yt, EpiStore<float>{v.xe});
+            __syncthreads();
+            double acc[1] = {0.0};
+            for (int j = tid; j < n; j += kBlock) acc[0] += static_cast<double>(v.xe[j]) * v.xe[j];
+            block_reduce_all<1, 0>(acc, red);
+            lam = red[0];
+            const double inv = lam > 0.0 ? 1.0 / sqrt(lam) : 0.0;
+            for (int j = tid; j < n; j += kBlock) v.xt[j] = static_cast<float>(v.xe[j] * inv);
+            __syncthreads();
+        }
+    }
+    // ---- initial primal weight ||D_c c|| / ||D_r b||, base points ----
+    {
+        double acc[2] = {0.0, 0.0};
+        for (int j = tid; j < n; j += kBlock) {
+            const double t = v.dc[j] * v.cmin[j];
+            acc[0] += t * t;
+            double xb = d.warm_x ? v.wx[j] : 0.0;
+            if (d.warm_x) xb = xb < v.lx[j] ? v.lx[j] : (xb > v.ux[j] ? v.ux[j] : xb);
+            v.xbase[j] = xb;
+        }
+        for (int i = tid; i < m; i += kBlock) {
+            double bh = 0.0;
+            if (v.lr[i] > -kInf) bh = fabs(v.lr[i]);
+            if (v.ur[i] < kInf && fabs(v.ur[i]) > bh) bh = fabs(v.ur[i]);
+            const double t = v.dr[i] * bh;
+            acc[1] += t * t;
+            v.ybase[i] = d.warm_y ? v.wy[i] : 0.0;
+        }
+        block_reduce_all<2, 0>(acc, red);
+        if (tid == 0) {
+            const double om = (red[0] > 1e-20 && red[1] > 1e-20) ? sqrt(red[0]) / sqrt(red[1]) : 1.0;
+            ctl.init(A.tol, A.max_iterations, A.max_refinements, om, d.warm_y != 0 && m > 0);
+            eta_s = lam > 0.0 ? 0.99 / sqrt(sqrt(lam)) : 1.0;
+        }
+        __syncthreads();
+    }
+    const double eta = eta_s;
+    bool round_start = true, have_cand = false;
+    int status = PM_SOLVE_ITERATION_LIMIT;
+    for (;;) {
+        if (round_start) {
+            const double dp = ctl.DP, dd = ctl.DD;
+            brows_any<double>(v.lat, n, v.cp, v.ri, v.oat, v.ybase,
+                              EpiRoundCol{v.cmin, v.lx, v.ux, v.xbase, v.dc, dp, dd, v.c32, v.lx32, v.ux32});
+            brows_any<double>(v.la, m, v.rp, v.ci, v.oa, v.xbase,
+                              EpiRoundRow{v.lr, v.ur, v.ybase, v.dr, dp, dd, v.lr32, v.ur32, v.g32});
+            __syncthreads();
+            for (int j = tid; j < n; j += kBlock) {
+                const float x0v = fmaxf(v.lx32[j], fminf(0.0f, v.ux32[j]));
+                v.x[j] = x0v;
+                v.x0[j] = x0v;
+                v.xl[j] = x0v;
+            }
+            for (int i = tid; i < m; i += kBlock) v.y[i] = v.y0[i] = v.yl[i] = 0.0f;
+            __syncthreads();
+            round_start = false;
+        }
+        const int chunk = ctl.chunk(A.check_frequency);
+        if (chunk <= 0) break;
+        const double om = ctl.omega;
+        const float tau = static_cast<float>(eta / om), sig = static_cast<float>(eta * om);
+        const bool use_g = ctl.use_g;
+        const int k0 = ctl.k;
+        for (int i = 0; i < chunk; ++i) {
+            const float lamk = Control::lambda(k0 + i);
+            const bool keep = i == chunk - 1;
+            brows_any<float>(v.lat, n, v.cp, v.ri, v.vat, v.y,
+                             EpiPrimalHal{tau, lamk, 1.0f, v.c32, v.lx32, v.ux32, v.x0, v.x, v.xe,
+                                          keep ? v.xt : nullptr});
+            __syncthreads();
+            brows_any<float>(v.la, m, v.rp, v.ci, v.va, v.xe,
+                             EpiDualHal{sig, lamk, 1.0f, v.lr32, v.ur32, use_g ? v.g32 : nullptr, v.y0, v.y,
+                                        keep ? v.yt : nullptr});
+            __syncthreads();
+        }
+        // ---- FP64 KKT of the candidate T(z) as a point of the original LP ----
+        const double invp = 1.0 / ctl.DP, invd = 1.0 / ctl.DD;
+        for (int j = tid; j < n; j += kBlock) {
+            double t = v.xbase[j] + v.dc[j] * static_cast<double>(v.xt[j]) * invp;
+            t = t < v.lx[j] ? v.lx[j] : (t > v.ux[j] ? v.ux[j] : t);
+            v.xtot[j] = t;
+        }
+        for (int i = tid; i < m; i += kBlock) {
+            double t = v.ybase[i] + v.dr[i] * static_cast<double>(v.yt[i]) * invd;
+            if (!(v.ur[i] < kInf) && t > 0.0) t = 0.0;
+            if (!(v.lr[i] > -kInf) && t < 0.0) t = 0.0;
+            v.ytot[i] = t;
+        }
+        __syncthreads();
+        have_cand = true;
+        brows_red_any(v.la, m, v.rp, v.ci, v.oa, v.xtot, EpiKktRow{v.lr, v.ur, v.ytot, v.dr}, red);
+        if (tid < 7) kkt_s[tid] = red[tid];
+        __syncthreads();
+        brows_red_any(v.lat, n, v.cp, v.ri, v.oat, v.ytot, EpiKktCol{v.cmin, v.lx, v.ux, v.xtot, v.dc}, red);
+        if (tid == 0) {
+            ctl.advance(chunk);
+            const double p = red[4], dd2 = kkt_s[5] + red[3];
+            const double rp = sqrt(kkt_s[0]) / (1.0 + sqrt(kkt_s[2]));
+            const double rdd = sqrt(kkt_s[3] + red[0]) / (1.0 + sqrt(red[2]));
+            const double rg = fabs(p - dd2) / (1.0 + fabs(p) + fabs(dd2));
+            double rel = rp > rdd ? rp : rdd;
+            rel = rel > rg ? rel : rg;
+            if (!(rel == rel)) rel = INFINITY;
+            d.rel_kkt = rel; d.rel_p = rp; d.rel_d = rdd; d.rel_g = rg; d.pobj = p; d.dobj = dd2;
+            action_s = ctl.on_check(rel);
+        }
+        __syncthreads();
+        const int act = action_s;
+        if (act == Control::OPTIMAL) { status = PM_SOLVE_OPTIMAL; break; }
+        if (act == Control::FAIL) { status = PM_SOLVE_NUMERICAL_FAILURE; break; }
+        if (act == Control::LIMIT) break;
+        if (act == Control::REFINE) {
+            for (int j = tid; j < n; j += kBlock) v.xbase[j] = v.xtot[j];
+            for (int i = tid; i < m; i += kBlock) v.ybase[i] = v.ytot[i];
+            __syncthreads();
+            round_start = true;
+            have_cand = false;
+            continue;
+        }
+        if (act == Control::RESTART) {
+            double acc[2] = {0.0, 0.0};
+            for (int j = tid; j < n; j += kBlock) {
+                const double df = static_cast<double>(v.xt[j]) - v.xl[j];
+                acc[0] += df * df;
+                v.x[j] = v.x0[j] = v.xl[j] = v.xt[j];
+            }
+            for (int i = tid; i < m; i += kBlock) {
+                const double df = static_cast<double>(v.yt[i]) - v.yl[i];
+                acc[1] += df * df;
+                v.y[i] = v.y0[i] = v.yl[i] = v.yt[i];
+            }
+            block_reduce_all<2, 0>(acc, red);
+            if (tid == 0) ctl.on_restart(sqrt(red[0]), sqrt(red[1]));
+            __syncthreads();
+        }
+    }
+    if (!have_cand) {
+        for (int j = tid; j < n; j += kBlock) v.xtot[j] = v.xbase[j];
+        for (int i = tid; i < m; i += kBlock) v.ytot[i] = v.ybase[i];
+        __syncthreads();
+        brows_red_any(v.la, m, v.rp, v.ci, v.oa, v.xtot, EpiKktRow{v.lr, v.ur, v.ytot, v.dr}, red);
+        if (tid < 7) kkt_s[tid] = red[tid];
+        __syncthreads();
+        brows_red_any(v.lat, n, v.cp, v.ri, v.oat, v.ytot, EpiKktCol{v.cmin, v.lx, v.ux, v.xtot, v.dc}, red);
+        if (tid == 0) {
+            const double p = red[4], dd2 = kkt_s[5] + red[3];
+            const double rp = sqrt(kkt_s[0]) / (1.0 + sqrt(kkt_s[2]));
+            const double rdd = sqrt(kkt_s[3] + red[0]) / (1.0 + sqrt(red[2]));
+            const double rg = fabs(p - dd2) / (1.0 + fabs(p) + fabs(dd2));
+            double rel = rp > rdd ? rp : rdd;
+            rel = rel > rg ? rel : rg;
+            d.rel_kkt = rel; d.rel_p = rp; d.rel_d = rdd; d.rel_g = rg; d.pobj = p; d.dobj = dd2;
+            if (rel <= A.tol) status = PM_SOLVE_OPTIMAL;
+        }
+    }
+    if (tid == 0) {
+        d.status = status;
+        d.iterations = ctl.total_it;
+        d.restarts = ctl.restarts;
+        d.refinements = ctl.rounds;
+        d.omega = ctl.omega;
+        d.t_start = t_start;
+        d.t_end = wall_clock64();
+    }
+}
+
+}  // namespace
+}  // namespace pm
+
+using namespace pm;
+
+struct pm_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int count = 0;
+    bool exact = true;
+    double tol = 1e-6;
+    int max_iterations = 100000, check_frequency = 64, max_refinements = 8;
+    std::vector<BatchDesc> desc;
+    std::vector<int> order;
+    std::vector<int> sense;
+    std::vector<double> offset;
+    std::vector<double> c_model;  // model-sense costs, concatenated
+    int64_t rows_total = 0, cols_total = 0, nnz_total = 0;
+    DevBuf<int> rp, ci, cp, ri, d_order;
+    DevBuf<float> va, vat, oa32, oat32;
+    DevBuf<double> oa64, oat64;
+    DevBuf<double> cmin, lr, ur, lx, ux, dr, dc, wx, wy;
+    DevBuf<float> c32, lr32, ur32, lx32, ux32, g32, x, xe, x0, xt, xl, y, y0, yt, yl;
+    DevBuf<double> xtot, ytot, xbase, ybase;
+    DevBuf<BatchDesc> d_desc;
+    double clock_khz = 100000.0;
+};
+
+namespace pm {
+namespace {
+
+void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_options* o) {
+    if (count <= 0) fail(PM_ERR_INVALID_ARG, "batch count must be positive");
+    if (!lps) fail(PM_ERR_INVALID_ARG, "lps is NULL");
+    pm_pdhg_options opts;
+    pm_pdhg_options_default(&opts);
+    if (o) opts = *o;
+    if (opts.check_frequency <= 0) fail(PM_ERR_INVALID_ARG, "check_frequency must be positive");
+    b->tol = opts.tolerance;
+    b->max_iterations = opts.max_iterations;
+    b->check_frequency = opts.check_frequency;
+    b->max_refinements = opts.max_refinements;
+    b->count = count;
+    b->desc.assign(count, BatchDesc{});
+    b->sense.resize(count);
+    b->offset.resize(count);
+    // Offsets and validation.
+    int64_t rpo = 0, cpo = 0, nzo = 0, ro = 0, co = 0;
+    for (int k = 0; k < count; ++k) {
+        const pm_lp& L = lps[k];
+        if (L.m < 0 || L.n <= 0 || L.nnz < 0) fail(PM_ERR_INVALID_ARG, "batch LP " + std::to_string(k) + ": bad size");
+        if (!L.row_ptr || (L.nnz > 0 && (!L.col_idx || !L.values)) || !L.c || !L.var_lower || !L.var_upper ||
+            (L.m > 0 && (!L.row_lower || !L.row_upper)))
+            fail(PM_ERR_INVALID_ARG, "batch LP " + std::to_string(k) + ": NULL array");
+        if (L.row_ptr[0] != 0 || L.row_ptr[L.m] != L.nnz)
+            fail(PM_ERR_INVALID_ARG, "batch LP " + std::to_string(k) + ": row_ptr inconsistent with nnz");
+        for (int i = 0; i < L.m; ++i)
+            if (L.row_ptr[i] > L.row_ptr[i + 1]) fail(PM_ERR_INVALID_ARG, "batch LP: row_ptr decreasing");
+        for (int q = 0; q < L.nnz; ++q)
+            if (L.col_idx[q] < 0 || L.col_idx[q] >= L.n) fail(PM_ERR_INVALID_ARG, "batch LP: col_idx out of range");
+        for (int j = 0; j < L.n; ++j)
+            if (!std::isfinite(L.c[j])) fail(PM_ERR_INVALID_ARG, "batch LP: non-finite cost");
+        BatchDesc& d = b->desc[k];
+        d.m = L.m;
+        d.n = L.n;
+        d.nnz = L.nnz;
+        d.rp_off = rpo;
+        d.cp_off = cpo;
+        d.nz_off = nzo;
+        d.row_off = ro;
+        d.col_off = co;
+        rpo += L.m + 1;
+        cpo += L.n + 1;
+        nzo += L.nnz;
+        ro += L.m;
+        co += L.n;
+        b->sense[k] = L.sense;
+        b->offset[k] = L.offset;
+        for (int q = 0; q < L.nnz && b->exact; ++q)
+            if (static_cast<double>(static_cast<float>(L.values[q])) != L.values[q]) b->exact = false;
+    }
+    b->rows_total = ro;
+    b->cols_total = co;
+    b->nnz_total = nzo;
+    const size_t R = std::max<int64_t>(ro, 1), Cn = std::max<int64_t>(co, 1), Z = std::max<int64_t>(nzo, 1);
+    std::vector<int> h_rp(rpo), h_cp(cpo), h_ci(Z), h_ri(Z);
+    std::vector<float> h_va(Z), h_vat(Z), h_oa32, h_oat32;
+    std::vector<double> h_oa64, h_oat64;
+    if (b->exact) { h_oa32.resize(Z); h_oat32.resize(Z); }
+    else { h_oa64.resize(Z); h_oat64.resize(Z); }
+    std::vector<double> h_cmin(Cn), h_lx(Cn), h_ux(Cn), h_dc(Cn), h_lr(R), h_ur(R), h_dr(R);
+    b->c_model.assign(Cn, 0.0);
+    // Per-LP preparation (transpose, Ruiz + PC scaling) on host threads.
+    auto prep = [&](int k) {
+        const pm_lp& L = lps[k];
+        BatchDesc& d = b->desc[k];
+        const int m = L.m, n = L.n, nnz = L.nnz;
+        std::vector<int> cp, ri;
+        std::vector<int64_t> perm;
+        csr_transpose(m, n, L.row_ptr, L.col_idx, cp, ri, perm);
+        std::vector<double> dr, dc;
+        const std::vector<double> vs =
+            ruiz_pc_scale(m, n, L.row_ptr, L.col_idx, L.values, opts.ruiz_iterations, dr, dc);
+        std::copy(L.row_ptr, L.row_ptr + m + 1, h_rp.begin() + d.rp_off);
+        std::copy(cp.begin(), cp.end(), h_cp.begin() + d.cp_off);
+        for (int q = 0; q < nnz; ++q) {
+            h_ci[d.nz_off + q] = L.col_idx[q];
+            h_ri[d.nz_off + q] = ri[q];
+            h_va[d.nz_off + q] = static_cast<float>(vs[q]);
+            h_vat[d.nz_off + q] = static_cast<float>(vs[perm[q]]);
+            if (b->exact) {
+                h_oa32[d.nz_off + q] = static_cast<float>(L.values[q]);
+                h_oat32[d.nz_off + q] = static_cast<float>(L.values[perm[q]]);
+            } else {
+                h_oa64[d.nz_off + q] = L.values[q];
+                h_oat64[d.nz_off + q] = L.values[perm[q]];
+            }
+        }
+        const double sg = L.sense == 1 ? -1.0 : 1.0;
+        for (int j = 0; j < n; ++j) {
+            h_cmin[d.col_off + j] = sg * L.c[j];
+            b->c_model[d.col_off + j] = L.c[j];
+            h_lx[d.col_off + j] = L.var_lower[j];
+            h_ux[d.col_off + j] = L.var_upper[j];
+            h_dc[d.col_off + j] = dc[j];
+        }
+        for (int i = 0; i < m; ++i) {
+            h_lr[d.row_off + i] = L.row_lower[i];
+            h_ur[d.row_off + i] = L.row_upper[i];
+            h_dr[d.row_off + i] = dr[i];
+        }
+        d.lanes_a = m > 0 ? lanes_for(static_cast<double>(nnz) / m) : 1;
+        d.lanes_at = lanes_for(static_cast<double>(nnz) / n);
+    };
+    {
+        const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        std::atomic<int> next{0};
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&] {
+                for (int k = next++; k < count; k = next++) prep(k);
+            });
+        for (auto& t : th) t.join();
+    }
+    // Largest LP first.
+    b->order.resize(count);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
+        return b->desc[a].nnz + b->desc[a].m + b->desc[a].n > b->desc[c].nnz + b->desc[c].m + b->desc[c].n;
+    });
+    hipStream_t s = b->stream;
+    b->rp.upload(h_rp, s);
+    b->cp.upload(h_cp, s);
+    b->ci.upload(h_ci, s);
+    b->ri.upload(h_ri, s);
+    b->va.upload(h_va, s);
+    b->vat.upload(h_vat, s);
+    if (b->exact) { b->oa32.upload(h_oa32, s); b->oat32.upload(h_oat32, s); }
+    else { b->oa64.upload(h_oa64, s); b->oat64.upload(h_oat64, s); }
+    b->cmin.upload(h_cmin, s);
+    b->lx.upload(h_lx, s);
+    b->ux.upload(h_ux, s);
+    b->dc.upload(h_dc, s);
+    b->lr.upload(h_lr, s);
+    b->ur.upload(h_ur, s);
+    b->dr.upload(h_dr, s);
+    b->d_order.upload(b->order, s);
+    for (auto* p : {&b->c32, &b->lx32, &b->ux32, &b->x, &b->xe, &b->x0, &b->xt, &b->xl}) p->resize(Cn);
+    for (auto* p : {&b->lr32, &b->ur32, &b->g32, &b->y, &b->y0, &b->yt, &b->yl}) p->resize(R);
+    for (auto* p : {&b->xtot, &b->xbase, &b->wx}) p->resize(Cn);
+    for (auto* p : {&b->ytot, &b->ybase, &b->wy}) p->resize(R);
+    b->d_desc.upload(b->desc, s);
+    PM_HIP(hipStreamSynchronize(s));
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0)
+        b->clock_khz = khz;
+}
+
+template <class VT>
+BatchArrays<VT> batch_arrays(pm_batch* b, const VT* oa, const VT* oat) {
+    BatchArrays<VT> A{};
+    A.rp = b->rp.get(); A.ci = b->ci.get(); A.va = b->va.get(); A.oa = oa;
+    A.cp = b->cp.get(); A.ri = b->ri.get(); A.vat = b->vat.get(); A.oat = oat;
+    A.cmin = b->cmin.get(); A.lr = b->lr.get(); A.ur = b->ur.get(); A.lx = b->lx.get(); A.ux = b->ux.get();
+    A.dr = b->dr.get(); A.dc = b->dc.get(); A.wx = b->wx.get(); A.wy = b->wy.get();
+    A.c32 = b->c32.get(); A.lr32 = b->lr32.get(); A.ur32 = b->ur32.get(); A.lx32 = b->lx32.get();
+    A.ux32 = b->ux32.get(); A.g32 = b->g32.get();
+    A.x = b->x.get(); A.xe = b->xe.get(); A.x0 = b->x0.get(); A.xt = b->xt.get(); A.xl = b->xl.get();
+    A.y = b->y.get(); A.y0 = b->y0.get(); A.yt = b->yt.get(); A.yl = b->yl.get();
+    A.xtot = b->xtot.get(); A.ytot = b->ytot.get(); A.xbase = b->xbase.get(); A.ybase = b->ybase.get();
+    A.desc = b->d_desc.get();
+    A.order = b->d_order.get();
+    A.tol = b->tol;
+    A.max_iterations = b->max_iterations;
+    A.check_frequency = b->check_frequency;
+    A.max_refinements = b->max_refinements;
+    return A;
+}
+
+void batch_launch(pm_batch* b) {
+    hipStream_t s = b->stream;
+    if (b->exact) {
+        auto A = batch_arrays<float>(b, b->oa32.get(), b->oat32.get());
+        hipLaunchKernelGGL(k_batch<float>, dim3(b->count), dim3(kBlock), 0, s, A);
+    } else {
+        auto A = batch_arrays<double>(b, b->oa64.get(), b->oat64.get());
+        hipLaunchKernelGGL(k_batch<double>, dim3(b->count), dim3(kBlock), 0, s, A);
+    }
+    PM_LAUNCHED();
+}
+
+void batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
+    hipStream_t s = b->stream;
+    if (!b->exec) {
+        PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        batch_launch(b);
+        PM_HIP(hipStreamEndCapture(s, &b->graph));
+        PM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    // Warm-start flags may have changed since the last run.
+    b->d_desc.upload(b->desc, s);
+    PM_HIP(hipEventRecord(b->ev0, s));
+    PM_HIP(hipGraphLaunch(b->exec, s));
+    PM_HIP(hipEventRecord(b->ev1, s));
+    PM_HIP(hipEventSynchronize(b->ev1));
+    float ms = 0.0f;
+    PM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    std::vector<BatchDesc> out(b->count);
+    b->d_desc.download(out.data(), b->count, s);
+    PM_HIP(hipStreamSynchronize(s));
+    if (stats) {
+        stats->device_seconds = ms * 1e-3;
+        if (stats->instance_seconds)
+            for (int k = 0; k < b->count; ++k)
+                stats->instance_seconds[k] =
+                    static_cast<double>(out[k].t_end - out[k].t_start) / (b->clock_khz * 1e3);
+    }
+    if (!results) return;
+    std::vector<double> xh, yh;
+    for (int k = 0; k < b->count; ++k) {
+        const BatchDesc& d = out[k];
+        pm_pdhg_result& r = results[k];
+        r.status = d.status;
+        r.iterations = d.iterations;
+        r.restarts = d.restarts;
+        r.refinements = d.refinements;
+        r.rel_kkt = d.rel_kkt;
+        r.rel_primal = d.rel_p;
+        r.rel_dual = d.rel_d;
+        r.rel_gap = d.rel_g;
+        r.primal_objective = d.pobj;
+        r.dual_objective = d.dobj;
+        r.primal_weight = d.omega;
+        r.device_seconds = static_cast<double>(d.t_end - d.t_start) / (b->clock_khz * 1e3);
+        r.primal_residual = NAN;
+        r.dual_residual = NAN;
+        const double sg = b->sense[k] == 1 ? -1.0 : 1.0;
+        r.objective = sg * d.pobj + b->offset[k];
+        if (r.x) PM_HIP(hipMemcpyAsync(r.x, b->xtot.get() + d.col_off, sizeof(double) * d.n, hipMemcpyDeviceToHost, s));
+        if (r.y && d.m > 0)
+            PM_HIP(hipMemcpyAsync(r.y, b->ytot.get() + d.row_off, sizeof(double) * d.m, hipMemcpyDeviceToHost, s));
+        if (r.row_activity) std::fill(r.row_activity, r.row_activity + d.m, NAN);
+    }
+    PM_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+}  // namespace pm
+
+extern "C" {
+
+int pm_batch_create(int32_t device, int32_t count, const pm_lp* lps, const pm_pdhg_options* opts, pm_batch** out) {
+    return pm::guarded([&] {
+        if (!out) pm::fail(PM_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        int nd = 0;
+        if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) {
+            (void)hipGetLastError();
+            pm::fail(PM_ERR_NO_DEVICE, "no HIP device visible");
+        }
+        if (device < 0 || device >= nd) pm::fail(PM_ERR_NO_DEVICE, "device ordinal out of range");
+        PM_HIP(hipSetDevice(device));
+        auto b = std::make_unique<pm_batch>();
+        b->device = device;
+        PM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        PM_HIP(hipEventCreate(&b->ev0));
+        PM_HIP(hipEventCreate(&b->ev1));
+        pm::batch_create(b.get(), count, lps, opts);
+        *out = b.release();
+    });
+}
+
+int pm_batch_set_warm_start(pm_batch* b, int32_t i, const double* x0, const double* y0) {
+    return pm::guarded([&] {
+        if (!b || i < 0 || i >= b->count) pm::fail(PM_ERR_INVALID_ARG, "bad batch or index");
+        PM_HIP(hipSetDevice(b->device));
+        pm::BatchDesc& d = b->desc[i];
+        d.warm_x = x0 != nullptr;
+        d.warm_y = y0 != nullptr && d.m > 0;
+        if (x0) PM_HIP(hipMemcpyAsync(b->wx.get() + d.col_off, x0, sizeof(double) * d.n, hipMemcpyHostToDevice, b->stream));
+        if (y0 && d.m > 0)
+            PM_HIP(hipMemcpyAsync(b->wy.get() + d.row_off, y0, sizeof(double) * d.m, hipMemcpyHostToDevice, b->stream));
+        PM_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+int pm_batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
+    return pm::guarded([&] {
+        if (!b) pm::fail(PM_ERR_INVALID_ARG, "batch is NULL");
+        PM_HIP(hipSetDevice(b->device));
+        pm::batch_run(b, results, stats);
+    });
+}
+
+int pm_batch_destroy(pm_batch* b) {
+    return pm::guarded([&] {
+        if (!b) return;
+        (void)hipSetDevice(b->device);
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->ev0) (void)hipEventDestroy(b->ev0);
+        if (b->ev1) (void)hipEventDestroy(b->ev1);
+        hipStream_t s = b->stream;
+        delete b;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int pm_batch_solve(int32_t count, const pm_lp* lps, const double* const* x0, const double* const* y0,
+                   const pm_pdhg_options* opts, pm_pdhg_result* results, pm_batch_stats* stats) {
+    pm_batch* b = nullptr;
+    int rc = pm_batch_create(opts ? opts->device : 0, count, lps, opts, &b);
+    if (rc != PM_OK) return rc;
+    for (int k = 0; k < count && rc == PM_OK; ++k)
+        if ((x0 && x0[k]) || (y0 && y0[k])) rc = pm_batch_set_warm_start(b, k, x0 ? x0[k] : nullptr, y0 ? y0[k] : nullptr);
+    if (rc == PM_OK) rc = pm_batch_run(b, results, stats);
+    const std::string err = rc != PM_OK ? std::string(pm_last_error()) : std::string();
+    pm_batch_destroy(b);
+    if (rc != PM_OK) pm::set_last_error(err);
+    return rc;
+}
+
+}  // extern "C"

@@ -27,6 +27,7 @@
 #include <numeric>
 #include <tuple>
 
+#include "control.hpp"
 #include "dispatch.hpp"
 
 namespace pm {
@@ -124,47 +125,6 @@ struct FnNormalize {
     __device__ void operator()(int i) const {
         const double s = ss[0];
         v[i] = s > 0.0 ? static_cast<float>(static_cast<double>(w[i]) / sqrt(s)) : 0.0f;
-    }
-};
-
-__device__ inline float to_f32_bound(double v) {
-    if (v >= 1e29) return kFltInf;
-    if (v <= -1e29) return -kFltInf;
-    return static_cast<float>(v);
-}
-
-// Correction-LP column data of a refinement round from FP64 A^T y_base:
-// c_hat = D_c DD (c + A^T y_base), bounds DP (l - x_base) / D_c.
-struct EpiRoundCol {
-    const double* c;
-    const double* lo;
-    const double* hi;
-    const double* xb;
-    const double* dc;
-    double dp, dd;
-    float* c32;
-    float* lo32;
-    float* hi32;
-    __device__ void operator()(int j, double aty) const {
-        c32[j] = to_f32_bound(dc[j] * dd * (c[j] + aty));
-        lo32[j] = lo[j] > -kInf ? to_f32_bound(dp * (lo[j] - xb[j]) / dc[j]) : -kFltInf;
-        hi32[j] = hi[j] < kInf ? to_f32_bound(dp * (hi[j] - xb[j]) / dc[j]) : kFltInf;
-    }
-};
-// Row data: bounds D_r DP (l - A x_base), shift g = -DD y_base / D_r.
-struct EpiRoundRow {
-    const double* lo;
-    const double* hi;
-    const double* yb;
-    const double* dr;
-    double dp, dd;
-    float* lo32;
-    float* hi32;
-    float* g32;
-    __device__ void operator()(int i, double ax) const {
-        lo32[i] = lo[i] > -kInf ? to_f32_bound(dr[i] * dp * (lo[i] - ax)) : -kFltInf;
-        hi32[i] = hi[i] < kInf ? to_f32_bound(dr[i] * dp * (hi[i] - ax)) : kFltInf;
-        g32[i] = to_f32_bound(-dd * yb[i] / dr[i]);
     }
 };
 
@@ -817,132 +777,82 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
     const int m = c->m, n = c->n;
     const auto& o = c->opts;
     const int K = std::max(1, o.check_frequency);
-    const double tol = o.tolerance;
 
-    // Round 0 base point: the warm start, or zero.
+    // Round 0 base point: the warm start (clamped to the bounds), or zero.
     if (!c->x0_h.empty()) {
         std::vector<double> xb(n);
         for (int j = 0; j < n; ++j) xb[j] = std::min(std::max(c->x0_h[j], c->lx[j]), c->ux[j]);
         c->xbase.upload(xb, s);
     } else {
         launch_elem(n, FnFill<double>{c->xbase.get(), 0.0}, s);
-        // zero is projected by the round data (bounds shift), so x_tot = clamp(0).
     }
     if (!c->y0_h.empty() && m > 0) c->ybase.upload(c->y0_h, s);
     else launch_elem(std::max(m, 1), FnFill<double>{c->ybase.get(), 0.0}, s);
-    const bool warm = !c->x0_h.empty() || !c->y0_h.empty();
 
-    double DP = 1.0, DD = 1.0;
-    double omega = c->omega0;
-    int total_it = 0, restarts = 0, rounds = 0;
+    Control ctl;
+    ctl.init(o.tolerance, o.max_iterations, o.max_refinements, c->omega0, !c->y0_h.empty() && m > 0);
     res->status = PM_SOLVE_ITERATION_LIMIT;
     const auto t0 = std::chrono::steady_clock::now();
-    Kkt last{};
-    last.rel = INFINITY;
-    bool have_last = false;
-    bool done = false;
-    const double b_suff = 0.2, b_nec = 0.8, b_art = 0.36, theta = 0.5, round_gain = 1e-3;
-    const int stall_n = 20;
-    bool use_g = warm && !c->y0_h.empty();
-
-    while (!done) {
-        round_data(c, DP, DD);
-        int k = 0;
-        double k_round_start = -1.0, k_last = -1.0, k_prev = INFINITY;
-        int it_since = 0;
-        std::vector<double> hist;
-        while (true) {
-            const int chunk = std::min(K, o.max_iterations - total_it);
-            if (chunk <= 0) { done = true; break; }
-            for (int i = 0; i < chunk; ++i) hal_iteration(c, k + i, omega, i == chunk - 1, use_g);
-            k += chunk;
-            total_it += chunk;
-            it_since += chunk;
-            // Candidate = T(z) of the last iteration, as a total point in FP64.
-            launch_elem(n, FnTotalX{c->xbase.get(), c->d_dc.get(), c->xt.get(), 1.0 / DP, c->d_lx.get(),
-                                    c->d_ux.get(), c->xtot.get()},
+    bool have_candidate = false;
+    bool round_start = true;
+    for (;;) {
+        if (round_start) {
+            round_data(c, ctl.DP, ctl.DD);
+            round_start = false;
+        }
+        const int chunk = ctl.chunk(K);
+        if (chunk <= 0) break;
+        for (int i = 0; i < chunk; ++i) hal_iteration(c, ctl.k + i, ctl.omega, i == chunk - 1, ctl.use_g);
+        ctl.advance(chunk);
+        // Candidate = T(z) of the last iteration, as a total point in FP64.
+        launch_elem(n, FnTotalX{c->xbase.get(), c->d_dc.get(), c->xt.get(), 1.0 / ctl.DP, c->d_lx.get(),
+                                c->d_ux.get(), c->xtot.get()},
+                    s);
+        if (m > 0)
+            launch_elem(m, FnTotalY{c->ybase.get(), c->d_dr.get(), c->yt.get(), 1.0 / ctl.DD, c->d_lr.get(),
+                                    c->d_ur.get(), c->ytot.get()},
                         s);
-            if (m > 0)
-                launch_elem(m, FnTotalY{c->ybase.get(), c->d_dr.get(), c->yt.get(), 1.0 / DD, c->d_lr.get(),
-                                        c->d_ur.get(), c->ytot.get()},
-                            s);
-            const Kkt kk = eval_kkt(c, c->xtot.get(), c->ytot.get());
-            last = kk;
-            have_last = true;
-            if (o.verbose)
-                fprintf(stderr, "[pramaan] it %d round %d kkt %.3e (p %.2e d %.2e g %.2e) omega %.3g restarts %d\n",
-                        total_it, rounds, kk.rel, kk.rel_p, kk.rel_d, kk.rel_g, omega, restarts);
-            if (!std::isfinite(kk.rel)) {
-                res->status = PM_SOLVE_NUMERICAL_FAILURE;
-                done = true;
-                break;
-            }
-            if (kk.rel <= tol) {
-                res->status = PM_SOLVE_OPTIMAL;
-                done = true;
-                break;
-            }
-            if (total_it >= o.max_iterations) { done = true; break; }
-            if (k_round_start < 0) k_round_start = kk.rel;
-            hist.push_back(kk.rel);
-            bool stalled = false;
-            if (static_cast<int>(hist.size()) > stall_n) {
-                double recent = INFINITY, older = INFINITY;
-                for (size_t q = 0; q < hist.size(); ++q) {
-                    if (q + stall_n >= hist.size()) recent = std::min(recent, hist[q]);
-                    else older = std::min(older, hist[q]);
-                }
-                stalled = recent > 0.5 * older;
-            }
-            if (rounds < o.max_refinements && (kk.rel <= k_round_start * round_gain || stalled)) {
-                // New refinement round around the candidate.
-                std::swap(c->xbase, c->xtot);
-                std::swap(c->ybase, c->ytot);
-                const double f = std::min(1.0 / std::max(kk.rel, 1e-300), std::max(DP, 1.0) * 1e4);
-                DP = DD = std::max(f, 1.0);
-                use_g = true;
-                ++rounds;
-                break;
-            }
-            if (k_last < 0) k_last = kk.rel;
-            const bool restart = kk.rel <= b_suff * k_last || (kk.rel <= b_nec * k_last && kk.rel > k_prev) ||
-                                 it_since >= b_art * total_it;
-            k_prev = kk.rel;
-            if (restart) {
-                // z <- T(z); anchor z0 <- z; primal weight from the distances moved.
-                double* red = c->red.get();
-                launch_elem_reduce(n, FnNormDiff{c->xt.get(), c->xl.get()}, c->partials.get(), red + 48, s);
-                launch_elem_reduce(m, FnNormDiff{c->yt.get(), c->yl.get()}, c->partials.get(), red + 49, s);
-                PM_HIP(hipMemcpyAsync(c->hred.get() + 48, red + 48, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-                launch_elem(n, FnCopy<float>{c->x.get(), c->xt.get()}, s);
-                launch_elem(n, FnCopy<float>{c->x0.get(), c->xt.get()}, s);
-                launch_elem(n, FnCopy<float>{c->xl.get(), c->xt.get()}, s);
-                launch_elem(m, FnCopy<float>{c->y.get(), c->yt.get()}, s);
-                launch_elem(m, FnCopy<float>{c->y0.get(), c->yt.get()}, s);
-                launch_elem(m, FnCopy<float>{c->yl.get(), c->yt.get()}, s);
-                sync(c);
-                const double dx = std::sqrt(c->hred[48]), dy = std::sqrt(c->hred[49]);
-                if (dx > 1e-10 && dy > 1e-10)
-                    omega = std::exp(theta * std::log(dy / dx) + (1.0 - theta) * std::log(omega));
-                k = 0;
-                k_last = kk.rel;
-                k_prev = INFINITY;
-                it_since = 0;
-                ++restarts;
-            }
+        have_candidate = true;
+        const Kkt kk = eval_kkt(c, c->xtot.get(), c->ytot.get());
+        const int act = ctl.on_check(kk.rel);
+        if (o.verbose)
+            fprintf(stderr, "[pramaan] it %d round %d kkt %.3e (p %.2e d %.2e g %.2e) omega %.3g restarts %d\n",
+                    ctl.total_it, ctl.rounds, kk.rel, kk.rel_p, kk.rel_d, kk.rel_g, ctl.omega, ctl.restarts);
+        if (act == Control::OPTIMAL) { res->status = PM_SOLVE_OPTIMAL; break; }
+        if (act == Control::FAIL) { res->status = PM_SOLVE_NUMERICAL_FAILURE; break; }
+        if (act == Control::LIMIT) break;
+        if (act == Control::REFINE) {
+            std::swap(c->xbase, c->xtot);
+            std::swap(c->ybase, c->ytot);
+            round_start = true;
+            have_candidate = false;
+            continue;
+        }
+        if (act == Control::RESTART) {
+            // z <- T(z); anchor z0 <- z; primal weight from the distances moved.
+            double* red = c->red.get();
+            launch_elem_reduce(n, FnNormDiff{c->xt.get(), c->xl.get()}, c->partials.get(), red + 48, s);
+            launch_elem_reduce(m, FnNormDiff{c->yt.get(), c->yl.get()}, c->partials.get(), red + 49, s);
+            PM_HIP(hipMemcpyAsync(c->hred.get() + 48, red + 48, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            launch_elem(n, FnCopy<float>{c->x.get(), c->xt.get()}, s);
+            launch_elem(n, FnCopy<float>{c->x0.get(), c->xt.get()}, s);
+            launch_elem(n, FnCopy<float>{c->xl.get(), c->xt.get()}, s);
+            launch_elem(m, FnCopy<float>{c->y.get(), c->yt.get()}, s);
+            launch_elem(m, FnCopy<float>{c->y0.get(), c->yt.get()}, s);
+            launch_elem(m, FnCopy<float>{c->yl.get(), c->yt.get()}, s);
+            sync(c);
+            ctl.on_restart(std::sqrt(c->hred[48]), std::sqrt(c->hred[49]));
         }
     }
     sync(c);
     res->device_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    res->iterations = total_it;
-    res->restarts = restarts;
-    res->refinements = rounds;
-    res->primal_weight = omega;
-    if (!have_last) {
-        // No iteration ran (max_iterations == 0): report the base point.
-        launch_elem(n, FnTotalX{c->xbase.get(), c->d_dc.get(), c->x.get(), 0.0, c->d_lx.get(), c->d_ux.get(),
-                                c->xtot.get()},
-                    s);
+    res->iterations = ctl.total_it;
+    res->restarts = ctl.restarts;
+    res->refinements = ctl.rounds;
+    res->primal_weight = ctl.omega;
+    if (!have_candidate) {
+        // No iteration since the last base point: report the base point.
+        launch_elem(n, FnCopy<double>{c->xtot.get(), c->xbase.get()}, s);
         if (m > 0) launch_elem(m, FnCopy<double>{c->ytot.get(), c->ybase.get()}, s);
     }
     std::vector<double> xh(n), yh(m);
@@ -950,7 +860,7 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
     if (m > 0) c->ytot.download(yh.data(), m, s);
     sync(c);
     double pv = 0, dv = 0;
-    ref_style_residuals(c, c->xtot.get(), c->ytot.get(), tol, pv, dv);
+    ref_style_residuals(c, c->xtot.get(), c->ytot.get(), o.tolerance, pv, dv);
     res->primal_residual = pv;
     res->dual_residual = dv;
     finish_result(c, res, c->xtot.get(), c->ytot.get(), xh, yh);

@@ -401,4 +401,45 @@ struct EpiKktCol {
     }
 };
 
+__device__ __forceinline__ float to_f32_bound(double v) {
+    if (v >= 1e29) return kFltInf;
+    if (v <= -1e29) return -kFltInf;
+    return static_cast<float>(v);
+}
+
+// Correction-LP column data of a refinement round from FP64 A^T y_base:
+// c_hat = D_c DD (c + A^T y_base), bounds DP (l - x_base) / D_c.
+struct EpiRoundCol {
+    const double* c;
+    const double* lo;
+    const double* hi;
+    const double* xb;
+    const double* dc;
+    double dp, dd;
+    float* c32;
+    float* lo32;
+    float* hi32;
+    __device__ void operator()(int j, double aty) const {
+        c32[j] = to_f32_bound(dc[j] * dd * (c[j] + aty));
+        lo32[j] = lo[j] > -kInf ? to_f32_bound(dp * (lo[j] - xb[j]) / dc[j]) : -kFltInf;
+        hi32[j] = hi[j] < kInf ? to_f32_bound(dp * (hi[j] - xb[j]) / dc[j]) : kFltInf;
+    }
+};
+// Row data: bounds D_r DP (l - A x_base), shift g = -DD y_base / D_r.
+struct EpiRoundRow {
+    const double* lo;
+    const double* hi;
+    const double* yb;
+    const double* dr;
+    double dp, dd;
+    float* lo32;
+    float* hi32;
+    float* g32;
+    __device__ void operator()(int i, double ax) const {
+        lo32[i] = lo[i] > -kInf ? to_f32_bound(dr[i] * dp * (lo[i] - ax)) : -kFltInf;
+        hi32[i] = hi[i] < kInf ? to_f32_bound(dr[i] * dp * (hi[i] - ax)) : kFltInf;
+        g32[i] = to_f32_bound(-dd * yb[i] / dr[i]);
+    }
+};
+
 }  // namespace pm
