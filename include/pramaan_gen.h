@@ -55,6 +55,32 @@ int pm_gen_view(const pm_gen_lp* g, pm_lp* lp, const double** x_star,
                 const double** y_star, double* p_star);
 void pm_gen_free(pm_gen_lp* g);
 
+/* ---- rolling-horizon blending LPs --------------------------------------
+ * A T-period blending plan (24 components, 15 products, 8 quality
+ * attributes, 10 processing units; 400 columns and 194 rows per period,
+ * periods linked by inventory-balance rows).  Window w is the LP of periods
+ * [w, w + window) with the exogenous opening stock of period w; column
+ * tau*400 + j and row tau*194 + i of a window are column j / row i of its
+ * period tau.  The default horizon is 523 periods, i.e. 500 windows of 24.
+ * demand_noise / blend_noise (with perturb_seed) scale every demand and
+ * blend coefficient by U(1 - noise, 1 + noise).  Windows carry no closed-
+ * form optimum (x_star / y_star views are empty, p_star is NaN). */
+typedef struct pm_rh_params {
+    int32_t periods_total;
+    int32_t window;
+    double demand_noise;
+    double blend_noise;
+    uint64_t seed;
+    uint64_t perturb_seed;
+} pm_rh_params;
+typedef struct pm_rh pm_rh;
+int pm_rh_params_default(pm_rh_params* p);
+int pm_rh_create(const pm_rh_params* p, pm_rh** out);
+int pm_rh_window_size(const pm_rh* h, int32_t* rows_per_period,
+                      int32_t* cols_per_period, int32_t* windows);
+int pm_rh_window(const pm_rh* h, int32_t start, pm_gen_lp** out);
+void pm_rh_free(pm_rh* h);
+
 #ifdef __cplusplus
 }
 #endif

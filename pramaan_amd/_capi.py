@@ -61,6 +61,11 @@ class pm_gen_params(C.Structure):
                 ("var_upper", C.c_double), ("seed", C.c_uint64)]
 
 
+class pm_rh_params(C.Structure):
+    _fields_ = [("periods_total", C.c_int32), ("window", C.c_int32), ("demand_noise", C.c_double),
+                ("blend_noise", C.c_double), ("seed", C.c_uint64), ("perturb_seed", C.c_uint64)]
+
+
 class pm_batch_stats(C.Structure):
     _fields_ = [("device_seconds", C.c_double), ("instance_seconds", _d)]
 
@@ -92,6 +97,20 @@ SIGNATURES = {
     "pm_gen_synthetic": (C.c_int, [C.POINTER(pm_gen_params), C.POINTER(C.c_void_p)]),
     "pm_gen_view": (C.c_int, [C.c_void_p, C.POINTER(pm_lp), C.POINTER(_d), C.POINTER(_d), _d]),
     "pm_gen_free": (None, [C.c_void_p]),
+    "pm_gnn_num_params": (C.c_int32, []),
+    "pm_gnn_create": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pm_gnn_destroy": (C.c_int, [C.c_void_p]),
+    "pm_gnn_get_params": (C.c_int, [C.c_void_p, _f]),
+    "pm_gnn_set_params": (C.c_int, [C.c_void_p, _f]),
+    "pm_gnn_set_graph": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _i, _i, _f, _i, _i, _f, _f, _f]),
+    "pm_gnn_forward": (C.c_int, [C.c_void_p, _f, _f]),
+    "pm_gnn_gradient": (C.c_int, [C.c_void_p, _f, _f, _f, _f]),
+    "pm_gnn_train_step": (C.c_int, [C.c_void_p, _f, _f, C.c_float, _f]),
+    "pm_rh_params_default": (C.c_int, [C.POINTER(pm_rh_params)]),
+    "pm_rh_create": (C.c_int, [C.POINTER(pm_rh_params), C.POINTER(C.c_void_p)]),
+    "pm_rh_window_size": (C.c_int, [C.c_void_p, _i, _i, _i]),
+    "pm_rh_window": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pm_rh_free": (None, [C.c_void_p]),
 }
 
 _lib = None
@@ -371,6 +390,68 @@ class Batch:
             pass
 
 
+class Gnn:
+    """pm_gnn_*: the warm-start network (forward, backward, Adam on the GPU)."""
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        self._h = C.c_void_p()
+        _check(lib().pm_gnn_create(device, seed, C.byref(self._h)))
+        self.n_params = lib().pm_gnn_num_params()
+        self.m = self.n = 0
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        _check(lib().pm_gnn_get_params(self._h, _p(out, C.c_float)))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().pm_gnn_set_params(self._h, _p(p, C.c_float)))
+
+    def set_graph(self, g: dict) -> None:
+        """g: dict with m, n, rp, ci, ar, cp, ri, ac, xr, xc (see warmstart.graph_of)."""
+        a = {k: np.ascontiguousarray(g[k], np.int32 if k in ("rp", "ci", "cp", "ri") else np.float32)
+             for k in ("rp", "ci", "ar", "cp", "ri", "ac", "xr", "xc")}
+        self._graph = a
+        self.m, self.n = int(g["m"]), int(g["n"])
+        _check(lib().pm_gnn_set_graph(self._h, self.m, self.n, len(a["ci"]), _p(a["rp"], C.c_int32),
+                                      _p(a["ci"], C.c_int32), _p(a["ar"], C.c_float), _p(a["cp"], C.c_int32),
+                                      _p(a["ri"], C.c_int32), _p(a["ac"], C.c_float), _p(a["xr"], C.c_float),
+                                      _p(a["xc"], C.c_float)))
+
+    def forward(self):
+        x = np.zeros(self.n, np.float32)
+        y = np.zeros(self.m, np.float32)
+        _check(lib().pm_gnn_forward(self._h, _p(x, C.c_float), _p(y, C.c_float)))
+        return x, y
+
+    def gradient(self, tx, ty):
+        tx, ty = np.ascontiguousarray(tx, np.float32), np.ascontiguousarray(ty, np.float32)
+        loss = C.c_float()
+        g = np.zeros(self.n_params, np.float32)
+        _check(lib().pm_gnn_gradient(self._h, _p(tx, C.c_float), _p(ty, C.c_float), C.byref(loss),
+                                     _p(g, C.c_float)))
+        return loss.value, g
+
+    def train_step(self, tx, ty, lr: float) -> float:
+        tx, ty = np.ascontiguousarray(tx, np.float32), np.ascontiguousarray(ty, np.float32)
+        loss = C.c_float()
+        _check(lib().pm_gnn_train_step(self._h, _p(tx, C.c_float), _p(ty, C.c_float), C.c_float(lr),
+                                       C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self._h:
+            lib().pm_gnn_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _csr32(row_ptr, col_idx, values):
     return (np.ascontiguousarray(row_ptr, np.int32), np.ascontiguousarray(col_idx, np.int32),
             np.ascontiguousarray(values, np.float32))
@@ -407,6 +488,52 @@ def launch_pdhg_iteration(x, y, gradient, ax_bar, b, tau, sigma):
                                           _p(a, C.c_float), _p(bb, C.c_float), len(x), len(y),
                                           C.c_float(tau), C.c_float(sigma)))
     return x, y
+
+
+def _lp_from_gen(h) -> LP:
+    s = pm_lp()
+    xs, ys = _d(), _d()
+    ps = C.c_double()
+    _check(lib().pm_gen_view(h, C.byref(s), C.byref(xs), C.byref(ys), C.byref(ps)))
+    m, n, nnz = s.m, s.n, s.nnz
+    arr = lambda ptr, k: np.ctypeslib.as_array(ptr, (k,)).copy() if k else np.zeros(0)
+    xstar = arr(xs, n) if xs and np.isfinite(ps.value) else None
+    ystar = arr(ys, m) if ys and np.isfinite(ps.value) else None
+    return LP(arr(s.c, n), np.ctypeslib.as_array(s.row_ptr, (m + 1,)).copy(),
+              np.ctypeslib.as_array(s.col_idx, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
+              arr(s.values, nnz), arr(s.row_lower, m), arr(s.row_upper, m), arr(s.var_lower, n),
+              arr(s.var_upper, n), 0, 0.0, xstar, ystar, ps.value if np.isfinite(ps.value) else None)
+
+
+class RollingHorizon:
+    """pm_rh_*: the multi-period blending plan and its windows."""
+
+    def __init__(self, **params):
+        p = pm_rh_params()
+        _check(lib().pm_rh_params_default(C.byref(p)))
+        for k, v in params.items():
+            setattr(p, k, v)
+        self.params = p
+        self._h = C.c_void_p()
+        _check(lib().pm_rh_create(C.byref(p), C.byref(self._h)))
+        r, c, w = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().pm_rh_window_size(self._h, C.byref(r), C.byref(c), C.byref(w)))
+        self.rows_per_period, self.cols_per_period, self.windows = r.value, c.value, w.value
+
+    def window(self, start: int) -> LP:
+        g = C.c_void_p()
+        _check(lib().pm_rh_window(self._h, start, C.byref(g)))
+        try:
+            return _lp_from_gen(g)
+        finally:
+            lib().pm_gen_free(g)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().pm_rh_free(self._h)
+        except Exception:
+            pass
 
 
 def generate(which: int | None = None, seed: int = 1, **params) -> LP:

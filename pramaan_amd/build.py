@@ -27,8 +27,8 @@ ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 ARCH = os.environ.get("PRAMAAN_ARCH", "gfx950")
 
-HIP_SOURCES = ["engine.hip", "launchers.hip", "batch.hip"]
-HOST_SOURCES = ["host/gen.cpp"]
+HIP_SOURCES = ["engine.hip", "launchers.hip", "batch.hip", "gnn.hip"]
+HOST_SOURCES = ["host/gen.cpp", "host/gen_rolling.cpp"]
 CXX_SOURCES = ["cxx/model_ir.cpp", "cxx/pdhg_solver.cpp", "cxx/backend_hip.cpp", "cxx/test_abi.cpp"]
 CXX_LIB = PKG / "lib" / "libpramaan_cxx.so"
 

@@ -1,0 +1,741 @@
+// gnn.hip -- forward, backward and Adam kernels of the warm-start network
+// (include/pramaan_gnn.h).
+//
+// Activations are N x 64 FP32 row-major (one 256-byte row per node).  The
+// 64x64 dense layers run on MFMA (v_mfma_f32_32x32x8_bf16): one wavefront
+// computes a 32-row x 64-column output tile as two 32x32 accumulators,
+// converting FP32 activations to bfloat16 (round to nearest even) as it
+// loads them and reading weights from a pre-converted bfloat16 copy laid
+// out k-contiguous per output column.  Weight gradients (64x64, summed over
+// all nodes) use the same instruction with K running over nodes, one
+// partial per workgroup, reduced in a fixed order.  Message passing is a
+// sparse-times-dense product with one wavefront per output node and one
+// lane per feature (64 lanes = 64 features, coalesced 256-byte rows).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "pramaan_gnn.h"
+
+namespace pm {
+namespace {
+
+constexpr int kH = 64, kF = 8;
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<short>(u >> 16);
+}
+
+// Row of a 32x32 MFMA accumulator element e held by `lane`
+// (column = lane % 32).
+__device__ __forceinline__ int acc_row(int e, int lane) { return 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3); }
+
+// Out[N][64] = act(A1 B1 (+ A2 B2) (+ bias)) (+ Out when ACC), with
+// B given as BT[c][k] = B[k][c] in bfloat16.  Block = 4 waves = 128 rows.
+template <bool TWO, bool BIAS, bool RELU, bool ACC>
+__global__ __launch_bounds__(256) void k_dense(int N, const float* __restrict__ A1, const short* __restrict__ B1T,
+                                               const float* __restrict__ A2, const short* __restrict__ B2T,
+                                               const float* __restrict__ bias, float* __restrict__ Out) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row0 = (blockIdx.x * 4 + w) * 32;
+    if (row0 >= N) return;
+    const int r = row0 + (lane & 31), kg = lane >> 5, c = lane & 31;
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc0[e] = acc1[e] = 0.0f;
+#pragma unroll
+    for (int in = 0; in < (TWO ? 2 : 1); ++in) {
+        const float* A = in == 0 ? A1 : A2;
+        const short* BT = in == 0 ? B1T : B2T;
+#pragma unroll
+        for (int ks = 0; ks < kH / 8; ++ks) {
+            const int k0 = ks * 8 + 4 * kg;
+            float4 av = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r < N) av = *reinterpret_cast<const float4*>(A + static_cast<size_t>(r) * kH + k0);
+            bf16x4 a;
+            a[0] = f2bf(av.x); a[1] = f2bf(av.y); a[2] = f2bf(av.z); a[3] = f2bf(av.w);
+            const bf16x4 b0 = *reinterpret_cast<const bf16x4*>(BT + c * kH + k0);
+            const bf16x4 b1 = *reinterpret_cast<const bf16x4*>(BT + (32 + c) * kH + k0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int row = row0 + acc_row(e, lane);
+        if (row >= N) continue;
+        float v0 = acc0[e], v1 = acc1[e];
+        if (BIAS) { v0 += bias[c]; v1 += bias[32 + c]; }
+        if (RELU) { v0 = fmaxf(v0, 0.0f); v1 = fmaxf(v1, 0.0f); }
+        float* o = Out + static_cast<size_t>(row) * kH;
+        if (ACC) { o[c] += v0; o[32 + c] += v1; }
+        else { o[c] = v0; o[32 + c] = v1; }
+    }
+}
+
+// Partial weight gradient over rows [b*rows_per, (b+1)*rows_per):
+// P[b][i][j] = sum_r A[r][i] D[r][j] (bfloat16 inputs, FP32 accumulate).
+// Wave w computes the 32x32 tile (w / 2, w % 2).
+__global__ __launch_bounds__(256) void k_wgrad(int N, int rows_per, const float* __restrict__ A,
+                                               const float* __restrict__ D, float* __restrict__ P) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ti = w >> 1, tj = w & 1, kg = lane >> 5, c = lane & 31;
+    const int rb = blockIdx.x * rows_per;
+    int re = rb + rows_per;
+    if (re > N) re = N;
+    f32x16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    for (int r0 = rb; r0 < re; r0 += 8) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = r0 + 4 * kg + q;
+            const bool ok = r < re;
+            a[q] = f2bf(ok ? A[static_cast<size_t>(r) * kH + 32 * ti + c] : 0.0f);
+            b[q] = f2bf(ok ? D[static_cast<size_t>(r) * kH + 32 * tj + c] : 0.0f);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    float* out = P + static_cast<size_t>(blockIdx.x) * kH * kH;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) out[(32 * ti + acc_row(e, lane)) * kH + 32 * tj + c] = acc[e];
+}
+
+// Partial column sums over row chunks: P[b][c] = sum_r D[r][c] (width 64),
+// and optionally of X^T D for the 8-feature input layer: P2[b][k][c].
+__global__ __launch_bounds__(256) void k_colsum(int N, int rows_per, const float* __restrict__ D,
+                                                const float* __restrict__ X, float* __restrict__ P,
+                                                float* __restrict__ P2) {
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;  // 4 row groups
+    const int rb = blockIdx.x * rows_per;
+    int re = rb + rows_per;
+    if (re > N) re = N;
+    float s = 0.0f, sx[kF];
+#pragma unroll
+    for (int k = 0; k < kF; ++k) sx[k] = 0.0f;
+    for (int r = rb + g; r < re; r += 4) {
+        const float d = D[static_cast<size_t>(r) * kH + c];
+        s += d;
+        if (X) {
+#pragma unroll
+            for (int k = 0; k < kF; ++k) sx[k] += X[static_cast<size_t>(r) * kF + k] * d;
+        }
+    }
+    __shared__ float red[4][kH * (kF + 1)];
+    red[g][c] = s;
+#pragma unroll
+    for (int k = 0; k < kF; ++k) red[g][kH + k * kH + c] = sx[k];
+    __syncthreads();
+    if (g == 0) {
+        float t = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+        P[static_cast<size_t>(blockIdx.x) * kH + c] = t;
+        if (X)
+            for (int k = 0; k < kF; ++k) {
+                const int o = kH + k * kH + c;
+                P2[(static_cast<size_t>(blockIdx.x) * kF + k) * kH + c] = red[0][o] + red[1][o] + red[2][o] + red[3][o];
+            }
+    }
+}
+
+// out[e] (+)= sum_b P[b][e], fixed order.
+__global__ __launch_bounds__(256) void k_sum_partials(int nb, int width, const float* __restrict__ P,
+                                                      float* __restrict__ out) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= width) return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s += P[static_cast<size_t>(b) * width + e];
+    out[e] = s;
+}
+
+// Out[i][:] (+)= sum_k val[k] H[idx[k]][:], one wavefront per row.
+template <bool ACC>
+__global__ __launch_bounds__(256) void k_spmm(int nrows, const int* __restrict__ rp, const int* __restrict__ idx,
+                                              const float* __restrict__ val, const float* __restrict__ Hin,
+                                              float* __restrict__ Out) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), f = threadIdx.x & 63;
+    if (row >= nrows) return;
+    float s = 0.0f;
+    for (int k = rp[row]; k < rp[row + 1]; ++k) s = __builtin_fmaf(val[k], Hin[static_cast<size_t>(idx[k]) * kH + f], s);
+    float* o = Out + static_cast<size_t>(row) * kH + f;
+    if (ACC) *o += s;
+    else *o = s;
+}
+
+// Input layer: Out = relu(X W + b), X [N][8], W [8][64] FP32.
+__global__ __launch_bounds__(256) void k_input(int N, const float* __restrict__ X, const float* __restrict__ W,
+                                               const float* __restrict__ b, float* __restrict__ Out) {
+    const int t = blockIdx.x * 256 + threadIdx.x, r = t >> 6, c = t & 63;
+    if (r >= N) return;
+    float s = b[c];
+#pragma unroll
+    for (int k = 0; k < kF; ++k) s = __builtin_fmaf(X[static_cast<size_t>(r) * kF + k], W[k * kH + c], s);
+    Out[static_cast<size_t>(r) * kH + c] = fmaxf(s, 0.0f);
+}
+
+// Head: out[r] = H[r] . w + b.
+__global__ __launch_bounds__(256) void k_head(int N, const float* __restrict__ Hin, const float* __restrict__ w,
+                                              const float* __restrict__ b, float* __restrict__ out) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const float4* h = reinterpret_cast<const float4*>(Hin + static_cast<size_t>(r) * kH);
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kH / 4; ++q) {
+        const float4 v = h[q];
+        s = __builtin_fmaf(v.x, w[4 * q], s);
+        s = __builtin_fmaf(v.y, w[4 * q + 1], s);
+        s = __builtin_fmaf(v.z, w[4 * q + 2], s);
+        s = __builtin_fmaf(v.w, w[4 * q + 3], s);
+    }
+    out[r] = s + b[0];
+}
+
+// Loss residual: d[r] = 2 (pred - target) / N, and per-block sums of
+// (pred - target)^2 / N into P[block].
+__global__ __launch_bounds__(256) void k_mse(int N, const float* __restrict__ pred, const float* __restrict__ tgt,
+                                             float* __restrict__ d, float* __restrict__ P) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    float e2 = 0.0f;
+    if (r < N) {
+        const float e = pred[r] - tgt[r];
+        d[r] = 2.0f * e / static_cast<float>(N);
+        e2 = e * e / static_cast<float>(N);
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (static_cast<int>(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) P[blockIdx.x] = red[0];
+}
+
+// dH[r][c] = d[r] w[c]   (head backward, initialises dH)
+__global__ __launch_bounds__(256) void k_head_bwd(int N, const float* __restrict__ d, const float* __restrict__ w,
+                                                  float* __restrict__ dH) {
+    const int t = blockIdx.x * 256 + threadIdx.x, r = t >> 6, c = t & 63;
+    if (r >= N) return;
+    dH[static_cast<size_t>(r) * kH + c] = d[r] * w[c];
+}
+
+// dP = dH * (Hout > 0), in place.
+__global__ __launch_bounds__(256) void k_relu_bwd(size_t total, const float* __restrict__ Hout, float* __restrict__ dH) {
+    const size_t t = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= total) return;
+    if (!(Hout[t] > 0.0f)) dH[t] = 0.0f;
+}
+
+// Partial d-weighted row sums for the head weights: P[b][c] = sum_r d[r] H[r][c].
+__global__ __launch_bounds__(256) void k_head_wgrad(int N, int rows_per, const float* __restrict__ d,
+                                                    const float* __restrict__ Hin, float* __restrict__ P,
+                                                    float* __restrict__ Pb) {
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int rb = blockIdx.x * rows_per;
+    int re = rb + rows_per;
+    if (re > N) re = N;
+    float s = 0.0f, sb = 0.0f;
+    for (int r = rb + g; r < re; r += 4) {
+        s += d[r] * Hin[static_cast<size_t>(r) * kH + c];
+        sb += d[r];
+    }
+    __shared__ float red[4][kH];
+    __shared__ float redb[4][kH];
+    red[g][c] = s;
+    redb[g][c] = sb;
+    __syncthreads();
+    if (g == 0) {
+        P[static_cast<size_t>(blockIdx.x) * kH + c] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+        if (c == 0) Pb[blockIdx.x] = redb[0][0] + redb[1][0] + redb[2][0] + redb[3][0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adam(int np, float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m1, float* __restrict__ m2, float lr, float c1,
+                                              float c2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= np) return;
+    const float gi = g[i];
+    const float a = 0.9f * m1[i] + 0.1f * gi;
+    const float b = 0.999f * m2[i] + 0.001f * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    p[i] -= lr * (a / c1) / (sqrtf(b / c2) + 1e-8f);
+}
+
+// bfloat16 copies of a 64x64 weight: WT[c][k] = W[k][c] (forward) and
+// WN[k][c] = W[k][c] (the B^T layout of W^T, for input gradients).
+__global__ __launch_bounds__(256) void k_prep_w(const float* __restrict__ W, short* __restrict__ WT,
+                                                short* __restrict__ WN) {
+    for (int e = threadIdx.x; e < kH * kH; e += 256) {
+        const int k = e / kH, c = e % kH;
+        const short v = f2bf(W[e]);
+        WT[c * kH + k] = v;
+        WN[e] = v;
+    }
+}
+
+// ---- parameter layout --------------------------------------------------
+struct Offsets {
+    int We_c, be_c, We_r, be_r;
+    int Wrr[3], Wrz[3], br[3], Wcc[3], Wcz[3], bc[3];
+    int wx, bx, wy, by, total;
+    Offsets() {
+        int o = 0;
+        auto take = [&](int k) { const int r = o; o += k; return r; };
+        We_c = take(kF * kH); be_c = take(kH); We_r = take(kF * kH); be_r = take(kH);
+        for (int l = 1; l <= 2; ++l) {
+            Wrr[l] = take(kH * kH); Wrz[l] = take(kH * kH); br[l] = take(kH);
+            Wcc[l] = take(kH * kH); Wcz[l] = take(kH * kH); bc[l] = take(kH);
+        }
+        wx = take(kH); bx = take(1); wy = take(kH); by = take(1);
+        total = o;
+    }
+};
+const Offsets kOff;
+
+int cdiv(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
+
+}  // namespace
+}  // namespace pm
+
+using namespace pm;
+
+struct pm_gnn {
+    int device = 0;
+    hipStream_t s = nullptr;
+    int step = 0;
+    int m = 0, n = 0, nnz = 0;
+    DevBuf<float> p, grad, m1, m2;
+    DevBuf<short> wt, wn;  // 8 dense weights x 4096 bf16 each, two layouts
+    DevBuf<int> rp, ci, cp, ri;
+    DevBuf<float> ar, ac, ar_t, ac_t, xr, xc;
+    // activations: Hc0..2, Hr0..2, Zr1..2, Zc1..2, and gradients
+    DevBuf<float> Hc[3], Hr[3], Zr[3], Zc[3], dHc, dHr, dHc2, dZ, dT, xhat, yhat, dx, dy, tx, ty, part;
+    std::vector<float> host_tmp;
+};
+
+namespace pm {
+namespace {
+
+__global__ __launch_bounds__(256) void k_fill(int n, float* __restrict__ p, float v) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// index of a dense 64x64 weight in the bf16 arrays: layer l (1,2), which:
+// 0 Wrr, 1 Wrz, 2 Wcc, 3 Wcz
+int widx(int l, int which) { return (l - 1) * 4 + which; }
+int woff(int l, int which) {
+    switch (which) {
+        case 0: return kOff.Wrr[l];
+        case 1: return kOff.Wrz[l];
+        case 2: return kOff.Wcc[l];
+        default: return kOff.Wcz[l];
+    }
+}
+
+void fill(pm_gnn* g, float* p, int n, float v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill, dim3(cdiv(n, 256)), dim3(256), 0, g->s, n, p, v);
+    PM_LAUNCHED();
+}
+
+void prep_weights(pm_gnn* g) {
+    for (int l = 1; l <= 2; ++l)
+        for (int w = 0; w < 4; ++w) {
+            hipLaunchKernelGGL(k_prep_w, dim3(1), dim3(256), 0, g->s, g->p.get() + woff(l, w),
+                               g->wt.get() + widx(l, w) * kH * kH, g->wn.get() + widx(l, w) * kH * kH);
+            PM_LAUNCHED();
+        }
+}
+
+const short* WT(pm_gnn* g, int l, int w) { return g->wt.get() + widx(l, w) * kH * kH; }
+const short* WN(pm_gnn* g, int l, int w) { return g->wn.get() + widx(l, w) * kH * kH; }
+
+void dense(pm_gnn* g, int N, const float* A1, const short* B1, const float* A2, const short* B2, const float* bias,
+           float* out, bool relu, bool acc) {
+    if (N <= 0) return;
+    const dim3 grid(cdiv(N, 128)), blk(256);
+    if (A2 && bias && relu && !acc)
+        hipLaunchKernelGGL((k_dense<true, true, true, false>), grid, blk, 0, g->s, N, A1, B1, A2, B2, bias, out);
+    else if (!A2 && !bias && !relu && !acc)
+        hipLaunchKernelGGL((k_dense<false, false, false, false>), grid, blk, 0, g->s, N, A1, B1, A2, B2, bias, out);
+    else if (!A2 && !bias && !relu && acc)
+        hipLaunchKernelGGL((k_dense<false, false, false, true>), grid, blk, 0, g->s, N, A1, B1, A2, B2, bias, out);
+    else
+        fail(PM_ERR_INTERNAL, "unsupported dense-layer configuration");
+    PM_LAUNCHED();
+}
+
+void spmm(pm_gnn* g, int nrows, const int* rp, const int* idx, const float* val, const float* Hin, float* out,
+          bool acc) {
+    if (nrows <= 0) return;
+    if (acc) hipLaunchKernelGGL(k_spmm<true>, dim3(cdiv(nrows, 4)), dim3(256), 0, g->s, nrows, rp, idx, val, Hin, out);
+    else hipLaunchKernelGGL(k_spmm<false>, dim3(cdiv(nrows, 4)), dim3(256), 0, g->s, nrows, rp, idx, val, Hin, out);
+    PM_LAUNCHED();
+}
+
+constexpr int kRowsPer = 2048;  // rows per workgroup in the gradient reductions
+
+// grad[off .. off+4096) = A^T D summed over all N rows.
+void wgrad(pm_gnn* g, int N, const float* A, const float* D, int off) {
+    if (N <= 0) return fill(g, g->grad.get() + off, kH * kH, 0.0f);
+    const int nb = cdiv(N, kRowsPer);
+    hipLaunchKernelGGL(k_wgrad, dim3(nb), dim3(256), 0, g->s, N, kRowsPer, A, D, g->part.get());
+    PM_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(kH * kH, 256)), dim3(256), 0, g->s, nb, kH * kH, g->part.get(),
+                       g->grad.get() + off);
+    PM_LAUNCHED();
+}
+
+// grad[boff .. +64) = column sums of D; with X: grad[xoff .. +512) = X^T D.
+void colsum(pm_gnn* g, int N, const float* D, const float* X, int boff, int xoff) {
+    if (N <= 0) {
+        fill(g, g->grad.get() + boff, kH, 0.0f);
+        if (X) fill(g, g->grad.get() + xoff, kF * kH, 0.0f);
+        return;
+    }
+    const int nb = cdiv(N, kRowsPer);
+    float* P = g->part.get();
+    float* P2 = P + static_cast<size_t>(nb) * kH;
+    hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, g->s, N, kRowsPer, D, X, P, X ? P2 : nullptr);
+    PM_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g->s, nb, kH, P, g->grad.get() + boff);
+    PM_LAUNCHED();
+    if (X) {
+        hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(kF * kH, 256)), dim3(256), 0, g->s, nb, kF * kH, P2,
+                           g->grad.get() + xoff);
+        PM_LAUNCHED();
+    }
+}
+
+void head_grads(pm_gnn* g, int N, const float* d, const float* Hin, int woff_, int boff) {
+    if (N <= 0) {
+        fill(g, g->grad.get() + woff_, kH, 0.0f);
+        fill(g, g->grad.get() + boff, 1, 0.0f);
+        return;
+    }
+    const int nb = cdiv(N, kRowsPer);
+    float* P = g->part.get();
+    float* Pb = P + static_cast<size_t>(nb) * kH;
+    hipLaunchKernelGGL(k_head_wgrad, dim3(nb), dim3(256), 0, g->s, N, kRowsPer, d, Hin, P, Pb);
+    PM_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g->s, nb, kH, P, g->grad.get() + woff_);
+    PM_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g->s, nb, 1, Pb, g->grad.get() + boff);
+    PM_LAUNCHED();
+}
+
+void relu_bwd(pm_gnn* g, int N, const float* Hout, float* dH) {
+    if (N <= 0) return;
+    const size_t total = static_cast<size_t>(N) * kH;
+    hipLaunchKernelGGL(k_relu_bwd, dim3(cdiv(static_cast<long long>(total), 256)), dim3(256), 0, g->s, total, Hout, dH);
+    PM_LAUNCHED();
+}
+
+void forward(pm_gnn* g) {
+    const int m = g->m, n = g->n;
+    const float* P = g->p.get();
+    if (n > 0)
+        hipLaunchKernelGGL(k_input, dim3(cdiv(static_cast<long long>(n) * kH, 256)), dim3(256), 0, g->s, n,
+                           g->xc.get(), P + kOff.We_c, P + kOff.be_c, g->Hc[0].get());
+    if (m > 0)
+        hipLaunchKernelGGL(k_input, dim3(cdiv(static_cast<long long>(m) * kH, 256)), dim3(256), 0, g->s, m,
+                           g->xr.get(), P + kOff.We_r, P + kOff.be_r, g->Hr[0].get());
+    PM_LAUNCHED();
+    for (int l = 1; l <= 2; ++l) {
+        spmm(g, m, g->rp.get(), g->ci.get(), g->ar.get(), g->Hc[l - 1].get(), g->Zr[l].get(), false);
+        dense(g, m, g->Hr[l - 1].get(), WT(g, l, 0), g->Zr[l].get(), WT(g, l, 1), P + kOff.br[l], g->Hr[l].get(), true,
+              false);
+        spmm(g, n, g->cp.get(), g->ri.get(), g->ac.get(), g->Hr[l].get(), g->Zc[l].get(), false);
+        dense(g, n, g->Hc[l - 1].get(), WT(g, l, 2), g->Zc[l].get(), WT(g, l, 3), P + kOff.bc[l], g->Hc[l].get(), true,
+              false);
+    }
+    if (n > 0)
+        hipLaunchKernelGGL(k_head, dim3(cdiv(n, 256)), dim3(256), 0, g->s, n, g->Hc[2].get(), P + kOff.wx, P + kOff.bx,
+                           g->xhat.get());
+    if (m > 0)
+        hipLaunchKernelGGL(k_head, dim3(cdiv(m, 256)), dim3(256), 0, g->s, m, g->Hr[2].get(), P + kOff.wy, P + kOff.by,
+                           g->yhat.get());
+    PM_LAUNCHED();
+}
+
+// Loss and gradient for targets already in g->tx / g->ty.
+float backward(pm_gnn* g) {
+    const int m = g->m, n = g->n;
+    const float* P = g->p.get();
+    float* part = g->part.get();
+    const int nbx = cdiv(std::max(n, 1), 256), nby = cdiv(std::max(m, 1), 256);
+    // MSE residuals; loss partials at the end of `part` (kept apart from
+    // the gradient partials by summing them first).
+    if (n > 0) hipLaunchKernelGGL(k_mse, dim3(nbx), dim3(256), 0, g->s, n, g->xhat.get(), g->tx.get(), g->dx.get(), part);
+    if (m > 0)
+        hipLaunchKernelGGL(k_mse, dim3(nby), dim3(256), 0, g->s, m, g->yhat.get(), g->ty.get(), g->dy.get(),
+                           part + (n > 0 ? nbx : 0));
+    PM_LAUNCHED();
+    float* loss_dev = g->grad.get() + kOff.total;  // one spare slot after the gradient
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g->s, (n > 0 ? nbx : 0) + (m > 0 ? nby : 0), 1, part,
+                       loss_dev);
+    PM_LAUNCHED();
+    head_grads(g, n, g->dx.get(), g->Hc[2].get(), kOff.wx, kOff.bx);
+    head_grads(g, m, g->dy.get(), g->Hr[2].get(), kOff.wy, kOff.by);
+    float* dHc = g->dHc.get();
+    float* dHr = g->dHr.get();
+    float* dHc2 = g->dHc2.get();
+    float* dZ = g->dZ.get();
+    if (n > 0) {
+        hipLaunchKernelGGL(k_head_bwd, dim3(cdiv(static_cast<long long>(n) * kH, 256)), dim3(256), 0, g->s, n,
+                           g->dx.get(), P + kOff.wx, dHc);
+        PM_LAUNCHED();
+    }
+    if (m > 0) {
+        hipLaunchKernelGGL(k_head_bwd, dim3(cdiv(static_cast<long long>(m) * kH, 256)), dim3(256), 0, g->s, m,
+                           g->dy.get(), P + kOff.wy, dHr);
+        PM_LAUNCHED();
+    }
+    for (int l = 2; l >= 1; --l) {
+        // column update of layer l: dHc holds dL/dHc(l)
+        relu_bwd(g, n, g->Hc[l].get(), dHc);
+        wgrad(g, n, g->Hc[l - 1].get(), dHc, kOff.Wcc[l]);
+        wgrad(g, n, g->Zc[l].get(), dHc, kOff.Wcz[l]);
+        colsum(g, n, dHc, nullptr, kOff.bc[l], 0);
+        dense(g, n, dHc, WN(g, l, 2), nullptr, nullptr, nullptr, dHc2, false, false);  // dHc(l-1)
+        dense(g, n, dHc, WN(g, l, 3), nullptr, nullptr, nullptr, dZ, false, false);    // dZc
+        spmm(g, m, g->rp.get(), g->ci.get(), g->ac_t.get(), dZ, dHr, true);            // dHr += Ac^T dZc
+        // row update of layer l: dHr holds dL/dHr(l)
+        relu_bwd(g, m, g->Hr[l].get(), dHr);
+        wgrad(g, m, g->Hr[l - 1].get(), dHr, kOff.Wrr[l]);
+        wgrad(g, m, g->Zr[l].get(), dHr, kOff.Wrz[l]);
+        colsum(g, m, dHr, nullptr, kOff.br[l], 0);
+        dense(g, m, dHr, WN(g, l, 1), nullptr, nullptr, nullptr, dZ, false, false);    // dZr
+        dense(g, m, dHr, WN(g, l, 0), nullptr, nullptr, nullptr, g->dT.get(), false, false);  // dHr(l-1)
+        spmm(g, n, g->cp.get(), g->ri.get(), g->ar_t.get(), dZ, dHc2, true);            // dHc(l-1) += Ar^T dZr
+        std::swap(g->dHc, g->dHc2);
+        std::swap(g->dHr, g->dT);
+        dHc = g->dHc.get();
+        dHc2 = g->dHc2.get();
+        dHr = g->dHr.get();
+    }
+    relu_bwd(g, n, g->Hc[0].get(), dHc);
+    colsum(g, n, dHc, g->xc.get(), kOff.be_c, kOff.We_c);
+    relu_bwd(g, m, g->Hr[0].get(), dHr);
+    colsum(g, m, dHr, g->xr.get(), kOff.be_r, kOff.We_r);
+    float loss = 0.0f;
+    PM_HIP(hipMemcpyAsync(&loss, loss_dev, sizeof(float), hipMemcpyDeviceToHost, g->s));
+    PM_HIP(hipStreamSynchronize(g->s));
+    return loss;
+}
+
+}  // namespace
+}  // namespace pm
+
+extern "C" {
+
+int32_t pm_gnn_num_params(void) { return pm::kOff.total; }
+
+int pm_gnn_create(int32_t device, uint64_t seed, pm_gnn** out) {
+    return pm::guarded([&] {
+        using namespace pm;
+        if (!out) fail(PM_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        int nd = 0;
+        if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) {
+            (void)hipGetLastError();
+            fail(PM_ERR_NO_DEVICE, "no HIP device visible");
+        }
+        if (device < 0 || device >= nd) fail(PM_ERR_NO_DEVICE, "device ordinal out of range");
+        PM_HIP(hipSetDevice(device));
+        auto g = std::make_unique<pm_gnn>();
+        g->device = device;
+        PM_HIP(hipStreamCreateWithFlags(&g->s, hipStreamNonBlocking));
+        // Scaled uniform init U(-1/sqrt(fan_in), 1/sqrt(fan_in)) from SplitMix64.
+        std::vector<float> h(kOff.total, 0.0f);
+        uint64_t z = seed * 0x9E3779B97F4A7C15ull + 1;
+        auto uni = [&]() {
+            z += 0x9E3779B97F4A7C15ull;
+            uint64_t t = z;
+            t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
+            t = (t ^ (t >> 27)) * 0x94D049BB133111EBull;
+            t ^= t >> 31;
+            return static_cast<double>(t >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+        };
+        auto init = [&](int off, int count, int fan_in) {
+            const double a = 1.0 / std::sqrt(static_cast<double>(fan_in));
+            for (int i = 0; i < count; ++i) h[off + i] = static_cast<float>(a * uni());
+        };
+        init(kOff.We_c, kF * kH, kF);
+        init(kOff.We_r, kF * kH, kF);
+        for (int l = 1; l <= 2; ++l) {
+            init(kOff.Wrr[l], kH * kH, 2 * kH);
+            init(kOff.Wrz[l], kH * kH, 2 * kH);
+            init(kOff.Wcc[l], kH * kH, 2 * kH);
+            init(kOff.Wcz[l], kH * kH, 2 * kH);
+        }
+        init(kOff.wx, kH, kH);
+        init(kOff.wy, kH, kH);
+        g->p.upload(h, g->s);
+        g->grad.resize(kOff.total + 1);
+        g->m1.resize(kOff.total);
+        g->m2.resize(kOff.total);
+        fill(g.get(), g->m1.get(), kOff.total, 0.0f);
+        fill(g.get(), g->m2.get(), kOff.total, 0.0f);
+        g->wt.resize(8 * kH * kH);
+        g->wn.resize(8 * kH * kH);
+        prep_weights(g.get());
+        PM_HIP(hipStreamSynchronize(g->s));
+        *out = g.release();
+    });
+}
+
+int pm_gnn_destroy(pm_gnn* g) {
+    return pm::guarded([&] {
+        if (!g) return;
+        (void)hipSetDevice(g->device);
+        if (g->s) (void)hipStreamSynchronize(g->s);
+        hipStream_t s = g->s;
+        delete g;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int pm_gnn_get_params(pm_gnn* g, float* out) {
+    return pm::guarded([&] {
+        if (!g || !out) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
+        PM_HIP(hipSetDevice(g->device));
+        g->p.download(out, pm::kOff.total, g->s);
+        PM_HIP(hipStreamSynchronize(g->s));
+    });
+}
+
+int pm_gnn_set_params(pm_gnn* g, const float* in) {
+    return pm::guarded([&] {
+        using namespace pm;
+        if (!g || !in) fail(PM_ERR_INVALID_ARG, "NULL argument");
+        PM_HIP(hipSetDevice(g->device));
+        g->p.upload(in, kOff.total, g->s);
+        fill(g, g->m1.get(), kOff.total, 0.0f);
+        fill(g, g->m2.get(), kOff.total, 0.0f);
+        g->step = 0;
+        prep_weights(g);
+        PM_HIP(hipStreamSynchronize(g->s));
+    });
+}
+
+int pm_gnn_set_graph(pm_gnn* g, int32_t m, int32_t n, int32_t nnz, const int32_t* rp, const int32_t* ci,
+                     const float* ar, const int32_t* cp, const int32_t* ri, const float* ac, const float* xr,
+                     const float* xc) {
+    return pm::guarded([&] {
+        using namespace pm;
+        if (!g || m < 0 || n < 0 || nnz < 0 || !rp || !cp) fail(PM_ERR_INVALID_ARG, "bad graph");
+        if (nnz > 0 && (!ci || !ar || !ri || !ac)) fail(PM_ERR_INVALID_ARG, "NULL graph arrays");
+        if ((m > 0 && !xr) || (n > 0 && !xc)) fail(PM_ERR_INVALID_ARG, "NULL features");
+        if (rp[0] != 0 || rp[m] != nnz || cp[0] != 0 || cp[n] != nnz) fail(PM_ERR_INVALID_ARG, "CSR inconsistent");
+        for (int i = 0; i < m; ++i)
+            if (rp[i] > rp[i + 1]) fail(PM_ERR_INVALID_ARG, "rp decreasing");
+        for (int j = 0; j < n; ++j)
+            if (cp[j] > cp[j + 1]) fail(PM_ERR_INVALID_ARG, "cp decreasing");
+        for (int k = 0; k < nnz; ++k)
+            if (ci[k] < 0 || ci[k] >= n || ri[k] < 0 || ri[k] >= m) fail(PM_ERR_INVALID_ARG, "index out of range");
+        PM_HIP(hipSetDevice(g->device));
+        g->m = m;
+        g->n = n;
+        g->nnz = nnz;
+        hipStream_t s = g->s;
+        g->rp.upload(rp, static_cast<size_t>(m) + 1, s);
+        g->cp.upload(cp, static_cast<size_t>(n) + 1, s);
+        g->ci.upload(ci, nnz, s);
+        g->ri.upload(ri, nnz, s);
+        g->ar.upload(ar, nnz, s);
+        g->ac.upload(ac, nnz, s);
+        // Transposed-order copies of the aggregation values for the backward
+        // pass: ar_t (Ar^T, CSR over columns) and ac_t (Ac^T, CSR over rows).
+        std::vector<float> ar_t(nnz), ac_t(nnz);
+        {
+            std::vector<int> cur(cp, cp + n);
+            for (int i = 0; i < m; ++i)
+                for (int k = rp[i]; k < rp[i + 1]; ++k) ar_t[cur[ci[k]]++] = ar[k];
+            std::vector<int> cur2(rp, rp + m);
+            for (int j = 0; j < n; ++j)
+                for (int k = cp[j]; k < cp[j + 1]; ++k) ac_t[cur2[ri[k]]++] = ac[k];
+        }
+        g->ar_t.upload(ar_t, s);
+        g->ac_t.upload(ac_t, s);
+        g->xr.upload(xr, static_cast<size_t>(m) * kF, s);
+        g->xc.upload(xc, static_cast<size_t>(n) * kF, s);
+        const size_t mr = std::max(m, 1), nc = std::max(n, 1);
+        for (int l = 0; l < 3; ++l) {
+            g->Hc[l].resize(nc * kH);
+            g->Hr[l].resize(mr * kH);
+            g->Zr[l].resize(mr * kH);
+            g->Zc[l].resize(nc * kH);
+        }
+        g->dHc.resize(nc * kH);
+        g->dHc2.resize(nc * kH);
+        g->dHr.resize(mr * kH);
+        g->dZ.resize(std::max(mr, nc) * kH);
+        g->dT.resize(mr * kH);
+        for (auto* b : {&g->xhat, &g->dx, &g->tx}) b->resize(nc);
+        for (auto* b : {&g->yhat, &g->dy, &g->ty}) b->resize(mr);
+        const size_t nb = static_cast<size_t>(cdiv(std::max(m, n), kRowsPer));
+        g->part.resize(std::max(nb * kH * kH, static_cast<size_t>(cdiv(std::max(m, 1), 256) + cdiv(std::max(n, 1), 256))) +
+                       nb * (kF + 1) * kH + 64);
+        PM_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int pm_gnn_forward(pm_gnn* g, float* xhat, float* yhat) {
+    return pm::guarded([&] {
+        if (!g) pm::fail(PM_ERR_INVALID_ARG, "NULL network");
+        PM_HIP(hipSetDevice(g->device));
+        pm::forward(g);
+        if (xhat && g->n) g->xhat.download(xhat, g->n, g->s);
+        if (yhat && g->m) g->yhat.download(yhat, g->m, g->s);
+        PM_HIP(hipStreamSynchronize(g->s));
+    });
+}
+
+int pm_gnn_gradient(pm_gnn* g, const float* tx, const float* ty, float* loss, float* grad) {
+    return pm::guarded([&] {
+        if (!g || !tx || !ty) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
+        PM_HIP(hipSetDevice(g->device));
+        if (g->n) g->tx.upload(tx, g->n, g->s);
+        if (g->m) g->ty.upload(ty, g->m, g->s);
+        pm::forward(g);
+        const float l = pm::backward(g);
+        if (loss) *loss = l;
+        if (grad) g->grad.download(grad, pm::kOff.total, g->s);
+        PM_HIP(hipStreamSynchronize(g->s));
+    });
+}
+
+int pm_gnn_train_step(pm_gnn* g, const float* tx, const float* ty, float lr, float* loss) {
+    return pm::guarded([&] {
+        using namespace pm;
+        if (!g || !tx || !ty) fail(PM_ERR_INVALID_ARG, "NULL argument");
+        PM_HIP(hipSetDevice(g->device));
+        if (g->n) g->tx.upload(tx, g->n, g->s);
+        if (g->m) g->ty.upload(ty, g->m, g->s);
+        forward(g);
+        const float l = backward(g);
+        ++g->step;
+        const float c1 = 1.0f - std::pow(0.9f, static_cast<float>(g->step));
+        const float c2 = 1.0f - std::pow(0.999f, static_cast<float>(g->step));
+        hipLaunchKernelGGL(k_adam, dim3(cdiv(kOff.total, 256)), dim3(256), 0, g->s, kOff.total, g->p.get(),
+                           g->grad.get(), g->m1.get(), g->m2.get(), lr, c1, c2);
+        PM_LAUNCHED();
+        prep_weights(g);
+        PM_HIP(hipStreamSynchronize(g->s));
+        if (loss) *loss = l;
+    });
+}
+
+}  // extern "C"

@@ -222,4 +222,28 @@ int pm_gen_view(const pm_gen_lp* g, pm_lp* lp, const double** x_star, const doub
 
 void pm_gen_free(pm_gen_lp* g) { delete g; }
 
+// Internal: wraps LP arrays (no known optimum) in a pm_gen_lp.
+pm_gen_lp* pm__gen_lp_from_arrays(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, const double* v,
+                                  const double* c, const double* rl, const double* ru, const double* xl,
+                                  const double* xu) {
+    try {
+        auto g = std::make_unique<pm_gen_lp>();
+        g->m = m;
+        g->n = n;
+        const int nnz = rp[m];
+        g->rp.assign(rp, rp + m + 1);
+        g->ci.assign(ci, ci + nnz);
+        g->v.assign(v, v + nnz);
+        g->c.assign(c, c + n);
+        g->rl.assign(rl, rl + m);
+        g->ru.assign(ru, ru + m);
+        g->xl.assign(xl, xl + n);
+        g->xu.assign(xu, xu + n);
+        g->pstar = NAN;
+        return g.release();
+    } catch (...) {
+        return nullptr;
+    }
+}
+
 }  // extern "C"
