@@ -4,12 +4,23 @@
 // objectives and infeasible/unbounded verdicts of the reference's
 // RevisedSimplex (src/simplex/revised_simplex.cpp:484-680) on small LPs.
 //
-//   ref_driver simplex <model.mps>   -> "status objective iterations"
+//   ref_driver simplex <model.mps> [x.out] -> "status objective iterations"
+//                                    (and x, one value per line, to x.out)
 //   ref_driver ruiz <model.mps> <passes> -> "sum_row_scale sum_col_scale sum|A|"
+//   ref_driver cert <model.mps> <x.txt> <status> <tolerance> <out.cert>
+//       the reference's generate_certificate + write_certificate
+//       (src/certificate/certificate.cpp:23-153) for the given x.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <limits>
 #include <string>
+#include <vector>
+
+#include "pramaan/certificate.hpp"
 
 #include "pramaan/mps_parser.hpp"
 #include "pramaan/presolve.hpp"
@@ -27,6 +38,11 @@ int main(int argc, char** argv) {
             pramaan::RevisedSimplex solver;
             const pramaan::SolveResult r = solver.solve(model);
             std::printf("%d %.17g %d\n", static_cast<int>(r.status), r.objective_value, r.iterations);
+            if (argc > 3) {
+                std::ofstream out(argv[3]);
+                out << std::setprecision(std::numeric_limits<double>::max_digits10);
+                for (double v : r.x) out << v << "\n";
+            }
             return 0;
         }
         if (std::strcmp(argv[1], "ruiz") == 0) {
@@ -38,6 +54,15 @@ int main(int argc, char** argv) {
             for (double v : f.col_scale) sc += v;
             for (double v : model.A.values()) sa += std::fabs(v);
             std::printf("%.17g %.17g %.17g\n", sr, sc, sa);
+            return 0;
+        }
+        if (std::strcmp(argv[1], "cert") == 0 && argc > 6) {
+            std::vector<double> x;
+            std::ifstream in(argv[3]);
+            for (double v; in >> v;) x.push_back(v);
+            pramaan::Certificate c = pramaan::generate_certificate(model, x, argv[4], 0);
+            c.tolerance = std::strtod(argv[5], nullptr);
+            pramaan::write_certificate(c, argv[6]);
             return 0;
         }
     } catch (const std::exception& e) {

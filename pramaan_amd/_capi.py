@@ -452,6 +452,48 @@ class Gnn:
             pass
 
 
+CXX_LIB_PATH = Path(__file__).resolve().parent / "lib" / "libpramaan_cxx.so"
+_cxx = None
+
+
+def cxx() -> C.CDLL:
+    """Loads the C++ interface library (certificates; include/pramaan_cert.h)."""
+    global _cxx
+    if _cxx is None:
+        if not CXX_LIB_PATH.exists():
+            raise ImportError(f"{CXX_LIB_PATH} is missing: build it with `python -m pramaan_amd.build`")
+        L = C.CDLL(str(CXX_LIB_PATH))
+        L.pm_cert_write.restype = C.c_int
+        L.pm_cert_write.argtypes = [C.POINTER(pm_lp), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_char_p,
+                                    C.c_double, C.c_char_p, C.c_char_p, C.c_int]
+        L.pm_cert_verify.restype = C.c_int
+        L.pm_cert_verify.argtypes = [C.POINTER(pm_lp), C.c_char_p, C.c_char_p, C.c_int]
+        _cxx = L
+    return _cxx
+
+
+def cert_write(lp: LP, x, path, y=None, status="OPTIMAL", tolerance=1e-6) -> None:
+    """Writes a V1 (y None) or V2 certificate; raises ValueError on error."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if len(x) != lp.n or (y is not None and len(y) != lp.m):
+        raise ValueError("certificate: x or y has the wrong length")
+    y = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    err = C.create_string_buffer(1024)
+    s = lp.struct()
+    rc = cxx().pm_cert_write(C.byref(s), _p(x, C.c_double), _p(y, C.c_double), status.encode(), float(tolerance),
+                             str(path).encode(), err, len(err))
+    if rc != 0:
+        raise ValueError(err.value.decode())
+
+
+def cert_verify(lp: LP, path) -> tuple[int, list[str]]:
+    """(0 valid | 1 invalid | 2 unreadable, failure messages)."""
+    msg = C.create_string_buffer(1 << 16)
+    s = lp.struct()
+    rc = cxx().pm_cert_verify(C.byref(s), str(path).encode(), msg, len(msg))
+    return rc, [l for l in msg.value.decode().split("\n") if l]
+
+
 def _csr32(row_ptr, col_idx, values):
     return (np.ascontiguousarray(row_ptr, np.int32), np.ascontiguousarray(col_idx, np.int32),
             np.ascontiguousarray(values, np.float32))
@@ -545,6 +587,7 @@ def generate(which: int | None = None, seed: int = 1, **params) -> LP:
         _check(lib().pm_gen_params_named(which, seed, C.byref(p)))
     else:
         p.support, p.var_upper, p.eq_frac, p.seed = 0.3, 10.0, 1.0, seed
+        p.k_min, p.k_max = 4, 16  # nonzeros per column (clamped to m)
     for k, v in params.items():
         setattr(p, k, v)
     h = C.c_void_p()

@@ -2,16 +2,20 @@
 //
 // Layout: every LP's A and A^T (CSR, local indices) and vectors are packed
 // back to back in shared device arrays; a per-LP descriptor holds the
-// offsets.  The kernel runs one 256-thread workgroup per LP, largest LP
-// first (blockIdx -> LP through `order`), so the hardware dispatcher hands
-// the next LP to whichever CU frees up.  Inside a workgroup the LP is solved
-// end to end: power iteration for ||A_hat||_2, initial primal weight,
-// refinement rounds, reflected Halpern iterations (the same epilogue
-// functors as the single-LP kernels, run as block-strided row loops with
-// L lanes per row), FP64 KKT checks with deterministic block reductions,
-// restarts and primal-weight updates.  Decisions come from pm::Control, the
-// same code the single-LP host loop runs.  Nothing is allocated or cleared
-// by the host between runs, so the launch is captured once in a HIP graph.
+// offsets.  Every phase kernel runs one 1024-thread workgroup per LP,
+// largest LP first (blockIdx -> LP through `order`), so the hardware
+// dispatcher hands the next LP to whichever CU frees up, and an LP that has
+// converged returns at once.  Phases: init (power iteration for ||A_hat||_2,
+// initial primal weight, base points), round (correction-LP data of a
+// refinement round), iterate (check_frequency reflected Halpern iterations:
+// the same epilogue functors as the single-LP kernels, run as block-strided
+// row loops with L lanes per row) and check (FP64 KKT with deterministic
+// block reductions, then the pm::Control decision -- the same code the
+// single-LP host loop runs -- and the restart / refinement it asks for).
+// Per-LP solver state lives in device memory between phases.  Four
+// round/iterate/check chunks are captured once in a HIP graph; the host
+// replays it until a device counter reports no active LP.  Nothing is
+// allocated or cleared with a memset between replays.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -28,6 +32,12 @@
 
 namespace pm {
 namespace {
+
+// Workgroup size of the batch kernel: 16 wavefronts per LP, so one LP keeps
+// enough loads in flight to stream its data from HBM.
+constexpr int kBB = 1024;
+// Chunks of check_frequency iterations per graph replay.
+constexpr int kChunksPerGraph = 4;
 
 struct BatchDesc {
     int32_t m, n, nnz;
@@ -66,7 +76,7 @@ struct BatchArrays {
 
 template <int NS, int NM>
 __device__ void block_reduce_all(double (&acc)[NS + NM], double* out) {
-    __shared__ double tmp[kBlock / 64][16];
+    __shared__ double tmp[kBB / 64][16];
 #pragma unroll
     for (int j = 0; j < NS + NM; ++j) {
         double a = acc[j];
@@ -86,7 +96,7 @@ __device__ void block_reduce_all(double (&acc)[NS + NM], double* out) {
     if (threadIdx.x < NS + NM) {
         const int j = threadIdx.x;
         double a = tmp[0][j];
-        for (int w = 1; w < kBlock / 64; ++w) a = (j < NS) ? a + tmp[w][j] : (tmp[w][j] > a ? tmp[w][j] : a);
+        for (int w = 1; w < kBB / 64; ++w) a = (j < NS) ? a + tmp[w][j] : (tmp[w][j] > a ? tmp[w][j] : a);
         out[j] = a;
     }
     __syncthreads();
@@ -95,7 +105,7 @@ __device__ void block_reduce_all(double (&acc)[NS + NM], double* out) {
 template <int L, class AT, class VT, class XT, class Epi>
 __device__ __forceinline__ void brows(int nrows, const int* rp, const int* ci, const VT* v, const XT* x,
                                       const Epi& epi) {
-    constexpr int G = kBlock / L;
+    constexpr int G = kBB / L;
     const int g = threadIdx.x / L, lane = threadIdx.x & (L - 1);
     for (int r0 = 0; r0 < nrows; r0 += G) {
         const int row = r0 + g;
@@ -125,7 +135,7 @@ __device__ void brows_any(int L, int nrows, const int* rp, const int* ci, const 
 template <int L, class VT, class XT, class Epi>
 __device__ __forceinline__ void brows_red(int nrows, const int* rp, const int* ci, const VT* v, const XT* x,
                                           const Epi& epi, double (&acc)[Epi::NS + Epi::NM]) {
-    constexpr int G = kBlock / L;
+    constexpr int G = kBB / L;
     const int g = threadIdx.x / L, lane = threadIdx.x & (L - 1);
     for (int r0 = 0; r0 < nrows; r0 += G) {
         const int row = r0 + g;
@@ -143,7 +153,7 @@ __device__ __forceinline__ void brows_red(int nrows, const int* rp, const int* c
 }
 
 template <class VT, class XT, class Epi>
-__device__ void brows_red_any(int L, int nrows, const int* rp, const int* ci, const VT* v, const XT* x,
+__device__ __attribute__((noinline)) void brows_red_any(int L, int nrows, const int* rp, const int* ci, const VT* v, const XT* x,
                               const Epi& epi, double* out) {
     double acc[Epi::NS + Epi::NM];
 #pragma unroll
@@ -171,15 +181,8 @@ struct View {
     double *xtot, *ytot, *xbase, *ybase;
 };
 
-struct SqDiff {
-    static constexpr int NS = 1, NM = 0;
-};
-
 template <class VT>
-__global__ __launch_bounds__(kBlock) void k_batch(BatchArrays<VT> A) {
-    const int lp = A.order[blockIdx.x];
-    BatchDesc& d = A.desc[lp];
-    const uint64_t t_start = wall_clock64();
+__device__ __forceinline__ View<VT> view_of(const BatchArrays<VT>& A, const BatchDesc& d) {
     View<VT> v;
     v.m = d.m;
     v.n = d.n;
@@ -202,15 +205,30 @@ __global__ __launch_bounds__(kBlock) void k_batch(BatchArrays<VT> A) {
     v.lr32 = A.lr32 + ro; v.ur32 = A.ur32 + ro; v.g32 = A.g32 + ro;
     v.y = A.y + ro; v.y0 = A.y0 + ro; v.yt = A.yt + ro; v.yl = A.yl + ro;
     v.ytot = A.ytot + ro; v.ybase = A.ybase + ro;
+    return v;
+}
+
+// Solver state of one LP between the phase kernels.
+struct BatchState {
+    Control ctl;
+    double eta;
+    int32_t status;
+    int32_t active;       // 1 while the LP is being solved
+    int32_t round_start;  // 1: the next round-data phase must run
+    int32_t have_cand;    // a candidate T(z) exists in xtot / ytot
+    int32_t last_chunk;   // iterations the last iterate phase ran
+};
+
+// Phase 0: power iteration for ||A_hat||_2, initial primal weight, base points.
+template <class VT>
+__global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S) {
+    const int lp = A.order[blockIdx.x];
+    BatchDesc& d = A.desc[lp];
+    const View<VT> v = view_of(A, d);
     const int m = v.m, n = v.n, tid = threadIdx.x;
-
-    __shared__ Control ctl;
     __shared__ double red[16];
-    __shared__ double eta_s, kkt_s[8];
-    __shared__ int action_s;
-
-    // ---- power iteration for ||A_hat||_2 (v in xt, u in yt, w in xe) ----
-    for (int j = tid; j < n; j += kBlock) v.xt[j] = rsqrtf(static_cast<float>(n));
+    const uint64_t t0 = wall_clock64();
+    for (int j = tid; j < n; j += kBB) v.xt[j] = rsqrtf(static_cast<float>(n));
     __syncthreads();
     double lam = 0.0;
     if (m > 0 && d.nnz > 0) {
@@ -220,169 +238,213 @@ __global__ __launch_bounds__(kBlock) void k_batch(BatchArrays<VT> A) {
             brows_any<float>(v.lat, n, v.cp, v.ri, v.vat, v.yt, EpiStore<float>{v.xe});
             __syncthreads();
             double acc[1] = {0.0};
-            for (int j = tid; j < n; j += kBlock) acc[0] += static_cast<double>(v.xe[j]) * v.xe[j];
+            for (int j = tid; j < n; j += kBB) acc[0] += static_cast<double>(v.xe[j]) * v.xe[j];
             block_reduce_all<1, 0>(acc, red);
             lam = red[0];
             const double inv = lam > 0.0 ? 1.0 / sqrt(lam) : 0.0;
-            for (int j = tid; j < n; j += kBlock) v.xt[j] = static_cast<float>(v.xe[j] * inv);
+            for (int j = tid; j < n; j += kBB) v.xt[j] = static_cast<float>(v.xe[j] * inv);
             __syncthreads();
         }
     }
-    // ---- initial primal weight ||D_c c|| / ||D_r b||, base points ----
-    {
-        double acc[2] = {0.0, 0.0};
-        for (int j = tid; j < n; j += kBlock) {
-            const double t = v.dc[j] * v.cmin[j];
-            acc[0] += t * t;
-            double xb = d.warm_x ? v.wx[j] : 0.0;
-            if (d.warm_x) xb = xb < v.lx[j] ? v.lx[j] : (xb > v.ux[j] ? v.ux[j] : xb);
-            v.xbase[j] = xb;
-        }
-        for (int i = tid; i < m; i += kBlock) {
-            double bh = 0.0;
-            if (v.lr[i] > -kInf) bh = fabs(v.lr[i]);
-            if (v.ur[i] < kInf && fabs(v.ur[i]) > bh) bh = fabs(v.ur[i]);
-            const double t = v.dr[i] * bh;
-            acc[1] += t * t;
-            v.ybase[i] = d.warm_y ? v.wy[i] : 0.0;
-        }
-        block_reduce_all<2, 0>(acc, red);
-        if (tid == 0) {
-            const double om = (red[0] > 1e-20 && red[1] > 1e-20) ? sqrt(red[0]) / sqrt(red[1]) : 1.0;
-            ctl.init(A.tol, A.max_iterations, A.max_refinements, om, d.warm_y != 0 && m > 0);
-            eta_s = lam > 0.0 ? 0.99 / sqrt(sqrt(lam)) : 1.0;
-        }
+    double acc[2] = {0.0, 0.0};
+    for (int j = tid; j < n; j += kBB) {
+        const double t = v.dc[j] * v.cmin[j];
+        acc[0] += t * t;
+        double xb = d.warm_x ? v.wx[j] : 0.0;
+        if (d.warm_x) xb = xb < v.lx[j] ? v.lx[j] : (xb > v.ux[j] ? v.ux[j] : xb);
+        v.xbase[j] = xb;
+    }
+    for (int i = tid; i < m; i += kBB) {
+        double bh = 0.0;
+        if (v.lr[i] > -kInf) bh = fabs(v.lr[i]);
+        if (v.ur[i] < kInf && fabs(v.ur[i]) > bh) bh = fabs(v.ur[i]);
+        const double t = v.dr[i] * bh;
+        acc[1] += t * t;
+        v.ybase[i] = d.warm_y ? v.wy[i] : 0.0;
+    }
+    block_reduce_all<2, 0>(acc, red);
+    if (tid == 0) {
+        BatchState& st = S[lp];
+        const double om = (red[0] > 1e-20 && red[1] > 1e-20) ? sqrt(red[0]) / sqrt(red[1]) : 1.0;
+        st.ctl.init(A.tol, A.max_iterations, A.max_refinements, om, d.warm_y != 0 && m > 0);
+        st.eta = lam > 0.0 ? 0.99 / sqrt(sqrt(lam)) : 1.0;
+        st.status = PM_SOLVE_ITERATION_LIMIT;
+        st.active = 1;
+        st.round_start = 1;
+        st.have_cand = 0;
+        st.last_chunk = 0;
+        d.t_start = t0;
+    }
+}
+
+// Phase 1: correction-LP data of a new refinement round, iterates reset.
+template <class VT>
+__global__ __launch_bounds__(kBB) void k_b_round(BatchArrays<VT> A, BatchState* S) {
+    const int lp = A.order[blockIdx.x];
+    BatchState& st = S[lp];
+    if (!st.active || !st.round_start) return;
+    const BatchDesc& d = A.desc[lp];
+    const View<VT> v = view_of(A, d);
+    const int m = v.m, n = v.n, tid = threadIdx.x;
+    const double dp = st.ctl.DP, dd = st.ctl.DD;
+    brows_any<double>(v.lat, n, v.cp, v.ri, v.oat, v.ybase,
+                      EpiRoundCol{v.cmin, v.lx, v.ux, v.xbase, v.dc, dp, dd, v.c32, v.lx32, v.ux32});
+    brows_any<double>(v.la, m, v.rp, v.ci, v.oa, v.xbase,
+                      EpiRoundRow{v.lr, v.ur, v.ybase, v.dr, dp, dd, v.lr32, v.ur32, v.g32});
+    __syncthreads();
+    for (int j = tid; j < n; j += kBB) {
+        const float x0v = fmaxf(v.lx32[j], fminf(0.0f, v.ux32[j]));
+        v.x[j] = x0v;
+        v.x0[j] = x0v;
+        v.xl[j] = x0v;
+    }
+    for (int i = tid; i < m; i += kBB) v.y[i] = v.y0[i] = v.yl[i] = 0.0f;
+    __syncthreads();
+    if (tid == 0) st.round_start = 0;
+}
+
+// Phase 2: up to K reflected-Halpern iterations; T(z) of the last one kept.
+template <class VT>
+__global__ __launch_bounds__(kBB) void k_b_iter(BatchArrays<VT> A, BatchState* S, int K) {
+    const int lp = A.order[blockIdx.x];
+    BatchState& st = S[lp];
+    if (!st.active) return;
+    const BatchDesc& d = A.desc[lp];
+    const View<VT> v = view_of(A, d);
+    const int chunk = st.ctl.chunk(K), k0 = st.ctl.k;
+    const double om = st.ctl.omega;
+    const float tau = static_cast<float>(st.eta / om), sig = static_cast<float>(st.eta * om);
+    const bool use_g = st.ctl.use_g;
+    for (int i = 0; i < chunk; ++i) {
+        const float lamk = Control::lambda(k0 + i);
+        const bool keep = i == chunk - 1;
+        brows_any<float>(v.lat, v.n, v.cp, v.ri, v.vat, v.y,
+                         EpiPrimalHal{tau, lamk, 1.0f, v.c32, v.lx32, v.ux32, v.x0, v.x, v.xe, keep ? v.xt : nullptr});
+        __syncthreads();
+        brows_any<float>(v.la, v.m, v.rp, v.ci, v.va, v.xe,
+                         EpiDualHal{sig, lamk, 1.0f, v.lr32, v.ur32, use_g ? v.g32 : nullptr, v.y0, v.y,
+                                    keep ? v.yt : nullptr});
         __syncthreads();
     }
-    const double eta = eta_s;
-    bool round_start = true, have_cand = false;
-    int status = PM_SOLVE_ITERATION_LIMIT;
-    for (;;) {
-        if (round_start) {
-            const double dp = ctl.DP, dd = ctl.DD;
-            brows_any<double>(v.lat, n, v.cp, v.ri, v.oat, v.ybase,
-                              EpiRoundCol{v.cmin, v.lx, v.ux, v.xbase, v.dc, dp, dd, v.c32, v.lx32, v.ux32});
-            brows_any<double>(v.la, m, v.rp, v.ci, v.oa, v.xbase,
-                              EpiRoundRow{v.lr, v.ur, v.ybase, v.dr, dp, dd, v.lr32, v.ur32, v.g32});
-            __syncthreads();
-            for (int j = tid; j < n; j += kBlock) {
-                const float x0v = fmaxf(v.lx32[j], fminf(0.0f, v.ux32[j]));
-                v.x[j] = x0v;
-                v.x0[j] = x0v;
-                v.xl[j] = x0v;
-            }
-            for (int i = tid; i < m; i += kBlock) v.y[i] = v.y0[i] = v.yl[i] = 0.0f;
-            __syncthreads();
-            round_start = false;
-        }
-        const int chunk = ctl.chunk(A.check_frequency);
-        if (chunk <= 0) break;
-        const double om = ctl.omega;
-        const float tau = static_cast<float>(eta / om), sig = static_cast<float>(eta * om);
-        const bool use_g = ctl.use_g;
-        const int k0 = ctl.k;
-        for (int i = 0; i < chunk; ++i) {
-            const float lamk = Control::lambda(k0 + i);
-            const bool keep = i == chunk - 1;
-            brows_any<float>(v.lat, n, v.cp, v.ri, v.vat, v.y,
-                             EpiPrimalHal{tau, lamk, 1.0f, v.c32, v.lx32, v.ux32, v.x0, v.x, v.xe,
-                                          keep ? v.xt : nullptr});
-            __syncthreads();
-            brows_any<float>(v.la, m, v.rp, v.ci, v.va, v.xe,
-                             EpiDualHal{sig, lamk, 1.0f, v.lr32, v.ur32, use_g ? v.g32 : nullptr, v.y0, v.y,
-                                        keep ? v.yt : nullptr});
-            __syncthreads();
-        }
-        // ---- FP64 KKT of the candidate T(z) as a point of the original LP ----
+    if (threadIdx.x == 0) st.last_chunk = chunk;
+}
+
+// FP64 relative KKT of (xtot, ytot) into the descriptor; returns rel on
+// thread 0 (every thread must call).
+template <class VT>
+__device__ double batch_kkt(const View<VT>& v, BatchDesc& d, double* red, double* keep) {
+    brows_red_any(v.la, v.m, v.rp, v.ci, v.oa, v.xtot, EpiKktRow{v.lr, v.ur, v.ytot, v.dr}, red);
+    if (threadIdx.x < 7) keep[threadIdx.x] = red[threadIdx.x];
+    __syncthreads();
+    brows_red_any(v.lat, v.n, v.cp, v.ri, v.oat, v.ytot, EpiKktCol{v.cmin, v.lx, v.ux, v.xtot, v.dc}, red);
+    double rel = 0.0;
+    if (threadIdx.x == 0) {
+        const double p = red[4], dd2 = keep[5] + red[3];
+        const double rp = sqrt(keep[0]) / (1.0 + sqrt(keep[2]));
+        const double rdd = sqrt(keep[3] + red[0]) / (1.0 + sqrt(red[2]));
+        const double rg = fabs(p - dd2) / (1.0 + fabs(p) + fabs(dd2));
+        rel = rp > rdd ? rp : rdd;
+        rel = rel > rg ? rel : rg;
+        if (!(rel == rel)) rel = INFINITY;
+        d.rel_kkt = rel; d.rel_p = rp; d.rel_d = rdd; d.rel_g = rg; d.pobj = p; d.dobj = dd2;
+    }
+    __syncthreads();
+    return rel;
+}
+
+// Phase 3: candidate -> FP64 KKT -> Control decision -> restart / refine /
+// finish.  Counts the LPs still active.
+template <class VT>
+__global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* S, int* active_count) {
+    const int lp = A.order[blockIdx.x];
+    BatchState& st = S[lp];
+    if (!st.active) return;
+    BatchDesc& d = A.desc[lp];
+    const View<VT> v = view_of(A, d);
+    const int m = v.m, n = v.n, tid = threadIdx.x;
+    __shared__ double red[16], keep[8];
+    __shared__ int act_s;
+    __shared__ Control ctl;
+    if (tid == 0) ctl = st.ctl;
+    __syncthreads();
+    const int chunk = st.last_chunk;
+    int act;
+    if (chunk > 0) {
         const double invp = 1.0 / ctl.DP, invd = 1.0 / ctl.DD;
-        for (int j = tid; j < n; j += kBlock) {
+        for (int j = tid; j < n; j += kBB) {
             double t = v.xbase[j] + v.dc[j] * static_cast<double>(v.xt[j]) * invp;
             t = t < v.lx[j] ? v.lx[j] : (t > v.ux[j] ? v.ux[j] : t);
             v.xtot[j] = t;
         }
-        for (int i = tid; i < m; i += kBlock) {
+        for (int i = tid; i < m; i += kBB) {
             double t = v.ybase[i] + v.dr[i] * static_cast<double>(v.yt[i]) * invd;
             if (!(v.ur[i] < kInf) && t > 0.0) t = 0.0;
             if (!(v.lr[i] > -kInf) && t < 0.0) t = 0.0;
             v.ytot[i] = t;
         }
         __syncthreads();
-        have_cand = true;
-        brows_red_any(v.la, m, v.rp, v.ci, v.oa, v.xtot, EpiKktRow{v.lr, v.ur, v.ytot, v.dr}, red);
-        if (tid < 7) kkt_s[tid] = red[tid];
-        __syncthreads();
-        brows_red_any(v.lat, n, v.cp, v.ri, v.oat, v.ytot, EpiKktCol{v.cmin, v.lx, v.ux, v.xtot, v.dc}, red);
+        const double rel = batch_kkt(v, d, red, keep);
         if (tid == 0) {
             ctl.advance(chunk);
-            const double p = red[4], dd2 = kkt_s[5] + red[3];
-            const double rp = sqrt(kkt_s[0]) / (1.0 + sqrt(kkt_s[2]));
-            const double rdd = sqrt(kkt_s[3] + red[0]) / (1.0 + sqrt(red[2]));
-            const double rg = fabs(p - dd2) / (1.0 + fabs(p) + fabs(dd2));
-            double rel = rp > rdd ? rp : rdd;
-            rel = rel > rg ? rel : rg;
-            if (!(rel == rel)) rel = INFINITY;
-            d.rel_kkt = rel; d.rel_p = rp; d.rel_d = rdd; d.rel_g = rg; d.pobj = p; d.dobj = dd2;
-            action_s = ctl.on_check(rel);
+            act_s = ctl.on_check(rel);
+            st.have_cand = 1;
         }
         __syncthreads();
-        const int act = action_s;
-        if (act == Control::OPTIMAL) { status = PM_SOLVE_OPTIMAL; break; }
-        if (act == Control::FAIL) { status = PM_SOLVE_NUMERICAL_FAILURE; break; }
-        if (act == Control::LIMIT) break;
-        if (act == Control::REFINE) {
-            for (int j = tid; j < n; j += kBlock) v.xbase[j] = v.xtot[j];
-            for (int i = tid; i < m; i += kBlock) v.ybase[i] = v.ytot[i];
-            __syncthreads();
-            round_start = true;
-            have_cand = false;
-            continue;
-        }
-        if (act == Control::RESTART) {
-            double acc[2] = {0.0, 0.0};
-            for (int j = tid; j < n; j += kBlock) {
-                const double df = static_cast<double>(v.xt[j]) - v.xl[j];
-                acc[0] += df * df;
-                v.x[j] = v.x0[j] = v.xl[j] = v.xt[j];
-            }
-            for (int i = tid; i < m; i += kBlock) {
-                const double df = static_cast<double>(v.yt[i]) - v.yl[i];
-                acc[1] += df * df;
-                v.y[i] = v.y0[i] = v.yl[i] = v.yt[i];
-            }
-            block_reduce_all<2, 0>(acc, red);
-            if (tid == 0) ctl.on_restart(sqrt(red[0]), sqrt(red[1]));
-            __syncthreads();
-        }
+        act = act_s;
+    } else {
+        act = Control::LIMIT;
     }
-    if (!have_cand) {
-        for (int j = tid; j < n; j += kBlock) v.xtot[j] = v.xbase[j];
-        for (int i = tid; i < m; i += kBlock) v.ytot[i] = v.ybase[i];
-        __syncthreads();
-        brows_red_any(v.la, m, v.rp, v.ci, v.oa, v.xtot, EpiKktRow{v.lr, v.ur, v.ytot, v.dr}, red);
-        if (tid < 7) kkt_s[tid] = red[tid];
-        __syncthreads();
-        brows_red_any(v.lat, n, v.cp, v.ri, v.oat, v.ytot, EpiKktCol{v.cmin, v.lx, v.ux, v.xtot, v.dc}, red);
+    bool done = act == Control::OPTIMAL || act == Control::FAIL || act == Control::LIMIT;
+    if (act == Control::REFINE) {
+        for (int j = tid; j < n; j += kBB) v.xbase[j] = v.xtot[j];
+        for (int i = tid; i < m; i += kBB) v.ybase[i] = v.ytot[i];
         if (tid == 0) {
-            const double p = red[4], dd2 = kkt_s[5] + red[3];
-            const double rp = sqrt(kkt_s[0]) / (1.0 + sqrt(kkt_s[2]));
-            const double rdd = sqrt(kkt_s[3] + red[0]) / (1.0 + sqrt(red[2]));
-            const double rg = fabs(p - dd2) / (1.0 + fabs(p) + fabs(dd2));
-            double rel = rp > rdd ? rp : rdd;
-            rel = rel > rg ? rel : rg;
-            d.rel_kkt = rel; d.rel_p = rp; d.rel_d = rdd; d.rel_g = rg; d.pobj = p; d.dobj = dd2;
-            if (rel <= A.tol) status = PM_SOLVE_OPTIMAL;
+            st.round_start = 1;
+            st.have_cand = 0;
+        }
+    } else if (act == Control::RESTART) {
+        double acc[2] = {0.0, 0.0};
+        for (int j = tid; j < n; j += kBB) {
+            const double df = static_cast<double>(v.xt[j]) - v.xl[j];
+            acc[0] += df * df;
+            v.x[j] = v.x0[j] = v.xl[j] = v.xt[j];
+        }
+        for (int i = tid; i < m; i += kBB) {
+            const double df = static_cast<double>(v.yt[i]) - v.yl[i];
+            acc[1] += df * df;
+            v.y[i] = v.y0[i] = v.yl[i] = v.yt[i];
+        }
+        block_reduce_all<2, 0>(acc, red);
+        if (tid == 0) ctl.on_restart(sqrt(red[0]), sqrt(red[1]));
+    }
+    if (done) {
+        __syncthreads();
+        if (!st.have_cand) {  // the budget ran out right after a refinement
+            for (int j = tid; j < n; j += kBB) v.xtot[j] = v.xbase[j];
+            for (int i = tid; i < m; i += kBB) v.ytot[i] = v.ybase[i];
+            __syncthreads();
+            const double rel = batch_kkt(v, d, red, keep);
+            if (tid == 0 && rel <= A.tol) act = Control::OPTIMAL;
+        }
+        if (tid == 0) {
+            d.status = act == Control::OPTIMAL ? PM_SOLVE_OPTIMAL
+                       : act == Control::FAIL  ? PM_SOLVE_NUMERICAL_FAILURE
+                                               : PM_SOLVE_ITERATION_LIMIT;
+            d.iterations = ctl.total_it;
+            d.restarts = ctl.restarts;
+            d.refinements = ctl.rounds;
+            d.omega = ctl.omega;
+            d.t_end = wall_clock64();
+            st.active = 0;
         }
     }
     if (tid == 0) {
-        d.status = status;
-        d.iterations = ctl.total_it;
-        d.restarts = ctl.restarts;
-        d.refinements = ctl.rounds;
-        d.omega = ctl.omega;
-        d.t_start = t_start;
-        d.t_end = wall_clock64();
+        st.ctl = ctl;
+        if (!done) atomicAdd(active_count, 1);
     }
 }
+
+__global__ void k_zero_int(int* p) { *p = 0; }
 
 }  // namespace
 }  // namespace pm
@@ -412,6 +474,9 @@ struct pm_batch {
     DevBuf<float> c32, lr32, ur32, lx32, ux32, g32, x, xe, x0, xt, xl, y, y0, yt, yl;
     DevBuf<double> xtot, ytot, xbase, ybase;
     DevBuf<BatchDesc> d_desc;
+    DevBuf<char> state;
+    DevBuf<int> counter;
+    PinnedBuf<int> h_counter;
     double clock_khz = 100000.0;
 };
 
@@ -559,6 +624,9 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     for (auto* p : {&b->xtot, &b->xbase, &b->wx}) p->resize(Cn);
     for (auto* p : {&b->ytot, &b->ybase, &b->wy}) p->resize(R);
     b->d_desc.upload(b->desc, s);
+    b->state.resize(sizeof(BatchState) * static_cast<size_t>(count));
+    b->counter.resize(1);
+    b->h_counter.resize(1);
     PM_HIP(hipStreamSynchronize(s));
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0)
@@ -586,30 +654,54 @@ BatchArrays<VT> batch_arrays(pm_batch* b, const VT* oa, const VT* oat) {
     return A;
 }
 
-void batch_launch(pm_batch* b) {
+template <class VT>
+void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
     hipStream_t s = b->stream;
-    if (b->exact) {
-        auto A = batch_arrays<float>(b, b->oa32.get(), b->oat32.get());
-        hipLaunchKernelGGL(k_batch<float>, dim3(b->count), dim3(kBlock), 0, s, A);
-    } else {
-        auto A = batch_arrays<double>(b, b->oa64.get(), b->oat64.get());
-        hipLaunchKernelGGL(k_batch<double>, dim3(b->count), dim3(kBlock), 0, s, A);
+    auto A = batch_arrays<VT>(b, oa, oat);
+    BatchState* S = reinterpret_cast<BatchState*>(b->state.get());
+    const dim3 g(b->count), blk(kBB);
+    switch (phase) {
+        case 0: hipLaunchKernelGGL(k_b_init<VT>, g, blk, 0, s, A, S); break;
+        case 1: hipLaunchKernelGGL(k_b_round<VT>, g, blk, 0, s, A, S); break;
+        case 2: hipLaunchKernelGGL(k_b_iter<VT>, g, blk, 0, s, A, S, b->check_frequency); break;
+        default: hipLaunchKernelGGL(k_b_check<VT>, g, blk, 0, s, A, S, b->counter.get()); break;
     }
     PM_LAUNCHED();
+}
+
+void phase(pm_batch* b, int p) {
+    if (b->exact) launch_phase<float>(b, p, b->oa32.get(), b->oat32.get());
+    else launch_phase<double>(b, p, b->oa64.get(), b->oat64.get());
+}
+
+// One chunk: clear the active counter, round data, K iterations, check.
+void chunk_launches(pm_batch* b) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, b->stream, b->counter.get());
+    PM_LAUNCHED();
+    phase(b, 1);
+    phase(b, 2);
+    phase(b, 3);
 }
 
 void batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
     hipStream_t s = b->stream;
     if (!b->exec) {
         PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        batch_launch(b);
+        for (int q = 0; q < kChunksPerGraph; ++q) chunk_launches(b);
         PM_HIP(hipStreamEndCapture(s, &b->graph));
         PM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     // Warm-start flags may have changed since the last run.
     b->d_desc.upload(b->desc, s);
     PM_HIP(hipEventRecord(b->ev0, s));
-    PM_HIP(hipGraphLaunch(b->exec, s));
+    phase(b, 0);
+    const int max_launches = b->max_iterations / std::max(1, b->check_frequency) + 4 * b->max_refinements + 8;
+    for (int q = 0; q < max_launches; q += kChunksPerGraph) {
+        PM_HIP(hipGraphLaunch(b->exec, s));
+        PM_HIP(hipMemcpyAsync(b->h_counter.get(), b->counter.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        PM_HIP(hipStreamSynchronize(s));
+        if (b->h_counter[0] == 0) break;
+    }
     PM_HIP(hipEventRecord(b->ev1, s));
     PM_HIP(hipEventSynchronize(b->ev1));
     float ms = 0.0f;
@@ -625,7 +717,6 @@ void batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
                     static_cast<double>(out[k].t_end - out[k].t_start) / (b->clock_khz * 1e3);
     }
     if (!results) return;
-    std::vector<double> xh, yh;
     for (int k = 0; k < b->count; ++k) {
         const BatchDesc& d = out[k];
         pm_pdhg_result& r = results[k];

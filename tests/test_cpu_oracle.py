@@ -14,9 +14,9 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 INF = 1e30
 
 
-def header_symbols():
+def header_symbols(headers=("pramaan_hip.h", "pramaan_gen.h", "pramaan_gnn.h")):
     syms = set()
-    for h in (ROOT / "include").glob("*.h"):
+    for h in (ROOT / "include" / name for name in headers):
         for name in re.findall(r"^\s*(?:[\w\*]+\s+)+\**(pm_\w+)\s*\(", h.read_text(), re.M):
             syms.add(name)
     return syms
@@ -29,6 +29,12 @@ def test_library_exports_every_declared_symbol(capi):
     missing = [s for s in sorted(syms) if not hasattr(lib, s)]
     assert not missing, missing
     assert set(capi.SIGNATURES) >= syms
+    # The certificate entry points live in the C++ interface library.
+    cert = header_symbols(("pramaan_cert.h",))
+    cxx = ctypes.CDLL(str(capi.CXX_LIB_PATH))
+    assert cert and not [s for s in sorted(cert) if not hasattr(cxx, s)]
+    assert {p.name for p in (ROOT / "include").glob("*.h")} == {
+        "pramaan_hip.h", "pramaan_gen.h", "pramaan_gnn.h", "pramaan_cert.h"}
 
 
 def test_no_device_is_reported_not_faked(capi):
