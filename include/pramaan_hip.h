@@ -204,11 +204,13 @@ int pm_ctx_spmv_variants(pm_ctx* ctx, int32_t* variant_a,
                          int32_t* variant_at);
 
 /* ---- batched solver ------------------------------------------------------
- * Many independent LPs solved by one kernel launch: one workgroup per LP
- * (largest first), each running the PM_ALG_PDLP algorithm (power
- * iteration, refinement rounds, reflected Halpern iterations, FP64 KKT
- * checks, restarts) entirely on the device.  The launch is captured once in
- * a HIP graph and replayed by every pm_batch_run.  Only opts->tolerance,
+ * Many independent LPs, resident on the device as one block-diagonal LP,
+ * each running the PM_ALG_PDLP algorithm (power iteration, refinement
+ * rounds, reflected Halpern iterations, FP64 KKT checks, restarts)
+ * entirely on the device: every iteration is two grid-wide launches over
+ * all LPs still running, and the per-LP checks and decisions run one
+ * workgroup per LP.  The launch sequence is captured once in a HIP graph
+ * and replayed by every pm_batch_run.  Only opts->tolerance,
  * max_iterations, check_frequency and max_refinements are used; the
  * algorithm is always PM_ALG_PDLP with PM_TERM_RELATIVE_KKT. */
 typedef struct pm_batch pm_batch;
@@ -223,9 +225,31 @@ int pm_batch_create(int32_t device, int32_t count, const pm_lp* lps,
  * space, y in the reference sign.  Copied at the call. */
 int pm_batch_set_warm_start(pm_batch* b, int32_t i, const double* x0,
                             const double* y0);
+/* Initial primal weight of LP i for the following runs (the weight of the
+ * Ruiz / Pock-Chambolle scaled problem, as pm_pdhg_result.primal_weight
+ * reports it); omega <= 0 restores the default ||c_hat||_2 / ||b_hat||_2. */
+int pm_batch_set_primal_weight(pm_batch* b, int32_t i, double omega);
 /* Solves every LP of the batch; results[i].x/.y filled when non-NULL. */
 int pm_batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats);
 int pm_batch_destroy(pm_batch* b);
+/* use_graph = 0 issues the same launches directly on the stream instead of
+ * replaying the captured graph (for profilers that do not follow graph
+ * launches); results are identical.  Default 1. */
+int pm_batch_set_graph(pm_batch* b, int32_t use_graph);
+/* Times the grid-wide iterate kernel `which` (PM_KERNEL_SPMV_A: A xe with
+ * the fused dual update, PM_KERNEL_SPMV_AT: A^T y with the fused primal
+ * update) with every LP of the batch running: `reps` launches back to back
+ * between two HIP events on the batch stream.  Reports the mean duration
+ * per launch and the algorithmic bytes of one launch (the single-LP
+ * formulas of pm_ctx_time_kernel summed over the LPs).  Leaves the iterates
+ * changed; the next pm_batch_run starts afresh. */
+int pm_batch_time_kernel(pm_batch* b, int32_t which, int32_t reps,
+                         double* avg_ms, double* alg_bytes);
+/* Lanes per row the iterate kernels use for A and A^T, as PM_SPMV_SCALAR /
+ * PM_SPMV_VECTORx: opts->spmv_variant when it names one of those, else
+ * autotuned at pm_batch_create over the whole batch. */
+int pm_batch_spmv_variants(pm_batch* b, int32_t* variant_a,
+                           int32_t* variant_at);
 /* create + run + destroy; x0/y0 may be NULL or arrays of count pointers. */
 int pm_batch_solve(int32_t count, const pm_lp* lps, const double* const* x0,
                    const double* const* y0, const pm_pdhg_options* opts,

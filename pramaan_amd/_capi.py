@@ -91,6 +91,10 @@ SIGNATURES = {
     "pm_batch_set_warm_start": (C.c_int, [C.c_void_p, C.c_int32, _d, _d]),
     "pm_batch_run": (C.c_int, [C.c_void_p, C.POINTER(pm_pdhg_result), C.POINTER(pm_batch_stats)]),
     "pm_batch_destroy": (C.c_int, [C.c_void_p]),
+    "pm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pm_batch_set_primal_weight": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "pm_batch_spmv_variants": (C.c_int, [C.c_void_p, _i, _i]),
+    "pm_batch_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _d, _d]),
     "pm_batch_solve": (C.c_int, [C.c_int32, C.POINTER(pm_lp), C.POINTER(_d), C.POINTER(_d),
                                  C.POINTER(pm_pdhg_options), C.POINTER(pm_pdhg_result), C.POINTER(pm_batch_stats)]),
     "pm_gen_params_named": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(pm_gen_params)]),
@@ -371,6 +375,23 @@ class Batch:
         out = [_result(res[i], xs[i] if want_vectors else None, ys[i] if want_vectors else None, None)
                for i in range(k)]
         return out, st.device_seconds, inst
+
+    def set_primal_weight(self, i: int, omega: float) -> None:
+        _check(lib().pm_batch_set_primal_weight(self._h, i, float(omega)))
+
+    def spmv_variants(self) -> tuple[int, int]:
+        a, at = C.c_int32(), C.c_int32()
+        _check(lib().pm_batch_spmv_variants(self._h, C.byref(a), C.byref(at)))
+        return a.value, at.value
+
+    def set_graph(self, use_graph: bool) -> None:
+        _check(lib().pm_batch_set_graph(self._h, 1 if use_graph else 0))
+
+    def time_kernel(self, which: int, reps: int = 20) -> tuple[float, float]:
+        """(mean ms per launch, algorithmic bytes per launch) of an iterate kernel."""
+        ms, by = C.c_double(), C.c_double()
+        _check(lib().pm_batch_time_kernel(self._h, which, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def close(self) -> None:
         if self._h:

@@ -1,18 +1,27 @@
-// batch.hip -- many independent LPs solved by one kernel launch.
+// batch.hip -- many independent LPs solved by one graph launch.
 //
 // Layout: every LP's A and A^T (CSR, local indices) and vectors are packed
-// back to back in shared device arrays; a per-LP descriptor holds the
-// offsets.  Every phase kernel runs one 1024-thread workgroup per LP,
-// largest LP first (blockIdx -> LP through `order`), so the hardware
-// dispatcher hands the next LP to whichever CU frees up, and an LP that has
-// converged returns at once.  Phases: init (power iteration for ||A_hat||_2,
-// initial primal weight, base points), round (correction-LP data of a
-// refinement round), iterate (check_frequency reflected Halpern iterations:
-// the same epilogue functors as the single-LP kernels, run as block-strided
-// row loops with L lanes per row) and check (FP64 KKT with deterministic
-// block reductions, then the pm::Control decision -- the same code the
-// single-LP host loop runs -- and the restart / refinement it asks for).
-// Per-LP solver state lives in device memory between phases.  Four
+// back to back in shared device arrays, i.e. the batch is one block-diagonal
+// LP; a per-LP descriptor holds the offsets, and global row / column pointer
+// arrays plus a row -> LP and column -> LP map address the concatenation.
+//
+// Phases:
+//  init, round, check -- one 1024-thread workgroup per LP (largest first):
+//    power iteration for ||A_hat||_2 and the initial primal weight; the
+//    correction-LP data of a refinement round; the FP64 KKT of the candidate
+//    with deterministic block reductions, then the pm::Control decision (the
+//    code the single-LP host loop runs) and the restart / refinement it asks
+//    for.  These run once per check_frequency iterations or less.
+//  iterate -- the hot loop: each reflected-Halpern iteration is two
+//    grid-wide launches over the whole concatenation (A^T y -> x, A xe ->
+//    y), the single-LP engine's row kernels with per-LP step sizes, Halpern
+//    weights and chunk lengths read from a small per-LP record.  Rows of
+//    finished LPs are skipped.  Each XCD owns one contiguous eighth of the
+//    rows (so it gathers from the vectors of its own LPs only, which stay
+//    in its L2), and inside that eighth consecutive row groups go to
+//    different workgroups, so even the last LP still running is spread over
+//    a whole XCD instead of a single CU.
+// Per-LP solver state lives in device memory between phases.  A few
 // round/iterate/check chunks are captured once in a HIP graph; the host
 // replays it until a device counter reports no active LP.  Nothing is
 // allocated or cleared with a memset between replays.
@@ -24,6 +33,7 @@
 #include <memory>
 #include <numeric>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "control.hpp"
@@ -36,19 +46,57 @@ namespace {
 // Workgroup size of the batch kernel: 16 wavefronts per LP, so one LP keeps
 // enough loads in flight to stream its data from HBM.
 constexpr int kBB = 1024;
-// Chunks of check_frequency iterations per graph replay.
-constexpr int kChunksPerGraph = 4;
+
+// Largest lane count the iterate kernels are built for.
+constexpr int kMaxLanes = 32;
+
+// Lanes of an explicit SpMV variant (0 for PM_SPMV_AUTO and the variants
+// the iterate kernels do not offer, which are autotuned instead).
+inline int variant_lanes_or_zero(int v) {
+    switch (v) {
+        case PM_SPMV_SCALAR: return 1;
+        case PM_SPMV_VECTOR2: return 2;
+        case PM_SPMV_VECTOR4: return 4;
+        case PM_SPMV_VECTOR8: return 8;
+        case PM_SPMV_VECTOR16: return 16;
+        case PM_SPMV_VECTOR32: return 32;
+        default: return 0;
+    }
+}
+
+inline int lanes_variant(int l) {
+    switch (l) {
+        case 1: return PM_SPMV_SCALAR;
+        case 2: return PM_SPMV_VECTOR2;
+        case 4: return PM_SPMV_VECTOR4;
+        case 16: return PM_SPMV_VECTOR16;
+        case 32: return PM_SPMV_VECTOR32;
+        default: return PM_SPMV_VECTOR8;
+    }
+}
 
 struct BatchDesc {
     int32_t m, n, nnz;
     int32_t lanes_a, lanes_at;
     int32_t warm_x, warm_y;
     int32_t pad;
+    double omega0;  // initial primal weight; <= 0: ||c_hat|| / ||b_hat||
     int64_t rp_off, cp_off, nz_off, row_off, col_off;
     // results
     int32_t status, iterations, restarts, refinements;
     double rel_kkt, rel_p, rel_d, rel_g, pobj, dobj, omega;
     uint64_t t_start, t_end;
+};
+
+// What the grid-wide iterate kernels need of one LP for the next chunk,
+// written by the init and check phases.
+struct LpIter {
+    float tau, sig;
+    int32_t k0;      // in-round Halpern index of the chunk's first iteration
+    int32_t chunk;   // iterations in the chunk (0: LP finished)
+    int32_t use_g;   // dual shift of a refinement round present
+    int32_t row_off, col_off;
+    int32_t pad;
 };
 
 template <class VT>
@@ -68,6 +116,9 @@ struct BatchArrays {
     double *xtot, *ytot, *xbase, *ybase;
     BatchDesc* desc;
     const int* order;
+    const int* grp;     // global row pointer of A   (nz offsets of the concatenation)
+    const int* gcp;     // global row pointer of A^T
+    LpIter* it;
     double tol;
     int max_iterations, check_frequency, max_refinements;
 };
@@ -216,8 +267,23 @@ struct BatchState {
     int32_t active;       // 1 while the LP is being solved
     int32_t round_start;  // 1: the next round-data phase must run
     int32_t have_cand;    // a candidate T(z) exists in xtot / ytot
-    int32_t last_chunk;   // iterations the last iterate phase ran
 };
+
+// The next chunk of LP lp as the iterate kernels read it.
+__device__ __forceinline__ void publish_iter(LpIter* it, int lp, const BatchState& st, const BatchDesc& d,
+                                             int K) {
+    LpIter r;
+    const double om = st.ctl.omega;
+    r.tau = static_cast<float>(st.eta / om);
+    r.sig = static_cast<float>(st.eta * om);
+    r.k0 = st.ctl.k;
+    r.chunk = st.active ? st.ctl.chunk(K) : 0;
+    r.use_g = st.ctl.use_g ? 1 : 0;
+    r.row_off = static_cast<int32_t>(d.row_off);
+    r.col_off = static_cast<int32_t>(d.col_off);
+    r.pad = 0;
+    it[lp] = r;
+}
 
 // Phase 0: power iteration for ||A_hat||_2, initial primal weight, base points.
 template <class VT>
@@ -265,15 +331,16 @@ __global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S
     block_reduce_all<2, 0>(acc, red);
     if (tid == 0) {
         BatchState& st = S[lp];
-        const double om = (red[0] > 1e-20 && red[1] > 1e-20) ? sqrt(red[0]) / sqrt(red[1]) : 1.0;
+        double om = (red[0] > 1e-20 && red[1] > 1e-20) ? sqrt(red[0]) / sqrt(red[1]) : 1.0;
+        if (d.omega0 > 0.0) om = d.omega0;
         st.ctl.init(A.tol, A.max_iterations, A.max_refinements, om, d.warm_y != 0 && m > 0);
         st.eta = lam > 0.0 ? 0.99 / sqrt(sqrt(lam)) : 1.0;
         st.status = PM_SOLVE_ITERATION_LIMIT;
         st.active = 1;
         st.round_start = 1;
         st.have_cand = 0;
-        st.last_chunk = 0;
         d.t_start = t0;
+        publish_iter(A.it, lp, st, d, A.check_frequency);
     }
 }
 
@@ -303,30 +370,110 @@ __global__ __launch_bounds__(kBB) void k_b_round(BatchArrays<VT> A, BatchState* 
     if (tid == 0) st.round_start = 0;
 }
 
-// Phase 2: up to K reflected-Halpern iterations; T(z) of the last one kept.
+// Work list of the iterate kernels, rebuilt after every check: one item
+// {lp, first global row, rows} per group of G rows of every LP still
+// running (G = 256 / lanes), in LP order; counts in wcnt[0] (rows of A) and
+// wcnt[1] (rows of A^T).  One 1024-thread workgroup scans the LPs.
 template <class VT>
-__global__ __launch_bounds__(kBB) void k_b_iter(BatchArrays<VT> A, BatchState* S, int K) {
-    const int lp = A.order[blockIdx.x];
-    BatchState& st = S[lp];
-    if (!st.active) return;
-    const BatchDesc& d = A.desc[lp];
-    const View<VT> v = view_of(A, d);
-    const int chunk = st.ctl.chunk(K), k0 = st.ctl.k;
-    const double om = st.ctl.omega;
-    const float tau = static_cast<float>(st.eta / om), sig = static_cast<float>(st.eta * om);
-    const bool use_g = st.ctl.use_g;
-    for (int i = 0; i < chunk; ++i) {
-        const float lamk = Control::lambda(k0 + i);
-        const bool keep = i == chunk - 1;
-        brows_any<float>(v.lat, v.n, v.cp, v.ri, v.vat, v.y,
-                         EpiPrimalHal{tau, lamk, 1.0f, v.c32, v.lx32, v.ux32, v.x0, v.x, v.xe, keep ? v.xt : nullptr});
+__global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int count, int ga, int gt,
+                                                     int4* __restrict__ wr, int4* __restrict__ wc,
+                                                     int* __restrict__ wcnt) {
+    __shared__ int sr[1024], sc[1024];
+    __shared__ int carry_r, carry_c;
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid == 0) carry_r = carry_c = 0;
+    __syncthreads();
+    for (int base = 0; base < count; base += 1024) {
+        const int k = base + tid;
+        int nr = 0, nc = 0, m = 0, n = 0;
+        int64_t ro = 0, co = 0;
+        if (k < count && A.it[k].chunk > 0) {
+            const BatchDesc& d = A.desc[k];
+            m = d.m;
+            n = d.n;
+            ro = d.row_off;
+            co = d.col_off;
+            nr = (m + ga - 1) / ga;
+            nc = (n + gt - 1) / gt;
+        }
+        sr[tid] = nr;
+        sc[tid] = nc;
         __syncthreads();
-        brows_any<float>(v.la, v.m, v.rp, v.ci, v.va, v.xe,
-                         EpiDualHal{sig, lamk, 1.0f, v.lr32, v.ur32, use_g ? v.g32 : nullptr, v.y0, v.y,
-                                    keep ? v.yt : nullptr});
+        for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan (Hillis-Steele)
+            const int ar = tid >= o ? sr[tid - o] : 0, ac = tid >= o ? sc[tid - o] : 0;
+            __syncthreads();
+            sr[tid] += ar;
+            sc[tid] += ac;
+            __syncthreads();
+        }
+        const int offr = carry_r + sr[tid] - nr, offc = carry_c + sc[tid] - nc;
+        for (int g = 0; g < nr; ++g)
+            wr[offr + g] = make_int4(k, static_cast<int>(ro) + g * ga, min(ga, m - g * ga), 0);
+        for (int g = 0; g < nc; ++g)
+            wc[offc + g] = make_int4(k, static_cast<int>(co) + g * gt, min(gt, n - g * gt), 0);
+        __syncthreads();
+        if (tid == 1023) {
+            carry_r += sr[1023];
+            carry_c += sc[1023];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) st.last_chunk = chunk;
+    if (tid == 0) {
+        wcnt[0] = carry_r;
+        wcnt[1] = carry_c;
+    }
+}
+
+// Phase 2: iteration i of the chunk, both halves as grid-wide launches over
+// the work list.  Block b runs on XCD b % 8 and takes the items of that
+// XCD's eighth of the list (contiguous LPs, whose vectors then stay in the
+// XCD's L2), strided by the number of blocks per XCD.
+__device__ __forceinline__ void xcd_slice(int total, int& w0, int& w1, int& step) {
+    const int nx = gridDim.x >= 8 ? 8 : 1;
+    const int x = static_cast<int>(blockIdx.x) % nx;
+    w0 = static_cast<int>(static_cast<int64_t>(total) * x / nx) + static_cast<int>(blockIdx.x) / nx;
+    w1 = static_cast<int>(static_cast<int64_t>(total) * (x + 1) / nx);
+    step = static_cast<int>(gridDim.x) / nx;
+}
+
+template <int L, class VT>
+__global__ __launch_bounds__(kBlock) void k_s_primal(BatchArrays<VT> A, const int4* __restrict__ work,
+                                                     const int* __restrict__ wcnt, int i) {
+    const int lane = static_cast<int>(threadIdx.x) & (L - 1), g = static_cast<int>(threadIdx.x) / L;
+    int w0, w1, step;
+    xcd_slice(wcnt[1], w0, w1, step);
+    for (int w = w0; w < w1; w += step) {
+        const int4 t = work[w];
+        const LpIter P = A.it[t.x];
+        if (g >= t.z || i >= P.chunk) continue;
+        const int j = t.y + g;
+        const float aty = row_dot<L, false, float>(A.gcp[j], A.gcp[j + 1], lane, A.ri, A.vat, A.y + P.row_off);
+        if (lane == 0) {
+            const bool keep = i == P.chunk - 1;
+            EpiPrimalHal{P.tau, Control::lambda(P.k0 + i), 1.0f, A.c32, A.lx32, A.ux32, A.x0, A.x, A.xe,
+                         keep ? A.xt : nullptr}(j, aty);
+        }
+    }
+}
+
+template <int L, class VT>
+__global__ __launch_bounds__(kBlock) void k_s_dual(BatchArrays<VT> A, const int4* __restrict__ work,
+                                                   const int* __restrict__ wcnt, int i) {
+    const int lane = static_cast<int>(threadIdx.x) & (L - 1), g = static_cast<int>(threadIdx.x) / L;
+    int w0, w1, step;
+    xcd_slice(wcnt[0], w0, w1, step);
+    for (int w = w0; w < w1; w += step) {
+        const int4 t = work[w];
+        const LpIter P = A.it[t.x];
+        if (g >= t.z || i >= P.chunk) continue;
+        const int r = t.y + g;
+        const float ax = row_dot<L, false, float>(A.grp[r], A.grp[r + 1], lane, A.ci, A.va, A.xe + P.col_off);
+        if (lane == 0) {
+            const bool keep = i == P.chunk - 1;
+            EpiDualHal{P.sig, Control::lambda(P.k0 + i), 1.0f, A.lr32, A.ur32, P.use_g ? A.g32 : nullptr, A.y0,
+                       A.y, keep ? A.yt : nullptr}(r, ax);
+        }
+    }
 }
 
 // FP64 relative KKT of (xtot, ytot) into the descriptor; returns rel on
@@ -367,7 +514,7 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
     __shared__ Control ctl;
     if (tid == 0) ctl = st.ctl;
     __syncthreads();
-    const int chunk = st.last_chunk;
+    const int chunk = A.it[lp].chunk;
     int act;
     if (chunk > 0) {
         const double invp = 1.0 / ctl.DP, invd = 1.0 / ctl.DD;
@@ -440,6 +587,7 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
     }
     if (tid == 0) {
         st.ctl = ctl;
+        publish_iter(A.it, lp, st, d, A.check_frequency);
         if (!done) atomicAdd(active_count, 1);
     }
 }
@@ -474,6 +622,13 @@ struct pm_batch {
     DevBuf<float> c32, lr32, ur32, lx32, ux32, g32, x, xe, x0, xt, xl, y, y0, yt, yl;
     DevBuf<double> xtot, ytot, xbase, ybase;
     DevBuf<BatchDesc> d_desc;
+    DevBuf<int> grp, gcp, wcnt;
+    DevBuf<int4> work_r, work_c;
+    DevBuf<LpIter> iter;
+    int lanes_a = 8, lanes_at = 8;  // lanes per row of the grid-wide iterate kernels
+    int iter_grid = 2048;           // workgroups of one iterate launch: 8 per CU, a multiple of 8
+    int chunks_per_graph = 4;
+    bool use_graph = true;
     DevBuf<char> state;
     DevBuf<int> counter;
     PinnedBuf<int> h_counter;
@@ -482,6 +637,8 @@ struct pm_batch {
 
 namespace pm {
 namespace {
+
+void batch_autotune(pm_batch* b);
 
 void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_options* o) {
     if (count <= 0) fail(PM_ERR_INVALID_ARG, "batch count must be positive");
@@ -533,6 +690,8 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         for (int q = 0; q < L.nnz && b->exact; ++q)
             if (static_cast<double>(static_cast<float>(L.values[q])) != L.values[q]) b->exact = false;
     }
+    if (nzo >= INT32_MAX || ro >= INT32_MAX || co >= INT32_MAX)
+        fail(PM_ERR_INVALID_ARG, "batch too large: rows, columns and nonzeros must each stay below 2^31");
     b->rows_total = ro;
     b->cols_total = co;
     b->nnz_total = nzo;
@@ -543,6 +702,9 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     if (b->exact) { h_oa32.resize(Z); h_oat32.resize(Z); }
     else { h_oa64.resize(Z); h_oat64.resize(Z); }
     std::vector<double> h_cmin(Cn), h_lx(Cn), h_ux(Cn), h_dc(Cn), h_lr(R), h_ur(R), h_dr(R);
+    std::vector<int> h_grp(R + 1, 0), h_gcp(Cn + 1, 0);
+    h_grp[ro] = static_cast<int>(nzo);
+    h_gcp[co] = static_cast<int>(nzo);
     b->c_model.assign(Cn, 0.0);
     // Per-LP preparation (transpose, Ruiz + PC scaling) on host threads.
     auto prep = [&](int k) {
@@ -557,6 +719,12 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
             ruiz_pc_scale(m, n, L.row_ptr, L.col_idx, L.values, opts.ruiz_iterations, dr, dc);
         std::copy(L.row_ptr, L.row_ptr + m + 1, h_rp.begin() + d.rp_off);
         std::copy(cp.begin(), cp.end(), h_cp.begin() + d.cp_off);
+        for (int i = 0; i < m; ++i) {
+            h_grp[d.row_off + i] = static_cast<int>(d.nz_off + L.row_ptr[i]);
+        }
+        for (int j = 0; j < n; ++j) {
+            h_gcp[d.col_off + j] = static_cast<int>(d.nz_off + cp[j]);
+        }
         for (int q = 0; q < nnz; ++q) {
             h_ci[d.nz_off + q] = L.col_idx[q];
             h_ri[d.nz_off + q] = ri[q];
@@ -619,6 +787,26 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->ur.upload(h_ur, s);
     b->dr.upload(h_dr, s);
     b->d_order.upload(b->order, s);
+    b->grp.upload(h_grp, s);
+    b->gcp.upload(h_gcp, s);
+    b->iter.resize(static_cast<size_t>(count));
+    // One lane count per product for the whole concatenation (its mean row
+    // length), four chunks per graph unless that makes the graph large.
+    b->lanes_a = ro > 0 ? lanes_for(static_cast<double>(nzo) / static_cast<double>(ro)) : 1;
+    b->lanes_at = lanes_for(static_cast<double>(nzo) / static_cast<double>(co));
+    // Work-list capacity for the largest lane count (the smallest groups).
+    constexpr int kMinGroup = kBlock / kMaxLanes;
+    size_t items_r = 0, items_c = 0;
+    for (const BatchDesc& d : b->desc) {
+        items_r += (d.m + kMinGroup - 1) / kMinGroup;
+        items_c += (d.n + kMinGroup - 1) / kMinGroup;
+    }
+    b->work_r.resize(std::max<size_t>(items_r, 1));
+    b->work_c.resize(std::max<size_t>(items_c, 1));
+    b->wcnt.resize(2);
+    const int fixed = variant_lanes_or_zero(opts.spmv_variant);
+    if (fixed > 0) b->lanes_a = b->lanes_at = fixed;
+    b->chunks_per_graph = std::max(1, std::min(4, 512 / std::max(1, b->check_frequency)));
     for (auto* p : {&b->c32, &b->lx32, &b->ux32, &b->x, &b->xe, &b->x0, &b->xt, &b->xl}) p->resize(Cn);
     for (auto* p : {&b->lr32, &b->ur32, &b->g32, &b->y, &b->y0, &b->yt, &b->yl}) p->resize(R);
     for (auto* p : {&b->xtot, &b->xbase, &b->wx}) p->resize(Cn);
@@ -628,6 +816,7 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->counter.resize(1);
     b->h_counter.resize(1);
     PM_HIP(hipStreamSynchronize(s));
+    if (fixed == 0) batch_autotune(b);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0)
         b->clock_khz = khz;
@@ -647,6 +836,9 @@ BatchArrays<VT> batch_arrays(pm_batch* b, const VT* oa, const VT* oat) {
     A.xtot = b->xtot.get(); A.ytot = b->ytot.get(); A.xbase = b->xbase.get(); A.ybase = b->ybase.get();
     A.desc = b->d_desc.get();
     A.order = b->d_order.get();
+    A.grp = b->grp.get();
+    A.gcp = b->gcp.get();
+    A.it = b->iter.get();
     A.tol = b->tol;
     A.max_iterations = b->max_iterations;
     A.check_frequency = b->check_frequency;
@@ -663,8 +855,33 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
     switch (phase) {
         case 0: hipLaunchKernelGGL(k_b_init<VT>, g, blk, 0, s, A, S); break;
         case 1: hipLaunchKernelGGL(k_b_round<VT>, g, blk, 0, s, A, S); break;
-        case 2: hipLaunchKernelGGL(k_b_iter<VT>, g, blk, 0, s, A, S, b->check_frequency); break;
-        default: hipLaunchKernelGGL(k_b_check<VT>, g, blk, 0, s, A, S, b->counter.get()); break;
+        case 2: {
+            // xcd_range partitions the row groups exactly only for a grid
+            // that is a multiple of 8.
+            if (b->iter_grid % 8 != 0) fail(PM_ERR_INTERNAL, "iterate grid must be a multiple of 8");
+            const dim3 gi(b->iter_grid), bi(kBlock);
+            for (int i = 0; i < b->check_frequency; ++i) {
+                with_lanes(b->lanes_at, [&](auto LC) {
+                    hipLaunchKernelGGL((k_s_primal<decltype(LC)::value, VT>), gi, bi, 0, s, A, b->work_c.get(),
+                                       b->wcnt.get(), i);
+                });
+                PM_LAUNCHED();
+                if (b->rows_total > 0) {
+                    with_lanes(b->lanes_a, [&](auto LC) {
+                        hipLaunchKernelGGL((k_s_dual<decltype(LC)::value, VT>), gi, bi, 0, s, A, b->work_r.get(),
+                                           b->wcnt.get(), i);
+                    });
+                    PM_LAUNCHED();
+                }
+            }
+            return;
+        }
+        case 4:
+            hipLaunchKernelGGL(k_b_worklist<VT>, dim3(1), dim3(1024), 0, s, A, b->count, kBlock / b->lanes_a,
+                               kBlock / b->lanes_at, b->work_r.get(), b->work_c.get(), b->wcnt.get());
+            break;
+        case 3: hipLaunchKernelGGL(k_b_check<VT>, g, blk, 0, s, A, S, b->counter.get()); break;
+        default: fail(PM_ERR_INTERNAL, "unknown batch phase");
     }
     PM_LAUNCHED();
 }
@@ -681,13 +898,14 @@ void chunk_launches(pm_batch* b) {
     phase(b, 1);
     phase(b, 2);
     phase(b, 3);
+    phase(b, 4);
 }
 
 void batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
     hipStream_t s = b->stream;
-    if (!b->exec) {
+    if (b->use_graph && !b->exec) {
         PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int q = 0; q < kChunksPerGraph; ++q) chunk_launches(b);
+        for (int q = 0; q < b->chunks_per_graph; ++q) chunk_launches(b);
         PM_HIP(hipStreamEndCapture(s, &b->graph));
         PM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
@@ -695,9 +913,14 @@ void batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
     b->d_desc.upload(b->desc, s);
     PM_HIP(hipEventRecord(b->ev0, s));
     phase(b, 0);
+    phase(b, 4);
     const int max_launches = b->max_iterations / std::max(1, b->check_frequency) + 4 * b->max_refinements + 8;
-    for (int q = 0; q < max_launches; q += kChunksPerGraph) {
-        PM_HIP(hipGraphLaunch(b->exec, s));
+    for (int q = 0; q < max_launches; q += b->chunks_per_graph) {
+        if (b->use_graph) {
+            PM_HIP(hipGraphLaunch(b->exec, s));
+        } else {
+            for (int c = 0; c < b->chunks_per_graph; ++c) chunk_launches(b);
+        }
         PM_HIP(hipMemcpyAsync(b->h_counter.get(), b->counter.get(), sizeof(int), hipMemcpyDeviceToHost, s));
         PM_HIP(hipStreamSynchronize(s));
         if (b->h_counter[0] == 0) break;
@@ -744,10 +967,135 @@ void batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
     PM_HIP(hipStreamSynchronize(s));
 }
 
+// Mean duration of one grid-wide iterate launch with every LP running.
+void batch_time_kernel(pm_batch* b, int which, int reps, double* avg_ms, double* alg_bytes) {
+    if (which != PM_KERNEL_SPMV_A && which != PM_KERNEL_SPMV_AT) fail(PM_ERR_INVALID_ARG, "unknown kernel id");
+    if (reps <= 0) fail(PM_ERR_INVALID_ARG, "reps must be positive");
+    hipStream_t s = b->stream;
+    b->d_desc.upload(b->desc, s);
+    phase(b, 0);  // every LP active, chunk = check_frequency
+    phase(b, 4);
+    auto launch = [&](auto A) {
+        using VT = std::remove_cv_t<std::remove_pointer_t<decltype(A.oa)>>;
+        const dim3 gi(b->iter_grid), bi(kBlock);
+        if (which == PM_KERNEL_SPMV_AT) {
+            with_lanes(b->lanes_at, [&](auto LC) {
+                hipLaunchKernelGGL((k_s_primal<decltype(LC)::value, VT>), gi, bi, 0, s, A, b->work_c.get(),
+                                   b->wcnt.get(), 0);
+            });
+        } else if (b->rows_total > 0) {
+            with_lanes(b->lanes_a, [&](auto LC) {
+                hipLaunchKernelGGL((k_s_dual<decltype(LC)::value, VT>), gi, bi, 0, s, A, b->work_r.get(),
+                                   b->wcnt.get(), 0);
+            });
+        }
+        PM_LAUNCHED();
+    };
+    auto one = [&] {
+        if (b->exact) launch(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
+        else launch(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
+    };
+    one();  // warm caches and code
+    PM_HIP(hipEventRecord(b->ev0, s));
+    for (int r = 0; r < reps; ++r) one();
+    PM_HIP(hipEventRecord(b->ev1, s));
+    PM_HIP(hipEventSynchronize(b->ev1));
+    float ms = 0.0f;
+    PM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    double bytes = 0.0;
+    for (const BatchDesc& d : b->desc) {
+        const double nnz = d.nnz, m = d.m, n = d.n;
+        bytes += which == PM_KERNEL_SPMV_A ? 8.0 * nnz + 4.0 * (m + 1) + 4.0 * n + 20.0 * m
+                                           : 8.0 * nnz + 4.0 * (n + 1) + 4.0 * m + 28.0 * n;
+    }
+    if (avg_ms) *avg_ms = ms / reps;
+    if (alg_bytes) *alg_bytes = bytes;
+}
+
+// Lanes per row of the two iterate kernels: each candidate timed over the
+// whole batch with every LP running (iterates zeroed, unit steps), fastest
+// kept.  Runs once at creation.
+void batch_autotune(pm_batch* b) {
+    hipStream_t s = b->stream;
+    for (auto* p : {&b->x, &b->xe, &b->x0, &b->xt, &b->xl, &b->c32, &b->lx32, &b->ux32})
+        PM_HIP(hipMemsetAsync(p->get(), 0, sizeof(float) * p->size(), s));
+    for (auto* p : {&b->y, &b->y0, &b->yt, &b->yl, &b->lr32, &b->ur32, &b->g32})
+        PM_HIP(hipMemsetAsync(p->get(), 0, sizeof(float) * p->size(), s));
+    std::vector<LpIter> it(b->count);
+    for (int k = 0; k < b->count; ++k) {
+        it[k] = LpIter{1e-3f, 1e-3f, 0, 1, 0, static_cast<int32_t>(b->desc[k].row_off),
+                       static_cast<int32_t>(b->desc[k].col_off), 0};
+    }
+    b->iter.upload(it, s);
+    const int cand[] = {1, 2, 4, 8, 16, 32};
+    for (int which : {PM_KERNEL_SPMV_AT, PM_KERNEL_SPMV_A}) {
+        if (which == PM_KERNEL_SPMV_A && b->rows_total == 0) continue;
+        int& lanes = which == PM_KERNEL_SPMV_A ? b->lanes_a : b->lanes_at;
+        double best = INFINITY;
+        int best_l = lanes;
+        for (int l : cand) {
+            lanes = l;
+            phase(b, 4);
+            auto launch = [&](auto A) {
+                using VT = std::remove_cv_t<std::remove_pointer_t<decltype(A.oa)>>;
+                const dim3 gi(b->iter_grid), bi(kBlock);
+                with_lanes(l, [&](auto LC) {
+                    if (which == PM_KERNEL_SPMV_AT)
+                        hipLaunchKernelGGL((k_s_primal<decltype(LC)::value, VT>), gi, bi, 0, s, A,
+                                           b->work_c.get(), b->wcnt.get(), 0);
+                    else
+                        hipLaunchKernelGGL((k_s_dual<decltype(LC)::value, VT>), gi, bi, 0, s, A,
+                                           b->work_r.get(), b->wcnt.get(), 0);
+                });
+                PM_LAUNCHED();
+            };
+            auto one = [&] {
+                if (b->exact) launch(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
+                else launch(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
+            };
+            one();
+            PM_HIP(hipEventRecord(b->ev0, s));
+            for (int r = 0; r < 3; ++r) one();
+            PM_HIP(hipEventRecord(b->ev1, s));
+            PM_HIP(hipEventSynchronize(b->ev1));
+            float ms = 0.0f;
+            PM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (ms < best) {
+                best = ms;
+                best_l = l;
+            }
+        }
+        lanes = best_l;
+    }
+}
+
 }  // namespace
 }  // namespace pm
 
 extern "C" {
+
+int pm_batch_spmv_variants(pm_batch* b, int32_t* variant_a, int32_t* variant_at) {
+    return pm::guarded([&] {
+        if (!b || !variant_a || !variant_at) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
+        *variant_a = pm::lanes_variant(b->lanes_a);
+        *variant_at = pm::lanes_variant(b->lanes_at);
+    });
+}
+
+int pm_batch_set_graph(pm_batch* b, int32_t use_graph) {
+    return pm::guarded([&] {
+        if (!b) pm::fail(PM_ERR_INVALID_ARG, "batch is NULL");
+        b->use_graph = use_graph != 0;
+    });
+}
+
+int pm_batch_time_kernel(pm_batch* b, int32_t which, int32_t reps, double* avg_ms, double* alg_bytes) {
+    return pm::guarded([&] {
+        if (!b) pm::fail(PM_ERR_INVALID_ARG, "batch is NULL");
+        PM_HIP(hipSetDevice(b->device));
+        pm::batch_time_kernel(b, which, reps, avg_ms, alg_bytes);
+    });
+}
 
 int pm_batch_create(int32_t device, int32_t count, const pm_lp* lps, const pm_pdhg_options* opts, pm_batch** out) {
     return pm::guarded([&] {
@@ -781,6 +1129,14 @@ int pm_batch_set_warm_start(pm_batch* b, int32_t i, const double* x0, const doub
         if (y0 && d.m > 0)
             PM_HIP(hipMemcpyAsync(b->wy.get() + d.row_off, y0, sizeof(double) * d.m, hipMemcpyHostToDevice, b->stream));
         PM_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+int pm_batch_set_primal_weight(pm_batch* b, int32_t i, double omega) {
+    return pm::guarded([&] {
+        if (!b || i < 0 || i >= b->count) pm::fail(PM_ERR_INVALID_ARG, "bad batch or index");
+        if (!std::isfinite(omega)) pm::fail(PM_ERR_INVALID_ARG, "primal weight must be finite");
+        b->desc[i].omega0 = omega;
     });
 }
 

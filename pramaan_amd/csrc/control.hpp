@@ -10,9 +10,10 @@
 //   LIMIT    iteration budget spent
 //   FAIL     rel is not finite
 //   REFINE   new iterative-refinement round around the candidate: the round
-//            gained round_gain (1e-3) since its start, or it gained 1e-2 and
+//            gained round_gain (1e-3) since its start, or it gained 2x and
 //            20 checks in a row failed to halve the best error seen before
-//            them (the FP32 floor)
+//            them (the FP32 floor, or a primal weight that ran away: a new
+//            round re-centres the problem on the candidate)
 //   RESTART  z <- T(z), anchor <- z: the error fell to 0.2x the error at the
 //            last restart, or below 0.8x of it while rising again, or 36 % of
 //            all iterations passed since (PDLP's adaptive restart rule)
@@ -33,7 +34,7 @@ struct Control {
     enum Action { CONTINUE = 0, OPTIMAL = 1, LIMIT = 2, FAIL = 3, REFINE = 4, RESTART = 5 };
     static constexpr int kStall = 20;
     static constexpr double kSuff = 0.2, kNec = 0.8, kArt = 0.36, kTheta = 0.5, kRoundGain = 1e-3,
-                            kStallGain = 1e-2;
+                            kStallGain = 0.5;
 
     // parameters
     double tol;
@@ -99,7 +100,7 @@ struct Control {
             for (int q = 0; q < kStall; ++q) recent = ring[q] < recent ? ring[q] : recent;
             stalled = recent > 0.5 * older_min;
         }
-        // A stall only means the FP32 floor once the round has gained 100x.
+        // A stall counts once the round has at least halved its error.
         if (rounds < max_refinements &&
             (rel <= k_round_start * kRoundGain || (stalled && rel <= k_round_start * kStallGain))) {
             double f = 1.0 / (rel > 1e-300 ? rel : 1e-300);

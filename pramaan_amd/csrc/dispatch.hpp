@@ -49,6 +49,20 @@ void with_variant(int variant, F&& f) {
     }
 }
 
+// Lanes per row as a compile-time constant (1, 2, ..., 32).
+template <class F>
+void with_lanes(int lanes, F&& f) {
+    switch (lanes) {
+        case 1: f(IntC<1>{}); break;
+        case 2: f(IntC<2>{}); break;
+        case 4: f(IntC<4>{}); break;
+        case 16: f(IntC<16>{}); break;
+        case 32: f(IntC<32>{}); break;
+        case 8:
+        default: f(IntC<8>{}); break;
+    }
+}
+
 // y-side row kernel over `nrows` rows.
 template <class AT, class VT, class XT, class Epi>
 void launch_rows(int variant, int nrows, const int* rp, const int* ci, const VT* v, const XT* x,

@@ -29,7 +29,9 @@ def test_batch_matches_closed_form(capi, oracle):
         k = oracle.relative_kkt(lp, r.x, r.y)
         assert k["rel"] <= 1.2e-7
         assert abs(k["rel"] - r.rel_kkt) <= 1e-12 + 1e-6 * r.rel_kkt
-        assert abs(r.objective - lp.p_star) <= 1e-6 * (1 + abs(lp.p_star))
+        # A relative KKT of 1e-7 leaves primal residuals of ~1e-7 (1 + |b|);
+        # priced by |y*| they move the objective by up to ~1e-5.
+        assert abs(r.objective - lp.p_star) <= 1e-5 * (1 + abs(lp.p_star))
 
 
 def test_batch_replay_is_deterministic(capi):
@@ -39,6 +41,52 @@ def test_batch_replay_is_deterministic(capi):
         r2, _, _ = b.run(want_vectors=True)
     for a, c in zip(r1, r2):
         assert a.iterations == c.iterations and np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y)
+
+
+def test_batch_direct_launches_match_graph(capi):
+    """The captured graph and the same launches issued directly give the
+    same bits (every reduction in the engine has a fixed order)."""
+    lps = small_batch(capi, count=6, seed0=900)
+    with capi.Batch(lps, tolerance=1e-6, check_frequency=32) as b:
+        rg, _, _ = b.run(want_vectors=True)
+        b.set_graph(False)
+        rd, _, _ = b.run(want_vectors=True)
+    for a, c in zip(rg, rd):
+        assert a.status == c.status == "OPTIMAL"
+        assert a.iterations == c.iterations and np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y)
+
+
+def test_batch_one_lp_and_mixed_sizes(capi):
+    """A batch of one LP and a batch whose sizes differ 100x (the work list
+    and the per-LP chunk lengths), against each LP's closed form."""
+    tiny = capi.generate(None, seed=11, m=5, n=12, k_min=1, k_max=3)
+    big = capi.generate(None, seed=12, m=600, n=1500, k_min=4, k_max=16)
+    for lps in ([big], [tiny, big, tiny], [big, tiny]):
+        with capi.Batch(lps, tolerance=1e-7, max_iterations=50000, check_frequency=64) as b:
+            res, _, _ = b.run()
+        for lp, r in zip(lps, res):
+            assert r.status == "OPTIMAL", (lp.m, r.iterations, r.rel_kkt)
+            assert abs(r.objective - lp.p_star) <= 1e-5 * (1 + abs(lp.p_star))
+
+
+def test_batch_iteration_limit(capi):
+    """max_iterations that is not a multiple of check_frequency: the last
+    chunk is cut short and the LP reports the limit with that count."""
+    lps = small_batch(capi, count=3, seed0=40)
+    with capi.Batch(lps, tolerance=1e-12, max_iterations=100, check_frequency=64) as b:
+        res, _, _ = b.run()
+    for r in res:
+        assert r.status == "ITERATION_LIMIT" and r.iterations == 100
+
+
+def test_batch_kernel_timing(capi):
+    lps = small_batch(capi, count=8, seed0=60)
+    with capi.Batch(lps, tolerance=1e-6, check_frequency=64) as b:
+        for which in (capi.PM_KERNEL_SPMV_A, capi.PM_KERNEL_SPMV_AT):
+            ms, by = b.time_kernel(which, 5)
+            assert ms > 0 and by > 8 * sum(lp.nnz for lp in lps)
+        res, _, _ = b.run()  # a run after timing starts afresh
+    assert all(r.status == "OPTIMAL" for r in res)
 
 
 def test_batch_agrees_with_single_lp_engine(capi):
