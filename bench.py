@@ -1,17 +1,35 @@
 #!/usr/bin/env python3
-"""Benchmark: time to relative-KKT tolerance of the GPU PDHG engine on the
-single large synthetic LP (m = 500,000 rows, n = 1,000,000 columns, 12
-nonzeros per column = 12M nonzeros, 60 % equality rows, +-5 % value noise).
+"""Benchmark: seconds to 1e-6 relative KKT on the 500-window rolling-horizon
+blending sequence (24-period windows shifted by one period; 4,656 rows x
+9,600 columns and 97k nonzeros per window), learned warm start vs plain
+cold-start PDHG, on one GPU, plus the SpMV HBM bandwidth.
 
-A step is one complete solve from a cold start (reflected Halpern PDHG in
-FP32 with FP64 KKT checks and refinement) with the LP already resident in
-HBM.  Prints one JSON line with the mean seconds per solve, the roofline of
-the dominant kernel (the fused A x / dual-update SpMV, timed with HIP events
-on the engine's stream) and a CPU baseline (the oracle's restatement of the
-reference PDHG, single thread, on a bounded number of iterations of the same
-LP, scaled to the GPU solve's iteration count).
+A step solves all 500 windows from scratch with every window already
+resident in HBM (one pm_batch):
+  learned = the network's graph and features built on the device from the
+            resident windows, one forward pass for all windows, each
+            prediction judged against the cold start by its FP64 relative
+            KKT error, then one batched, graph-captured PDHG solve
+            (reflected Halpern, FP32 iterations, FP64 checks and
+            refinement) of all windows to 1e-6;
+  cold    = the same solve from x = 0, y = 0.
+`value` is the learned step's mean wall time (features + inference + solve);
+the cold step is timed the same way beside it.  The network weights are
+the ones trained by `python -m pramaan_amd.rolling` (pramaan_amd/data/).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--tol 1e-6]
+roofline: the dominant iterate kernel of the batched solve (A^T y with the
+fused primal update, or A xe with the fused dual update, over all 500
+windows), timed with HIP events on the batch's stream; algorithmic bytes
+per launch = sum over windows of 8 nnz + 4 (n+1) + 4 m + 28 n (A^T) or
+8 nnz + 4 (m+1) + 4 n + 20 m (A).  spmv_lp1: the same for the single-LP
+engine's fused SpMV on the 500k x 1M LP (12M nonzeros).
+cpu_baseline: the oracle's restatement of the reference PDHG (FP32, one
+thread) timed on window 0 for a bounded number of iterations; value =
+seconds per iteration x the iterations the cold GPU solve used over all
+windows.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --workload lp1   (the single 500k x 1M LP instead)
 """
 from __future__ import annotations
 
@@ -22,115 +40,208 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+WEIGHTS = ROOT / "pramaan_amd" / "data" / "warmstart_gnn.npz"
+METRIC = "s to 1e-6 rel KKT, learned warm-start vs plain PDHG on 1 GPU; SpMV HBM GB/s"
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(lp, gpu_iterations: int, budget_s: float) -> dict:
-    """Restated reference PDHG (oracle, FP32, 1 thread) on the same LP: time a
-    bounded number of iterations, report iterations/s and the projected time
-    for the GPU solve's iteration count."""
-    import numpy as np  # noqa: F401
+def oracle_seconds_per_iteration(lp, budget_s: float):
+    """Restated reference PDHG (oracle, FP32, 1 thread): seconds per
+    iteration on lp, setup subtracted, over a bounded run."""
     from oracle import binding as O
     t0 = time.perf_counter()
     O.pdhg_solve(lp, tol=0.0, max_iterations=0, check_frequency=10**9)
     setup = time.perf_counter() - t0
-    iters = 2
+    iters = 20
     t0 = time.perf_counter()
     O.pdhg_solve(lp, tol=0.0, max_iterations=iters, check_frequency=10**9)
     per_it = max((time.perf_counter() - t0 - setup) / iters, 1e-9)
-    iters = max(2, min(400, int(budget_s / per_it)))
+    iters = max(20, min(200000, int(budget_s / per_it)))
     t0 = time.perf_counter()
     O.pdhg_solve(lp, tol=0.0, max_iterations=iters, check_frequency=10**9)
-    per_it = max((time.perf_counter() - t0 - setup) / iters, 1e-9)
-    return {"value": per_it * gpu_iterations, "unit": "s", "cores": 1, "kind": "port",
+    return max((time.perf_counter() - t0 - setup) / iters, 1e-9), iters
+
+
+def spmv_lp1(P, reps: int) -> dict:
+    lp = P.generate(1, seed=1)
+    with P.Context(0) as ctx:
+        ctx.load(lp, algorithm="pdlp", termination="relative_kkt", tolerance=1e-6, max_iterations=64,
+                 check_frequency=64)
+        out = {}
+        for name, kid in (("spmv_a_dual", P.PM_KERNEL_SPMV_A), ("spmv_at_primal", P.PM_KERNEL_SPMV_AT)):
+            ms, b = ctx.time_kernel(kid, reps)
+            out[name] = {"us": ms * 1e3, "alg_bytes": b, "GBps": b / (ms * 1e-3) / 1e9,
+                         "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        out["variants"] = ctx.spmv_variants()
+        out["lp"] = {"m": lp.m, "n": lp.n, "nnz": lp.nnz}
+    return out
+
+
+def bench_rolling(args, P) -> dict:
+    from pramaan_amd import rolling as R
+    if not WEIGHTS.exists():
+        raise SystemExit(f"{WEIGHTS} is missing: train it with `python -m pramaan_amd.rolling --out {WEIGHTS}`")
+    rh = P.RollingHorizon()
+    nw = min(args.windows, rh.windows)
+    t0 = time.perf_counter()
+    lps = [rh.window(w) for w in range(nw)]
+    net = R.load(WEIGHTS)
+    b = P.Batch(lps, tolerance=args.tol, max_iterations=args.max_iters, check_frequency=64)
+    log(f"{nw} windows resident in {time.perf_counter() - t0:.1f}s; spmv lanes (A, A^T) {b.spmv_variants()}")
+    cpp, rpp = rh.cols_per_period, rh.rows_per_period
+
+    def clear():
+        for i in range(nw):
+            b.set_warm_start(i, None, None)
+
+    def cold_step():
+        clear()
+        t = time.perf_counter()
+        res, dev, _ = b.run()
+        return time.perf_counter() - t, dev, res
+
+    def learned_step():
+        clear()
+        t = time.perf_counter()
+        inf = b.learned_warm_start(net, cpp, rpp)
+        res, dev, _ = b.run()
+        return time.perf_counter() - t, dev + inf, res, inf
+
+    for w in range(args.warmup):
+        cold_step()
+        learned_step()
+    cold_t, learned_t, infer_t = [], [], []
+    cold_res = learned_res = None
+    for s in range(args.steps):
+        wall, dev, cold_res = cold_step()
+        cold_t.append(wall)
+        wall2, dev2, learned_res, inf = learned_step()
+        learned_t.append(wall2)
+        infer_t.append(inf)
+        log(f"step {s}: cold {wall:.3f}s (device {dev:.3f}s)  learned {wall2:.3f}s (device {dev2:.3f}s, "
+            f"inference {inf * 1e3:.1f} ms)")
+    acc, start_kkt = b.warm_start_verdicts()
+    sec_l, sec_c = float(np.mean(learned_t)), float(np.mean(cold_t))
+
+    it_c = np.array([r.iterations for r in cold_res])
+    it_l = np.array([r.iterations for r in learned_res])
+    fx = json.loads((ROOT / "tests" / "golden" / "fixtures.json").read_text())["rolling"]
+    obj_err = max(abs(learned_res[int(w)].objective - v) / (1 + abs(v)) for w, v in fx.items() if int(w) < nw)
+
+    kern = {}
+    for name, kid in (("batch_spmv_a_dual", P.PM_KERNEL_SPMV_A), ("batch_spmv_at_primal", P.PM_KERNEL_SPMV_AT)):
+        ms, by = b.time_kernel(kid, args.kernel_reps)
+        kern[name] = (ms, by)
+        log(f"kernel {name}: {ms * 1e3:.1f} us, {by / 1e6:.1f} MB -> {by / ms / 1e6:.0f} GB/s")
+    dom = max(kern, key=lambda k: kern[k][0])
+    ms, by = kern[dom]
+    achieved = by / (ms * 1e-3) / 1e9
+    b.close()
+
+    out = {
+        "metric": METRIC, "value": sec_l, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec_l * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (rolling-horizon blending plan, 523 periods; network trained on perturbed "
+                                "plans solved by this engine)",
+        "config": {"workload": f"{nw}-window rolling-horizon sequence, 24-period windows of "
+                               f"{lps[0].m} rows x {lps[0].n} cols ({lps[0].nnz} nnz), solved to rel KKT "
+                               f"{args.tol:g} in one batch: learned warm start (value) vs cold start",
+                   "windows": nw, "tolerance": args.tol, "rows": lps[0].m, "cols": lps[0].n, "nnz": lps[0].nnz},
+        "cold_seconds": sec_c, "learned_seconds": sec_l, "speedup_learned_vs_cold": sec_c / sec_l,
+        "inference_ms": 1e3 * float(np.mean(infer_t)),
+        "optimal": {"cold": int(sum(r.status == "OPTIMAL" for r in cold_res)),
+                    "learned": int(sum(r.status == "OPTIMAL" for r in learned_res))},
+        "iterations": {"cold_mean": float(it_c.mean()), "cold_max": int(it_c.max()),
+                       "learned_mean": float(it_l.mean()), "learned_max": int(it_l.max())},
+        "warm_starts_kept": int((acc == 1).sum()), "warm_starts_replaced_by_cold": int((acc == 0).sum()),
+        "objective_rel_error_vs_highs": obj_err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "kernel_us": ms * 1e3,
+                     "alg_bytes_per_launch": by},
+        "batch_kernels": {k: {"us": v[0] * 1e3, "alg_bytes": v[1], "GBps": v[1] / v[0] / 1e6}
+                          for k, v in kern.items()},
+    }
+    if not args.no_lp1:
+        out["spmv_lp1"] = spmv_lp1(P, args.kernel_reps)
+    if not args.no_cpu:
+        per_it, iters = oracle_seconds_per_iteration(lps[0], args.cpu_budget)
+        out["cpu_baseline"] = {
+            "value": per_it * float(it_c.sum()), "unit": "s", "cores": 1, "kind": "port",
             "iterations_per_s": 1.0 / per_it,
-            "sample": (f"{iters} iterations of the oracle's restated reference PDHG (FP32, FP64 setup) on the "
-                       f"same LP, single thread, setup subtracted; value = s/iteration x the GPU solve's "
-                       f"{gpu_iterations} iterations")}
+            "sample": (f"{iters} iterations of the oracle's restated reference PDHG (FP32, 1 thread) on window 0, "
+                       f"setup subtracted; value = s/iteration x the {int(it_c.sum())} iterations the cold GPU "
+                       f"solve used over all {nw} windows")}
+    return out
+
+
+def bench_lp1(args, P) -> dict:
+    lp = P.generate(1, seed=1)
+    ctx = P.Context(0)
+    ctx.load(lp, algorithm="pdlp", termination="relative_kkt", tolerance=args.tol, max_iterations=args.max_iters,
+             check_frequency=64)
+    for _ in range(args.warmup):
+        ctx.solve()
+    times, res = [], None
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        res = ctx.solve()
+        times.append(time.perf_counter() - t0)
+        log(f"step {s}: {res.status} it={res.iterations} kkt={res.rel_kkt:.2e} t={times[-1]:.3f}s")
+    sec = float(np.mean(times))
+    kern = {}
+    for name, kid in (("spmv_a_dual", P.PM_KERNEL_SPMV_A), ("spmv_at_primal", P.PM_KERNEL_SPMV_AT)):
+        kern[name] = ctx.time_kernel(kid, args.kernel_reps)
+    dom = max(kern, key=lambda k: kern[k][0])
+    ms, by = kern[dom]
+    achieved = by / (ms * 1e-3) / 1e9
+    ctx.close()
+    out = {
+        "metric": METRIC, "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"single LP m={lp.m} n={lp.n} nnz={lp.nnz}, cold start to rel KKT {args.tol:g}",
+                   "m": lp.m, "n": lp.n, "nnz": lp.nnz, "tolerance": args.tol},
+        "status": res.status, "iterations": res.iterations, "rel_kkt": res.rel_kkt,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "kernel_us": ms * 1e3,
+                     "alg_bytes_per_launch": by},
+    }
+    if not args.no_cpu:
+        per_it, iters = oracle_seconds_per_iteration(lp, args.cpu_budget)
+        out["cpu_baseline"] = {"value": per_it * res.iterations, "unit": "s", "cores": 1, "kind": "port",
+                               "sample": f"{iters} oracle iterations on the same LP x the GPU's {res.iterations}"}
+    return out
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["rolling500", "lp1"], default="rolling500")
+    ap.add_argument("--windows", type=int, default=500)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--max-iters", type=int, default=60000)
-    ap.add_argument("--workload", type=int, default=1, help="generator id: 1 = 500k x 1M, 2 = 1/10 scale")
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--max-iters", type=int, default=200000)
+    ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--no-lp1", action="store_true", help="skip the 500k x 1M SpMV measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every LP of this workload is solved on a single device; "
-              "multi-GPU runs are not part of this build (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: this build runs every LP of the workload on a single device and has no "
+              "multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
-
     from pramaan_amd import _capi as P
-
-    t0 = time.perf_counter()
-    lp = P.generate(args.workload, seed=args.seed)
-    log(f"generated m={lp.m} n={lp.n} nnz={lp.nnz} in {time.perf_counter() - t0:.1f}s")
-    ctx = P.Context(0)
-    t0 = time.perf_counter()
-    ctx.load(lp, algorithm="pdlp", termination="relative_kkt", tolerance=args.tol,
-             max_iterations=args.max_iters, check_frequency=64, verbose=1 if args.verbose else 0)
-    log(f"loaded in {time.perf_counter() - t0:.1f}s, spmv variants {ctx.spmv_variants()}")
-
-    for w in range(args.warmup):
-        r = ctx.solve(want_vectors=False)
-        log(f"warmup {w}: {r.status} it={r.iterations} kkt={r.rel_kkt:.2e} t={r.device_seconds:.3f}s")
-    times, res = [], None
-    for s in range(args.steps):
-        t0 = time.perf_counter()
-        res = ctx.solve(want_vectors=False)  # ends with a device synchronisation
-        times.append(time.perf_counter() - t0)
-        log(f"step {s}: {res.status} it={res.iterations} kkt={res.rel_kkt:.2e} t={times[-1]:.3f}s")
-    sec = sum(times) / len(times)
-
-    # Roofline of the dominant kernel, timed with HIP events on the engine stream.
-    kern = {}
-    for name, kid in (("spmv_a_dual", P.PM_KERNEL_SPMV_A), ("spmv_at_primal", P.PM_KERNEL_SPMV_AT)):
-        ms, b = ctx.time_kernel(kid, args.kernel_reps)
-        kern[name] = (ms, b)
-        log(f"kernel {name}: {ms * 1e3:.1f} us, {b / 1e6:.1f} MB -> {b / ms / 1e6:.0f} GB/s")
-    dom = max(kern, key=lambda k: kern[k][0])
-    ms, b = kern[dom]
-    achieved = b / (ms * 1e-3) / 1e9
-    it_ms, it_b = ctx.time_kernel(P.PM_KERNEL_ITERATION, args.kernel_reps)
-
-    objerr = abs(res.primal_objective - lp.p_star) / (1 + abs(lp.p_star))
-    out = {
-        "metric": "s to 1e-6 rel KKT, learned warm-start vs plain PDHG on 1 GPU; SpMV HBM GB/s",
-        "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"single sparse LP m={lp.m} n={lp.n} nnz={lp.nnz} (60% equality rows, 5% noise), "
-                               f"cold-start PDHG to rel KKT {args.tol:g}",
-                   "m": lp.m, "n": lp.n, "nnz": lp.nnz, "tolerance": args.tol, "seed": args.seed,
-                   "algorithm": "reflected Halpern PDHG, FP32 iterations, FP64 KKT + refinement"},
-        "status": res.status, "iterations": res.iterations, "rel_kkt": res.rel_kkt,
-        "restarts": res.restarts, "refinements": res.refinements,
-        "objective_rel_error_vs_closed_form": objerr,
-        "iterations_per_s": res.iterations / sec if sec > 0 else None,
-        "spmv_variants": ctx.spmv_variants(),
-        "iteration_kernels_us": it_ms * 1e3, "iteration_alg_bytes": it_b,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                     "kernel_us": ms * 1e3, "alg_bytes_per_launch": b},
-    }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(lp, res.iterations, args.cpu_budget)
-    ctx.close()
+    out = bench_rolling(args, P) if args.workload == "rolling500" else bench_lp1(args, P)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -29,6 +29,27 @@ extern "C" {
 #define PM_GNN_HIDDEN 64
 #define PM_GNN_FEATURES 8
 
+struct pm_lp;
+
+/* Host-side graph and features of one LP (no device work):
+ *   cp[n+1], ri[nnz], ac[nnz]  A^T in CSR (rows ascending per column), each
+ *                              column scaled to unit max-norm;
+ *   ar[nnz]                    A's values (CSR order of lp) with each row
+ *                              scaled to unit max-norm;
+ *   xc[n][8] column features   c/cs, l/s, u/s (0 when infinite), l finite,
+ *                              u finite, nnz/max nnz, period position, 1;
+ *   xr[m][8] row features      l/s, u/s, l finite, u finite, equality,
+ *                              nnz/max nnz, period position, 1;
+ *   scales[2] = {s, cs}        s = max(1, largest finite |bound|),
+ *                              cs = max(1e-12, max |c|).
+ * The period position is (index / period) / (count / period - 1) when a
+ * period length is given (> 0), else 0.  Targets of the network are x / s
+ * and y / cs. */
+int pm_gnn_lp_features(const struct pm_lp* lp, int32_t period_cols,
+                       int32_t period_rows, int32_t* cp, int32_t* ri,
+                       float* ar, float* ac, float* xr, float* xc,
+                       double* scales);
+
 typedef struct pm_gnn pm_gnn;
 
 /* Number of FP32 parameters. */
@@ -56,6 +77,24 @@ int pm_gnn_gradient(pm_gnn* g, const float* tx, const float* ty, float* loss,
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the current graph. */
 int pm_gnn_train_step(pm_gnn* g, const float* tx, const float* ty, float lr,
                       float* loss);
+
+/* Learned warm start of a resident batch (pramaan_hip.h): the graph and
+ * features of every LP of the batch are built on the device from the
+ * batch's own arrays, exactly as pm_gnn_lp_features builds them on the host
+ * (period lengths as there), one forward pass runs over the block-diagonal
+ * graph of all LPs, and x0 = clamp(xhat * s, l, u), y0 = yhat * cs become
+ * the warm starts of every LP for the following pm_batch_run, which judges
+ * each against the cold start.  The network's current graph is replaced
+ * (call pm_gnn_set_graph again before training).  *seconds (may be NULL)
+ * receives the device time of the whole step. */
+struct pm_batch;
+int pm_batch_learned_warm_start(struct pm_batch* b, pm_gnn* g,
+                                int32_t period_cols, int32_t period_rows,
+                                double* seconds);
+/* The warm start LP i of the batch will use (original space, y in the
+ * reference sign); fails if it has none. */
+int pm_batch_get_warm_start(struct pm_batch* b, int32_t i, double* x0,
+                            double* y0);
 
 #ifdef __cplusplus
 }

@@ -229,9 +229,18 @@ int pm_batch_set_warm_start(pm_batch* b, int32_t i, const double* x0,
  * Ruiz / Pock-Chambolle scaled problem, as pm_pdhg_result.primal_weight
  * reports it); omega <= 0 restores the default ||c_hat||_2 / ||b_hat||_2. */
 int pm_batch_set_primal_weight(pm_batch* b, int32_t i, double omega);
-/* Solves every LP of the batch; results[i].x/.y filled when non-NULL. */
+/* Solves every LP of the batch; results[i].x/.y filled when non-NULL.
+ * A warm start is judged first: its FP64 relative KKT error is compared
+ * with that of the cold start (x = 0 clipped to the bounds, y = 0), and
+ * the LP starts from whichever is smaller. */
 int pm_batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats);
 int pm_batch_destroy(pm_batch* b);
+/* After a run, per LP (arrays of count, either may be NULL): accepted = 1
+ * if its warm start was used, 0 if the cold start replaced it, -1 without
+ * a warm start; start_rel_kkt = FP64 relative KKT error of the starting
+ * point used (-1 without a warm start). */
+int pm_batch_warm_start_verdicts(pm_batch* b, int32_t* accepted,
+                                 double* start_rel_kkt);
 /* use_graph = 0 issues the same launches directly on the stream instead of
  * replaying the captured graph (for profilers that do not follow graph
  * launches); results are identical.  Default 1. */

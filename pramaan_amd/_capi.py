@@ -92,6 +92,9 @@ SIGNATURES = {
     "pm_batch_run": (C.c_int, [C.c_void_p, C.POINTER(pm_pdhg_result), C.POINTER(pm_batch_stats)]),
     "pm_batch_destroy": (C.c_int, [C.c_void_p]),
     "pm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pm_batch_learned_warm_start": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, _d]),
+    "pm_batch_get_warm_start": (C.c_int, [C.c_void_p, C.c_int32, _d, _d]),
+    "pm_batch_warm_start_verdicts": (C.c_int, [C.c_void_p, _i, _d]),
     "pm_batch_set_primal_weight": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
     "pm_batch_spmv_variants": (C.c_int, [C.c_void_p, _i, _i]),
     "pm_batch_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _d, _d]),
@@ -108,6 +111,7 @@ SIGNATURES = {
     "pm_gnn_set_params": (C.c_int, [C.c_void_p, _f]),
     "pm_gnn_set_graph": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _i, _i, _f, _i, _i, _f, _f, _f]),
     "pm_gnn_forward": (C.c_int, [C.c_void_p, _f, _f]),
+    "pm_gnn_lp_features": (C.c_int, [C.POINTER(pm_lp), C.c_int32, C.c_int32, _i, _i, _f, _f, _f, _f, _d]),
     "pm_gnn_gradient": (C.c_int, [C.c_void_p, _f, _f, _f, _f]),
     "pm_gnn_train_step": (C.c_int, [C.c_void_p, _f, _f, C.c_float, _f]),
     "pm_rh_params_default": (C.c_int, [C.POINTER(pm_rh_params)]),
@@ -376,6 +380,26 @@ class Batch:
                for i in range(k)]
         return out, st.device_seconds, inst
 
+    def learned_warm_start(self, net: "Gnn", period_cols: int = 0, period_rows: int = 0) -> float:
+        """Network predictions as the warm starts of every LP, built on the
+        device; returns the device seconds of the step."""
+        secs = C.c_double()
+        _check(lib().pm_batch_learned_warm_start(self._h, net._h, period_cols, period_rows, C.byref(secs)))
+        return secs.value
+
+    def get_warm_start(self, i: int) -> tuple[np.ndarray, np.ndarray]:
+        lp = self.lps[i]
+        x, y = np.zeros(lp.n), np.zeros(lp.m)
+        _check(lib().pm_batch_get_warm_start(self._h, i, _p(x, C.c_double), _p(y, C.c_double)))
+        return x, y
+
+    def warm_start_verdicts(self) -> tuple[np.ndarray, np.ndarray]:
+        """(accepted: 1 used / 0 replaced by cold / -1 none, FP64 rel. KKT of the start used)."""
+        acc = np.zeros(len(self.lps), np.int32)
+        kkt = np.zeros(len(self.lps))
+        _check(lib().pm_batch_warm_start_verdicts(self._h, _p(acc, C.c_int32), _p(kkt, C.c_double)))
+        return acc, kkt
+
     def set_primal_weight(self, i: int, omega: float) -> None:
         _check(lib().pm_batch_set_primal_weight(self._h, i, float(omega)))
 
@@ -491,6 +515,55 @@ def cxx() -> C.CDLL:
         L.pm_cert_verify.argtypes = [C.POINTER(pm_lp), C.c_char_p, C.c_char_p, C.c_int]
         _cxx = L
     return _cxx
+
+
+class pm_ladder_options(C.Structure):
+    _fields_ = [("residual_threshold", C.c_double), ("gpu_max_iterations", C.c_int32),
+                ("gpu_check_frequency", C.c_int32), ("gpu_tolerance", C.c_double), ("gpu_refined_rung", C.c_int32),
+                ("refined_max_iterations", C.c_int32)]
+
+
+class pm_ladder_result(C.Structure):
+    _fields_ = [("stage", C.c_int32), ("escalated", C.c_int32), ("escalation_reason", C.c_int32),
+                ("gpu_status", C.c_int32), ("gpu_iterations", C.c_int32), ("gpu_primal_residual", C.c_double),
+                ("gpu_dual_residual", C.c_double), ("gpu_objective", C.c_double), ("refined_rung_run", C.c_int32),
+                ("refined_status", C.c_int32), ("refined_iterations", C.c_int32), ("refined_rel_kkt", C.c_double),
+                ("polish_unavailable", C.c_int32), ("final_status", C.c_int32), ("final_objective", C.c_double),
+                ("final_primal_residual", C.c_double), ("final_dual_residual", C.c_double),
+                ("x", C.POINTER(C.c_double))]
+
+
+LADDER_STAGES = ("GPU_FP32", "CPU_FP64_POLISH", "GPU_FP64_REFINED")
+ESCALATION_REASONS = ("none", "high_primal_residual", "high_dual_residual", "gpu_iteration_limit",
+                      "gpu_numerical_failure", "gpu_not_optimal", "gpu_disabled")
+
+
+def ladder_solve(lp: LP, residual_threshold=1e-6, gpu_max_iterations=100000, gpu_check_frequency=100,
+                 gpu_tolerance=1e-6, refined_rung=False, refined_max_iterations=200000) -> dict:
+    """pm_ladder_solve (include/pramaan_ladder.h); returns a dict with the
+    stage, escalation reason, per-rung figures and the final x."""
+    L = cxx()
+    if not hasattr(L.pm_ladder_solve, "_bound"):
+        L.pm_ladder_solve.restype = C.c_int
+        L.pm_ladder_solve.argtypes = [C.POINTER(pm_lp), C.POINTER(pm_ladder_options), C.POINTER(pm_ladder_result),
+                                      C.c_char_p, C.c_int]
+        L.pm_ladder_solve._bound = True
+    o = pm_ladder_options(residual_threshold, gpu_max_iterations, gpu_check_frequency, gpu_tolerance,
+                          1 if refined_rung else 0, refined_max_iterations)
+    x = np.zeros(lp.n)
+    r = pm_ladder_result()
+    r.x = _p(x, C.c_double)
+    err = C.create_string_buffer(1024)
+    s = lp.struct()
+    if L.pm_ladder_solve(C.byref(s), C.byref(o), C.byref(r), err, len(err)) != 0:
+        raise RuntimeError(err.value.decode())
+    out = {f: getattr(r, f) for f, _ in pm_ladder_result._fields_ if f != "x"}
+    out["stage"] = LADDER_STAGES[r.stage]
+    out["escalation_reason"] = ESCALATION_REASONS[r.escalation_reason]
+    for k in ("gpu_status", "refined_status", "final_status"):
+        out[k] = SOLVE_STATUS[out[k]]
+    out["x"] = x
+    return out
 
 
 def cert_write(lp: LP, x, path, y=None, status="OPTIMAL", tolerance=1e-6) -> None:

@@ -36,6 +36,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "batch_internal.hpp"
 #include "control.hpp"
 #include "dispatch.hpp"
 #include "prep.hpp"
@@ -74,30 +75,6 @@ inline int lanes_variant(int l) {
         default: return PM_SPMV_VECTOR8;
     }
 }
-
-struct BatchDesc {
-    int32_t m, n, nnz;
-    int32_t lanes_a, lanes_at;
-    int32_t warm_x, warm_y;
-    int32_t pad;
-    double omega0;  // initial primal weight; <= 0: ||c_hat|| / ||b_hat||
-    int64_t rp_off, cp_off, nz_off, row_off, col_off;
-    // results
-    int32_t status, iterations, restarts, refinements;
-    double rel_kkt, rel_p, rel_d, rel_g, pobj, dobj, omega;
-    uint64_t t_start, t_end;
-};
-
-// What the grid-wide iterate kernels need of one LP for the next chunk,
-// written by the init and check phases.
-struct LpIter {
-    float tau, sig;
-    int32_t k0;      // in-round Halpern index of the chunk's first iteration
-    int32_t chunk;   // iterations in the chunk (0: LP finished)
-    int32_t use_g;   // dual shift of a refinement round present
-    int32_t row_off, col_off;
-    int32_t pad;
-};
 
 template <class VT>
 struct BatchArrays {
@@ -285,31 +262,58 @@ __device__ __forceinline__ void publish_iter(LpIter* it, int lp, const BatchStat
     it[lp] = r;
 }
 
-// Phase 0: power iteration for ||A_hat||_2, initial primal weight, base points.
+// FP64 relative KKT of (xtot, ytot) into the descriptor; returns rel on
+// thread 0 (every thread must call).
+template <class VT>
+__device__ double batch_kkt(const View<VT>& v, BatchDesc& d, double* red, double* keep) {
+    brows_red_any(v.la, v.m, v.rp, v.ci, v.oa, v.xtot, EpiKktRow{v.lr, v.ur, v.ytot, v.dr}, red);
+    if (threadIdx.x < 7) keep[threadIdx.x] = red[threadIdx.x];
+    __syncthreads();
+    brows_red_any(v.lat, v.n, v.cp, v.ri, v.oat, v.ytot, EpiKktCol{v.cmin, v.lx, v.ux, v.xtot, v.dc}, red);
+    double rel = 0.0;
+    if (threadIdx.x == 0) {
+        const double p = red[4], dd2 = keep[5] + red[3];
+        const double rp = sqrt(keep[0]) / (1.0 + sqrt(keep[2]));
+        const double rdd = sqrt(keep[3] + red[0]) / (1.0 + sqrt(red[2]));
+        const double rg = fabs(p - dd2) / (1.0 + fabs(p) + fabs(dd2));
+        rel = rp > rdd ? rp : rdd;
+        rel = rel > rg ? rel : rg;
+        if (!(rel == rel)) rel = INFINITY;
+        d.rel_kkt = rel; d.rel_p = rp; d.rel_d = rdd; d.rel_g = rg; d.pobj = p; d.dobj = dd2;
+    }
+    __syncthreads();
+    return rel;
+}
+
+// Phase 0: power iteration for ||A_hat||_2 (first run of the batch only),
+// initial primal weight, base points, and the judgement of a warm start.
 template <class VT>
 __global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S) {
     const int lp = A.order[blockIdx.x];
     BatchDesc& d = A.desc[lp];
     const View<VT> v = view_of(A, d);
     const int m = v.m, n = v.n, tid = threadIdx.x;
-    __shared__ double red[16];
+    __shared__ double red[16], keep[8];
+    __shared__ double rel_s[2];
     const uint64_t t0 = wall_clock64();
-    for (int j = tid; j < n; j += kBB) v.xt[j] = rsqrtf(static_cast<float>(n));
-    __syncthreads();
     double lam = 0.0;
-    if (m > 0 && d.nnz > 0) {
-        for (int it = 0; it < 48; ++it) {
-            brows_any<float>(v.la, m, v.rp, v.ci, v.va, v.xt, EpiStore<float>{v.yt});
-            __syncthreads();
-            brows_any<float>(v.lat, n, v.cp, v.ri, v.vat, v.yt, EpiStore<float>{v.xe});
-            __syncthreads();
-            double acc[1] = {0.0};
-            for (int j = tid; j < n; j += kBB) acc[0] += static_cast<double>(v.xe[j]) * v.xe[j];
-            block_reduce_all<1, 0>(acc, red);
-            lam = red[0];
-            const double inv = lam > 0.0 ? 1.0 / sqrt(lam) : 0.0;
-            for (int j = tid; j < n; j += kBB) v.xt[j] = static_cast<float>(v.xe[j] * inv);
-            __syncthreads();
+    if (d.eta <= 0.0) {  // ||A_hat||_2 by power iteration, once per batch
+        for (int j = tid; j < n; j += kBB) v.xt[j] = rsqrtf(static_cast<float>(n));
+        __syncthreads();
+        if (m > 0 && d.nnz > 0) {
+            for (int it = 0; it < 48; ++it) {
+                brows_any<float>(v.la, m, v.rp, v.ci, v.va, v.xt, EpiStore<float>{v.yt});
+                __syncthreads();
+                brows_any<float>(v.lat, n, v.cp, v.ri, v.vat, v.yt, EpiStore<float>{v.xe});
+                __syncthreads();
+                double acc[1] = {0.0};
+                for (int j = tid; j < n; j += kBB) acc[0] += static_cast<double>(v.xe[j]) * v.xe[j];
+                block_reduce_all<1, 0>(acc, red);
+                lam = red[0];
+                const double inv = lam > 0.0 ? 1.0 / sqrt(lam) : 0.0;
+                for (int j = tid; j < n; j += kBB) v.xt[j] = static_cast<float>(v.xe[j] * inv);
+                __syncthreads();
+            }
         }
     }
     double acc[2] = {0.0, 0.0};
@@ -326,15 +330,44 @@ __global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S
         if (v.ur[i] < kInf && fabs(v.ur[i]) > bh) bh = fabs(v.ur[i]);
         const double t = v.dr[i] * bh;
         acc[1] += t * t;
-        v.ybase[i] = d.warm_y ? v.wy[i] : 0.0;
+        double yb = d.warm_y ? v.wy[i] : 0.0;
+        if (!(v.ur[i] < kInf) && yb > 0.0) yb = 0.0;  // the sign each row allows
+        if (!(v.lr[i] > -kInf) && yb < 0.0) yb = 0.0;
+        v.ybase[i] = yb;
     }
     block_reduce_all<2, 0>(acc, red);
+    const double c_norm2 = red[0], b_norm2 = red[1];
+    // A warm start is judged by its FP64 relative KKT error against the
+    // cold start's: the worse of the two is not used.
+    bool warm = d.warm_x || d.warm_y;
+    if (warm) {
+        for (int j = tid; j < n; j += kBB) v.xtot[j] = v.xbase[j];
+        for (int i = tid; i < m; i += kBB) v.ytot[i] = v.ybase[i];
+        __syncthreads();
+        const double rw = batch_kkt(v, d, red, keep);
+        if (tid == 0) rel_s[0] = rw;
+        for (int j = tid; j < n; j += kBB) v.xtot[j] = v.lx[j] > 0.0 ? v.lx[j] : (v.ux[j] < 0.0 ? v.ux[j] : 0.0);
+        for (int i = tid; i < m; i += kBB) v.ytot[i] = 0.0;
+        __syncthreads();
+        const double r0 = batch_kkt(v, d, red, keep);
+        if (tid == 0) rel_s[1] = r0;
+        __syncthreads();
+        warm = rel_s[0] <= rel_s[1];
+        if (!warm) {
+            for (int j = tid; j < n; j += kBB) v.xbase[j] = 0.0;
+            for (int i = tid; i < m; i += kBB) v.ybase[i] = 0.0;
+        }
+        __syncthreads();
+    }
     if (tid == 0) {
         BatchState& st = S[lp];
-        double om = (red[0] > 1e-20 && red[1] > 1e-20) ? sqrt(red[0]) / sqrt(red[1]) : 1.0;
+        double om = (c_norm2 > 1e-20 && b_norm2 > 1e-20) ? sqrt(c_norm2) / sqrt(b_norm2) : 1.0;
         if (d.omega0 > 0.0) om = d.omega0;
-        st.ctl.init(A.tol, A.max_iterations, A.max_refinements, om, d.warm_y != 0 && m > 0);
-        st.eta = lam > 0.0 ? 0.99 / sqrt(sqrt(lam)) : 1.0;
+        if (d.eta <= 0.0) d.eta = lam > 0.0 ? 0.99 / sqrt(sqrt(lam)) : 1.0;
+        d.warm_accepted = (d.warm_x || d.warm_y) ? (warm ? 1 : 0) : -1;
+        d.start_kkt = (d.warm_x || d.warm_y) ? (warm ? rel_s[0] : rel_s[1]) : -1.0;
+        st.ctl.init(A.tol, A.max_iterations, A.max_refinements, om, warm && d.warm_y != 0 && m > 0);
+        st.eta = d.eta;
         st.status = PM_SOLVE_ITERATION_LIMIT;
         st.active = 1;
         st.round_start = 1;
@@ -476,29 +509,6 @@ __global__ __launch_bounds__(kBlock) void k_s_dual(BatchArrays<VT> A, const int4
     }
 }
 
-// FP64 relative KKT of (xtot, ytot) into the descriptor; returns rel on
-// thread 0 (every thread must call).
-template <class VT>
-__device__ double batch_kkt(const View<VT>& v, BatchDesc& d, double* red, double* keep) {
-    brows_red_any(v.la, v.m, v.rp, v.ci, v.oa, v.xtot, EpiKktRow{v.lr, v.ur, v.ytot, v.dr}, red);
-    if (threadIdx.x < 7) keep[threadIdx.x] = red[threadIdx.x];
-    __syncthreads();
-    brows_red_any(v.lat, v.n, v.cp, v.ri, v.oat, v.ytot, EpiKktCol{v.cmin, v.lx, v.ux, v.xtot, v.dc}, red);
-    double rel = 0.0;
-    if (threadIdx.x == 0) {
-        const double p = red[4], dd2 = keep[5] + red[3];
-        const double rp = sqrt(keep[0]) / (1.0 + sqrt(keep[2]));
-        const double rdd = sqrt(keep[3] + red[0]) / (1.0 + sqrt(red[2]));
-        const double rg = fabs(p - dd2) / (1.0 + fabs(p) + fabs(dd2));
-        rel = rp > rdd ? rp : rdd;
-        rel = rel > rg ? rel : rg;
-        if (!(rel == rel)) rel = INFINITY;
-        d.rel_kkt = rel; d.rel_p = rp; d.rel_d = rdd; d.rel_g = rg; d.pobj = p; d.dobj = dd2;
-    }
-    __syncthreads();
-    return rel;
-}
-
 // Phase 3: candidate -> FP64 KKT -> Control decision -> restart / refine /
 // finish.  Counts the LPs still active.
 template <class VT>
@@ -598,42 +608,6 @@ __global__ void k_zero_int(int* p) { *p = 0; }
 }  // namespace pm
 
 using namespace pm;
-
-struct pm_batch {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int count = 0;
-    bool exact = true;
-    double tol = 1e-6;
-    int max_iterations = 100000, check_frequency = 64, max_refinements = 8;
-    std::vector<BatchDesc> desc;
-    std::vector<int> order;
-    std::vector<int> sense;
-    std::vector<double> offset;
-    std::vector<double> c_model;  // model-sense costs, concatenated
-    int64_t rows_total = 0, cols_total = 0, nnz_total = 0;
-    DevBuf<int> rp, ci, cp, ri, d_order;
-    DevBuf<float> va, vat, oa32, oat32;
-    DevBuf<double> oa64, oat64;
-    DevBuf<double> cmin, lr, ur, lx, ux, dr, dc, wx, wy;
-    DevBuf<float> c32, lr32, ur32, lx32, ux32, g32, x, xe, x0, xt, xl, y, y0, yt, yl;
-    DevBuf<double> xtot, ytot, xbase, ybase;
-    DevBuf<BatchDesc> d_desc;
-    DevBuf<int> grp, gcp, wcnt;
-    DevBuf<int4> work_r, work_c;
-    DevBuf<LpIter> iter;
-    int lanes_a = 8, lanes_at = 8;  // lanes per row of the grid-wide iterate kernels
-    int iter_grid = 2048;           // workgroups of one iterate launch: 8 per CU, a multiple of 8
-    int chunks_per_graph = 4;
-    bool use_graph = true;
-    DevBuf<char> state;
-    DevBuf<int> counter;
-    PinnedBuf<int> h_counter;
-    double clock_khz = 100000.0;
-};
 
 namespace pm {
 namespace {
@@ -932,6 +906,11 @@ void batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
     std::vector<BatchDesc> out(b->count);
     b->d_desc.download(out.data(), b->count, s);
     PM_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < b->count; ++k) {
+        b->desc[k].eta = out[k].eta;  // measured once, reused by later runs
+        b->desc[k].warm_accepted = out[k].warm_accepted;
+        b->desc[k].start_kkt = out[k].start_kkt;
+    }
     if (stats) {
         stats->device_seconds = ms * 1e-3;
         if (stats->instance_seconds)
@@ -1079,6 +1058,16 @@ int pm_batch_spmv_variants(pm_batch* b, int32_t* variant_a, int32_t* variant_at)
         if (!b || !variant_a || !variant_at) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
         *variant_a = pm::lanes_variant(b->lanes_a);
         *variant_at = pm::lanes_variant(b->lanes_at);
+    });
+}
+
+int pm_batch_warm_start_verdicts(pm_batch* b, int32_t* accepted, double* start_rel_kkt) {
+    return pm::guarded([&] {
+        if (!b) pm::fail(PM_ERR_INVALID_ARG, "batch is NULL");
+        for (int k = 0; k < b->count; ++k) {
+            if (accepted) accepted[k] = b->desc[k].warm_accepted;
+            if (start_rel_kkt) start_rel_kkt[k] = b->desc[k].start_kkt;
+        }
     });
 }
 

@@ -1,0 +1,77 @@
+// batch_internal.hpp -- the batched engine's state (pm_batch) and per-LP
+// records, shared by batch.hip and the learned warm start (learned.hip).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "control.hpp"
+
+namespace pm {
+
+// Per-LP offsets into the concatenated arrays, options and results.
+struct BatchDesc {
+    int32_t m, n, nnz;
+    int32_t lanes_a, lanes_at;
+    int32_t warm_x, warm_y;
+    int32_t warm_accepted;  // -1 no warm start, 1 kept, 0 replaced by the cold start
+    double omega0;          // initial primal weight; <= 0: ||c_hat|| / ||b_hat||
+    double eta;             // 0.99 / ||A_hat||_2, 0 until the first run measured it
+    double start_kkt;       // FP64 relative KKT of the starting point that was used
+    int64_t rp_off, cp_off, nz_off, row_off, col_off;
+    // results
+    int32_t status, iterations, restarts, refinements;
+    double rel_kkt, rel_p, rel_d, rel_g, pobj, dobj, omega;
+    uint64_t t_start, t_end;
+};
+
+// What the grid-wide iterate kernels need of one LP for the next chunk,
+// written by the init and check phases.
+struct LpIter {
+    float tau, sig;
+    int32_t k0;      // in-round Halpern index of the chunk's first iteration
+    int32_t chunk;   // iterations in the chunk (0: LP finished)
+    int32_t use_g;   // dual shift of a refinement round present
+    int32_t row_off, col_off;
+    int32_t pad;
+};
+
+}  // namespace pm
+
+struct pm_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int count = 0;
+    bool exact = true;
+    double tol = 1e-6;
+    int max_iterations = 100000, check_frequency = 64, max_refinements = 8;
+    std::vector<pm::BatchDesc> desc;
+    std::vector<int> order;
+    std::vector<int> sense;
+    std::vector<double> offset;
+    std::vector<double> c_model;  // model-sense costs, concatenated
+    pm::DevBuf<double> c_model_dev;  // the same on the device (uploaded for the learned warm start)
+    int64_t rows_total = 0, cols_total = 0, nnz_total = 0;
+    pm::DevBuf<int> rp, ci, cp, ri, d_order;
+    pm::DevBuf<float> va, vat, oa32, oat32;
+    pm::DevBuf<double> oa64, oat64;
+    pm::DevBuf<double> cmin, lr, ur, lx, ux, dr, dc, wx, wy;
+    pm::DevBuf<float> c32, lr32, ur32, lx32, ux32, g32, x, xe, x0, xt, xl, y, y0, yt, yl;
+    pm::DevBuf<double> xtot, ytot, xbase, ybase;
+    pm::DevBuf<pm::BatchDesc> d_desc;
+    pm::DevBuf<int> grp, gcp, wcnt;
+    pm::DevBuf<int4> work_r, work_c;
+    pm::DevBuf<pm::LpIter> iter;
+    int lanes_a = 8, lanes_at = 8;  // lanes per row of the grid-wide iterate kernels
+    int iter_grid = 2048;           // workgroups of one iterate launch: 8 per CU, a multiple of 8
+    int chunks_per_graph = 4;
+    bool use_graph = true;
+    pm::DevBuf<char> state;
+    pm::DevBuf<int> counter;
+    pm::PinnedBuf<int> h_counter;
+    double clock_khz = 100000.0;
+};

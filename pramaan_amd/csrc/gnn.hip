@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gnn_internal.hpp"
 #include "pramaan_gnn.h"
 
 namespace pm {
@@ -309,20 +310,6 @@ int cdiv(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
 
 using namespace pm;
 
-struct pm_gnn {
-    int device = 0;
-    hipStream_t s = nullptr;
-    int step = 0;
-    int m = 0, n = 0, nnz = 0;
-    DevBuf<float> p, grad, m1, m2;
-    DevBuf<short> wt, wn;  // 8 dense weights x 4096 bf16 each, two layouts
-    DevBuf<int> rp, ci, cp, ri;
-    DevBuf<float> ar, ac, ar_t, ac_t, xr, xc;
-    // activations: Hc0..2, Hr0..2, Zr1..2, Zc1..2, and gradients
-    DevBuf<float> Hc[3], Hr[3], Zr[3], Zc[3], dHc, dHr, dHc2, dZ, dT, xhat, yhat, dx, dy, tx, ty, part;
-    std::vector<float> host_tmp;
-};
-
 namespace pm {
 namespace {
 
@@ -538,6 +525,39 @@ float backward(pm_gnn* g) {
 }  // namespace
 }  // namespace pm
 
+namespace pm {
+
+void gnn_reserve(pm_gnn* g, int m, int n, int nnz, bool training) {
+    g->m = m;
+    g->n = n;
+    g->nnz = nnz;
+    const size_t mr = std::max(m, 1), nc = std::max(n, 1);
+    for (int l = 0; l < 3; ++l) {
+        g->Hc[l].resize(nc * kH);
+        g->Hr[l].resize(mr * kH);
+        g->Zr[l].resize(mr * kH);
+        g->Zc[l].resize(nc * kH);
+    }
+    g->xhat.resize(nc);
+    g->yhat.resize(mr);
+    g->grad_ready = training;
+    if (!training) return;
+    g->dHc.resize(nc * kH);
+    g->dHc2.resize(nc * kH);
+    g->dHr.resize(mr * kH);
+    g->dZ.resize(std::max(mr, nc) * kH);
+    g->dT.resize(mr * kH);
+    for (auto* b : {&g->dx, &g->tx}) b->resize(nc);
+    for (auto* b : {&g->dy, &g->ty}) b->resize(mr);
+    const size_t nb = static_cast<size_t>(cdiv(std::max(m, n), kRowsPer));
+    g->part.resize(std::max(nb * kH * kH, static_cast<size_t>(cdiv(std::max(m, 1), 256) + cdiv(std::max(n, 1), 256))) +
+                   nb * (kF + 1) * kH + 64);
+}
+
+void gnn_forward(pm_gnn* g) { forward(g); }
+
+}  // namespace pm
+
 extern "C" {
 
 int32_t pm_gnn_num_params(void) { return pm::kOff.total; }
@@ -671,23 +691,7 @@ int pm_gnn_set_graph(pm_gnn* g, int32_t m, int32_t n, int32_t nnz, const int32_t
         g->ac_t.upload(ac_t, s);
         g->xr.upload(xr, static_cast<size_t>(m) * kF, s);
         g->xc.upload(xc, static_cast<size_t>(n) * kF, s);
-        const size_t mr = std::max(m, 1), nc = std::max(n, 1);
-        for (int l = 0; l < 3; ++l) {
-            g->Hc[l].resize(nc * kH);
-            g->Hr[l].resize(mr * kH);
-            g->Zr[l].resize(mr * kH);
-            g->Zc[l].resize(nc * kH);
-        }
-        g->dHc.resize(nc * kH);
-        g->dHc2.resize(nc * kH);
-        g->dHr.resize(mr * kH);
-        g->dZ.resize(std::max(mr, nc) * kH);
-        g->dT.resize(mr * kH);
-        for (auto* b : {&g->xhat, &g->dx, &g->tx}) b->resize(nc);
-        for (auto* b : {&g->yhat, &g->dy, &g->ty}) b->resize(mr);
-        const size_t nb = static_cast<size_t>(cdiv(std::max(m, n), kRowsPer));
-        g->part.resize(std::max(nb * kH * kH, static_cast<size_t>(cdiv(std::max(m, 1), 256) + cdiv(std::max(n, 1), 256))) +
-                       nb * (kF + 1) * kH + 64);
+        pm::gnn_reserve(g, m, n, nnz, true);
         PM_HIP(hipStreamSynchronize(s));
     });
 }
@@ -706,6 +710,7 @@ int pm_gnn_forward(pm_gnn* g, float* xhat, float* yhat) {
 int pm_gnn_gradient(pm_gnn* g, const float* tx, const float* ty, float* loss, float* grad) {
     return pm::guarded([&] {
         if (!g || !tx || !ty) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
+        if (!g->grad_ready) pm::fail(PM_ERR_INVALID_ARG, "no training graph: call pm_gnn_set_graph first");
         PM_HIP(hipSetDevice(g->device));
         if (g->n) g->tx.upload(tx, g->n, g->s);
         if (g->m) g->ty.upload(ty, g->m, g->s);
@@ -721,6 +726,7 @@ int pm_gnn_train_step(pm_gnn* g, const float* tx, const float* ty, float lr, flo
     return pm::guarded([&] {
         using namespace pm;
         if (!g || !tx || !ty) fail(PM_ERR_INVALID_ARG, "NULL argument");
+        if (!g->grad_ready) pm::fail(PM_ERR_INVALID_ARG, "no training graph: call pm_gnn_set_graph first");
         PM_HIP(hipSetDevice(g->device));
         if (g->n) g->tx.upload(tx, g->n, g->s);
         if (g->m) g->ty.upload(ty, g->m, g->s);

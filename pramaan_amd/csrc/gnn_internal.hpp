@@ -1,0 +1,31 @@
+// gnn_internal.hpp -- the warm-start network's state (pm_gnn), shared by
+// gnn.hip and the learned warm start of a resident batch (learned.hip).
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+#include "pramaan_gnn.h"
+
+struct pm_gnn {
+    int device = 0;
+    hipStream_t s = nullptr;
+    int step = 0;
+    int m = 0, n = 0, nnz = 0;
+    bool grad_ready = false;  // graph set by pm_gnn_set_graph (backward buffers and A^T values present)
+    pm::DevBuf<float> p, grad, m1, m2;
+    pm::DevBuf<short> wt, wn;  // 8 dense weights x 4096 bf16 each, two layouts
+    pm::DevBuf<int> rp, ci, cp, ri;
+    pm::DevBuf<float> ar, ac, ar_t, ac_t, xr, xc;
+    // activations: Hc0..2, Hr0..2, Zr1..2, Zc1..2, and gradients
+    pm::DevBuf<float> Hc[3], Hr[3], Zr[3], Zc[3], dHc, dHr, dHc2, dZ, dT, xhat, yhat, dx, dy, tx, ty, part;
+    std::vector<float> host_tmp;
+};
+
+namespace pm {
+// Device buffers for a graph of m rows, n columns and nnz edges (the
+// gradient buffers too when training); the graph arrays are not touched.
+void gnn_reserve(pm_gnn* g, int m, int n, int nnz, bool training);
+// Forward pass over the graph in g->rp ... g->xc into g->xhat, g->yhat.
+void gnn_forward(pm_gnn* g);
+}  // namespace pm

@@ -1,79 +1,114 @@
 """Rolling-horizon solving on the GPU: cold start, previous-window warm start
 and learned warm start (pm_gnn_* network) for the windows of a
-multi-period blending plan (pm_rh_*).
+multi-period blending plan (pm_rh_*), and the training of that network on
+solutions the batch engine produces.
 
 * cold:      every window in one batched, graph-captured launch from zero.
 * previous:  windows one after another on the single-LP engine, window w
              starting from window w-1's solution shifted by one period
-             (columns/rows of periods 1..23 move to 0..22, the last period
-             repeats) -- inherently sequential.
+             (columns / rows of periods 1..23 move to 0..22, the last
+             period repeats) -- inherently sequential.
 * learned:   the network predicts x, y for every window at once, then all
              windows are solved in one batched launch from those points.
+             Every prediction is judged by its FP64 relative KKT error
+             (the engine's own check): a window whose prediction is worse
+             than the cold start's zero point falls back to cold.
+
+Training data are (window, x, y) from perturbed plans (demand and blend
+coefficients redrawn under new seeds) solved cold by the batch engine; the
+windows are regenerated from their plan when a mini-batch is built, so only
+the targets are stored.
+
+    python -m pramaan_amd.rolling --plans 40 --epochs 8 --out weights.npz
 """
 from __future__ import annotations
 
+import argparse
+import json
 import time
-from dataclasses import dataclass
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import _capi as P
 from . import warmstart as W
 
+TOL = 1e-6
+
 
 def shift(v: np.ndarray, per: int) -> np.ndarray:
+    """A window's vector moved one period earlier, last period repeated."""
     return np.concatenate([v[per:], v[-per:]])
 
 
 @dataclass
 class RunStats:
-    seconds: float            # device-side time of the solves (and inference)
+    seconds: float            # device time of the solves (plus inference for "learned")
     iterations: np.ndarray
     statuses: list
     objectives: np.ndarray
     rel_kkt: np.ndarray
     instance_seconds: np.ndarray | None = None
     inference_seconds: float = 0.0
+    fallbacks: int = 0
+
+    def summary(self) -> dict:
+        return dict(seconds=self.seconds, iterations_mean=float(self.iterations.mean()),
+                    iterations_p50=float(np.median(self.iterations)), iterations_max=int(self.iterations.max()),
+                    optimal=int(sum(s == "OPTIMAL" for s in self.statuses)), windows=len(self.statuses),
+                    inference_seconds=self.inference_seconds, fallbacks=self.fallbacks)
 
 
-def _stats(res, seconds, inst=None, infer=0.0):
+def _stats(res, seconds, inst=None, infer=0.0, fallbacks=0):
     return RunStats(seconds, np.array([r.iterations for r in res]), [r.status for r in res],
-                    np.array([r.objective for r in res]), np.array([r.rel_kkt for r in res]), inst, infer)
+                    np.array([r.objective for r in res]), np.array([r.rel_kkt for r in res]), inst, infer, fallbacks)
 
 
-def solve_cold(lps, tol, max_iterations=200000, want_vectors=False):
-    with P.Batch(lps, tolerance=tol, max_iterations=max_iterations, check_frequency=64) as b:
+def solve_cold(lps, tol=TOL, max_iterations=200000, want_vectors=False, batch=None):
+    own = batch is None
+    b = batch or P.Batch(lps, tolerance=tol, max_iterations=max_iterations, check_frequency=64)
+    try:
+        for i in range(len(lps)):
+            b.set_warm_start(i, None, None)
         res, secs, inst = b.run(want_vectors=want_vectors)
+    finally:
+        if own:
+            b.close()
     return _stats(res, secs, inst), res
 
 
-def solve_previous(lps, tol, cols_pp, rows_pp, max_iterations=200000):
+def solve_previous(lps, cols_pp, rows_pp, tol=TOL, max_iterations=200000):
+    """Sequential previous-window warm start on the single-LP engine."""
     ctx = P.Context(0)
     res, secs = [], 0.0
     x0 = y0 = None
-    for lp in lps:
-        kw = dict(algorithm="pdlp", termination="relative_kkt", tolerance=tol, max_iterations=max_iterations,
-                  check_frequency=64)
-        if x0 is not None:
-            kw.update(x0=shift(x0, cols_pp), y0=shift(y0, rows_pp))
-        ctx.load(lp, **kw)
-        t0 = time.perf_counter()
-        r = ctx.solve(want_vectors=True)
-        secs += time.perf_counter() - t0
-        res.append(r)
-        x0, y0 = r.x, r.y
-    ctx.close()
+    try:
+        for lp in lps:
+            kw = dict(algorithm="pdlp", termination="relative_kkt", tolerance=tol, max_iterations=max_iterations,
+                      check_frequency=64)
+            if x0 is not None:
+                kw.update(x0=shift(x0, cols_pp), y0=shift(y0, rows_pp))
+            ctx.load(lp, **kw)
+            t0 = time.perf_counter()
+            r = ctx.solve(want_vectors=True)
+            secs += time.perf_counter() - t0
+            res.append(r)
+            x0, y0 = r.x, r.y
+    finally:
+        ctx.close()
     return _stats(res, secs), res
 
 
-def predict(net: "P.Gnn", lps, graphs, chunk=32):
-    """Warm starts for every window; returns (list of (x0, y0), device seconds)."""
+def predict(net: P.Gnn, lps, graphs, chunk=32):
+    """Warm starts for every window: list of (x0, y0), and the seconds spent
+    in the network (graph upload and forward)."""
     out, secs = [], 0.0
     for s in range(0, len(lps), chunk):
         gs = graphs[s:s + chunk]
         bg = W.batch_graphs(gs)
-        net.set_graph(bg)
         t0 = time.perf_counter()
+        net.set_graph(bg)
         xh, yh = net.forward()
         secs += time.perf_counter() - t0
         xo = yo = 0
@@ -84,54 +119,198 @@ def predict(net: "P.Gnn", lps, graphs, chunk=32):
     return out, secs
 
 
-def solve_learned(lps, graphs, net, tol, max_iterations=200000):
+def solve_learned(lps, graphs, net, tol=TOL, max_iterations=200000, batch=None):
+    """Learned warm start for every window, one batched solve.  The engine
+    judges each prediction by its FP64 relative KKT error against the cold
+    start's and starts from the better one (pm_batch_run)."""
     starts, infer = predict(net, lps, graphs)
-    with P.Batch(lps, tolerance=tol, max_iterations=max_iterations, check_frequency=64) as b:
+    own = batch is None
+    b = batch or P.Batch(lps, tolerance=tol, max_iterations=max_iterations, check_frequency=64)
+    try:
         for i, (x0, y0) in enumerate(starts):
             b.set_warm_start(i, x0, y0)
         res, secs, inst = b.run()
-    return _stats(res, secs + infer, inst, infer), res
+        acc, _ = b.warm_start_verdicts()
+        for i in range(len(lps)):
+            b.set_warm_start(i, None, None)
+    finally:
+        if own:
+            b.close()
+    return _stats(res, secs + infer, inst, infer, int((acc == 0).sum())), res
 
 
-def training_data(seeds, windows_per_seed, tol=1e-5, demand_noise=0.2, blend_noise=0.1, log=print):
-    """(graph, tx, ty) tuples from perturbed plans solved by the batch engine."""
-    data = []
+# ---------------------------------------------------------------------------
+# training data
+# ---------------------------------------------------------------------------
+
+@dataclass
+class TrainingSet:
+    plans: list = field(default_factory=list)        # RollingHorizon objects
+    index: list = field(default_factory=list)        # (plan, window)
+    tx: list = field(default_factory=list)           # x / s, float32
+    ty: list = field(default_factory=list)           # y / cs, float32
+    omega: list = field(default_factory=list)
+
+    def __len__(self):
+        return len(self.index)
+
+
+def training_set(seeds, windows_per_plan=500, tol=1e-5, demand_noise=0.2, blend_noise=0.1, log=print):
+    """Solves the windows of perturbed plans (seed 1000 + s, perturbation
+    seed s) cold in batches and keeps the optimal ones as targets."""
+    ts = TrainingSet()
     for sd in seeds:
-        rh = P.RollingHorizon(seed=1000 + sd, demand_noise=demand_noise, blend_noise=blend_noise,
-                              perturb_seed=sd)
-        lps = [rh.window(w) for w in range(min(windows_per_seed, rh.windows))]
+        rh = P.RollingHorizon(seed=1000 + sd, demand_noise=demand_noise, blend_noise=blend_noise, perturb_seed=sd)
+        pi = len(ts.plans)
+        ts.plans.append(rh)
+        nw = min(windows_per_plan, rh.windows)
+        lps = [rh.window(w) for w in range(nw)]
         st, res = solve_cold(lps, tol, want_vectors=True)
         ok = 0
-        for lp, r in zip(lps, res):
+        for w, (lp, r) in enumerate(zip(lps, res)):
             if r.status != "OPTIMAL":
                 continue
-            g = W.graph_of(lp, rh.cols_per_period, rh.rows_per_period)
-            tx, ty = W.targets(lp, g, r.x, r.y)
-            data.append((g, tx.astype(np.float32), ty.astype(np.float32)))
+            s, cs = _scales(lp)
+            ts.index.append((pi, w))
+            ts.tx.append((r.x / s).astype(np.float32))
+            ts.ty.append((r.y / cs).astype(np.float32))
+            ts.omega.append(r.primal_weight)
             ok += 1
-        log(f"seed {sd}: {ok}/{len(lps)} windows solved in {st.seconds:.2f}s (iters mean {st.iterations.mean():.0f})")
-    return data
+        log(f"plan {sd}: {ok}/{nw} windows solved in {st.seconds:.2f}s (iterations mean {st.iterations.mean():.0f})")
+    return ts
 
 
-def train(net, data, epochs=2, batch=8, lr=2e-3, log=print, seed=0):
-    """Adam over mini-batches of windows; returns (samples/s, final loss)."""
+def _scales(lp):
+    b = np.concatenate([lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper])
+    b = np.abs(b[np.abs(b) < 1e29])
+    s = max(1.0, float(b.max()) if len(b) else 1.0)
+    cs = max(1e-12, float(np.abs(lp.c).max())) if lp.n else 1.0
+    return s, cs
+
+
+def _sample_graph(ts: TrainingSet, k: int):
+    pi, w = ts.index[k]
+    rh = ts.plans[pi]
+    lp = rh.window(w)
+    return W.graph_of(lp, rh.cols_per_period, rh.rows_per_period)
+
+
+def _minibatch(ts, idx, pool):
+    gs = list(pool.map(lambda k: _sample_graph(ts, k), idx))
+    return W.batch_graphs(gs), np.concatenate([ts.tx[k] for k in idx]), np.concatenate([ts.ty[k] for k in idx])
+
+
+def evaluate_loss(net, ts, idx, batch, pool):
+    tot, cnt, base = 0.0, 0, 0.0
+    for s in range(0, len(idx), batch):
+        sel = idx[s:s + batch]
+        bg, tx, ty = _minibatch(ts, sel, pool)
+        net.set_graph(bg)
+        loss, _ = net.gradient(tx, ty)
+        tot += loss * len(sel)
+        base += (float(np.mean(tx.astype(np.float64) ** 2)) + float(np.mean(ty.astype(np.float64) ** 2))) * len(sel)
+        cnt += len(sel)
+    return tot / max(cnt, 1), base / max(cnt, 1)
+
+
+def train(net, ts, epochs=8, batch=16, lr=2e-3, val_frac=0.05, log=print, seed=0, threads=8):
+    """Adam over mini-batches of windows with a cosine learning rate;
+    returns a dict with samples/s and train / validation / zero-predictor
+    losses."""
     rng = np.random.default_rng(seed)
-    n = len(data)
-    t0 = time.perf_counter()
+    order = rng.permutation(len(ts))
+    nval = max(1, int(len(ts) * val_frac))
+    val, tr = order[:nval], order[nval:]
+    steps_total = epochs * ((len(tr) + batch - 1) // batch)
+    step = 0
     seen = 0
-    loss = float("nan")
-    for ep in range(epochs):
-        order = rng.permutation(n)
-        losses = []
-        for s in range(0, n, batch):
-            idx = order[s:s + batch]
-            bg = W.batch_graphs([data[i][0] for i in idx])
-            tx = np.concatenate([data[i][1] for i in idx])
-            ty = np.concatenate([data[i][2] for i in idx])
-            net.set_graph(bg)
-            step_lr = lr * (0.5 * (1 + np.cos(np.pi * (ep * n + s) / (epochs * n))))
-            losses.append(net.train_step(tx, ty, lr=float(step_lr)))
-            seen += len(idx)
-        loss = float(np.mean(losses[-max(1, len(losses) // 10):]))
-        log(f"epoch {ep}: loss {loss:.4e} ({seen / (time.perf_counter() - t0):.1f} samples/s)")
-    return seen / (time.perf_counter() - t0), loss
+    t_train = 0.0
+    hist = []
+    # One thread assembles the next mini-batch (its windows built by the
+    # pool) while the GPU trains on the current one.
+    with ThreadPoolExecutor(threads) as pool, ThreadPoolExecutor(1) as ahead:
+        for ep in range(epochs):
+            perm = rng.permutation(tr)
+            losses = []
+            fut = ahead.submit(_minibatch, ts, perm[:batch], pool) if len(perm) else None
+            for s in range(0, len(perm), batch):
+                bg, tx, ty = fut.result()
+                nxt = perm[s + batch:s + 2 * batch]
+                fut = ahead.submit(_minibatch, ts, nxt, pool) if len(nxt) else None
+                step_lr = lr * 0.5 * (1.0 + np.cos(np.pi * step / max(1, steps_total)))
+                t0 = time.perf_counter()
+                net.set_graph(bg)
+                losses.append(net.train_step(tx, ty, lr=float(step_lr)))
+                t_train += time.perf_counter() - t0
+                seen += len(perm[s:s + batch])
+                step += 1
+            vloss, vbase = evaluate_loss(net, ts, val, batch, pool)
+            rec = dict(epoch=ep, train_loss=float(np.mean(losses[-max(1, len(losses) // 5):])), val_loss=vloss,
+                       zero_predictor_loss=vbase, samples_per_s=seen / max(t_train, 1e-9))
+            hist.append(rec)
+            log(json.dumps(rec))
+    return dict(samples_per_s=seen / max(t_train, 1e-9), history=hist)
+
+
+def save(net: P.Gnn, path, meta: dict | None = None) -> None:
+    np.savez(path, params=net.get_params(), meta=json.dumps(meta or {}))
+
+
+def load(path, device=0) -> P.Gnn:
+    with np.load(path, allow_pickle=False) as z:
+        net = P.Gnn(device=device)
+        net.set_params(z["params"])
+    return net
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--plans", type=int, default=40, help="perturbed training plans (500 windows each)")
+    ap.add_argument("--windows", type=int, default=500)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--out", default=None, help="write the trained parameters (.npz)")
+    ap.add_argument("--no-eval", action="store_true")
+    ap.add_argument("--previous", type=int, default=0, help="windows for the sequential previous-window run")
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    t0 = time.perf_counter()
+    ts = training_set(range(1, a.plans + 1), a.windows, log=log)
+    log(f"training set: {len(ts)} windows in {time.perf_counter() - t0:.1f}s")
+    net = P.Gnn(seed=a.seed)
+    tr = train(net, ts, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, seed=a.seed)
+    if a.out:
+        save(net, a.out, dict(plans=a.plans, windows=a.windows, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                              seed=a.seed, samples=len(ts)))
+    if a.no_eval:
+        return 0
+    out = dict(training=dict(samples=len(ts), samples_per_s=tr["samples_per_s"], history=tr["history"]))
+    out.update(evaluate(net, previous=a.previous, log=log))
+    log(json.dumps(out))
+    return 0
+
+
+def evaluate(net, nwin=500, tol=TOL, previous=0, log=print) -> dict:
+    """Cold vs learned (and optionally sequential previous-window) on the
+    unperturbed 500-window sequence."""
+    rh = P.RollingHorizon()
+    lps = [rh.window(w) for w in range(min(nwin, rh.windows))]
+    graphs = [W.graph_of(lp, rh.cols_per_period, rh.rows_per_period) for lp in lps]
+    out = {}
+    with P.Batch(lps, tolerance=tol, max_iterations=200000, check_frequency=64) as b:
+        cold, _ = solve_cold(lps, tol, batch=b)
+        learned, _ = solve_learned(lps, graphs, net, tol, batch=b)
+    out["cold"] = cold.summary()
+    out["learned"] = learned.summary()
+    out["speedup_learned_vs_cold"] = cold.seconds / learned.seconds
+    if previous:
+        prev, _ = solve_previous(lps[:previous], rh.cols_per_period, rh.rows_per_period, tol)
+        out["previous"] = prev.summary()
+    log(json.dumps(out))
+    return out
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

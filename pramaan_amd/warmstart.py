@@ -21,41 +21,24 @@ from ._capi import INF, LP
 FEAT = 8
 
 
-def scales(lp: LP) -> tuple[float, float]:
-    b = np.concatenate([lp.row_lower, lp.row_upper, lp.var_lower, lp.var_upper])
-    b = np.abs(b[np.abs(b) < 1e29])
-    s = max(1.0, float(b.max()) if len(b) else 1.0)
-    cs = max(1e-12, float(np.abs(lp.c).max()) if lp.n else 1.0)
-    return s, cs
-
-
 def graph_of(lp: LP, period_cols: int | None = None, period_rows: int | None = None) -> dict:
-    """Graph arrays of one LP (see the module docstring)."""
-    m, n = lp.m, lp.n
-    s, cs = scales(lp)
-    rp, ci, v = lp.row_ptr.astype(np.int64), lp.col_idx.astype(np.int64), lp.values
-    rowlen = np.diff(rp)
-    row_of = np.repeat(np.arange(m), rowlen)
-    absv = np.abs(v)
-    rmax = np.zeros(m)
-    np.maximum.at(rmax, row_of, absv)
-    cmax = np.zeros(n)
-    np.maximum.at(cmax, ci, absv)
-    ar = v / np.where(rmax[row_of] > 0, rmax[row_of], 1.0)
-    order = np.lexsort((row_of, ci))  # column-major, rows ascending
-    cp = np.concatenate([[0], np.cumsum(np.bincount(ci, minlength=n))])
-    ri = row_of[order]
-    ac = v[order] / np.where(cmax[ci[order]] > 0, cmax[ci[order]], 1.0)
-    collen = np.diff(cp)
-    fin = lambda a: np.where(np.abs(a) < 1e29, a, 0.0)
-    cpos = (np.arange(n) // period_cols) / max(1, (n // period_cols) - 1) if period_cols else np.zeros(n)
-    rpos = (np.arange(m) // period_rows) / max(1, (m // period_rows) - 1) if period_rows else np.zeros(m)
-    xc = np.stack([lp.c / cs, fin(lp.var_lower) / s, fin(lp.var_upper) / s, (lp.var_lower > -1e29) * 1.0,
-                   (lp.var_upper < 1e29) * 1.0, collen / max(1, collen.max()), cpos, np.ones(n)], axis=1)
-    xr = np.stack([fin(lp.row_lower) / s, fin(lp.row_upper) / s, (lp.row_lower > -1e29) * 1.0,
-                   (lp.row_upper < 1e29) * 1.0, (lp.row_lower == lp.row_upper) * 1.0,
-                   rowlen / max(1, rowlen.max()), rpos, np.ones(m)], axis=1)
-    return dict(m=m, n=n, rp=rp, ci=ci, ar=ar, cp=cp, ri=ri, ac=ac, xr=xr, xc=xc, s=s, cs=cs)
+    """Graph arrays of one LP (see the module docstring), built by
+    pm_gnn_lp_features."""
+    from . import _capi as P
+    m, n, nnz = lp.m, lp.n, lp.nnz
+    cp = np.zeros(n + 1, np.int32)
+    ri = np.zeros(max(nnz, 1), np.int32)
+    ar = np.zeros(max(nnz, 1), np.float32)
+    ac = np.zeros(max(nnz, 1), np.float32)
+    xr = np.zeros((m, FEAT), np.float32)
+    xc = np.zeros((n, FEAT), np.float32)
+    sc = np.zeros(2)
+    s = lp.struct()
+    P._check(P.lib().pm_gnn_lp_features(P.C.byref(s), period_cols or 0, period_rows or 0, P._p(cp, P.C.c_int32),
+                                        P._p(ri, P.C.c_int32), P._p(ar, P.C.c_float), P._p(ac, P.C.c_float),
+                                        P._p(xr, P.C.c_float), P._p(xc, P.C.c_float), P._p(sc, P.C.c_double)))
+    return dict(m=m, n=n, rp=lp.row_ptr, ci=lp.col_idx, ar=ar[:nnz], cp=cp, ri=ri[:nnz], ac=ac[:nnz], xr=xr, xc=xc,
+                s=float(sc[0]), cs=float(sc[1]))
 
 
 def batch_graphs(graphs: list[dict]) -> dict:

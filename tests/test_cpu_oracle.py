@@ -30,11 +30,11 @@ def test_library_exports_every_declared_symbol(capi):
     assert not missing, missing
     assert set(capi.SIGNATURES) >= syms
     # The certificate entry points live in the C++ interface library.
-    cert = header_symbols(("pramaan_cert.h",))
+    cert = header_symbols(("pramaan_cert.h", "pramaan_ladder.h"))
     cxx = ctypes.CDLL(str(capi.CXX_LIB_PATH))
     assert cert and not [s for s in sorted(cert) if not hasattr(cxx, s)]
     assert {p.name for p in (ROOT / "include").glob("*.h")} == {
-        "pramaan_hip.h", "pramaan_gen.h", "pramaan_gnn.h", "pramaan_cert.h"}
+        "pramaan_hip.h", "pramaan_gen.h", "pramaan_gnn.h", "pramaan_cert.h", "pramaan_ladder.h"}
 
 
 def test_no_device_is_reported_not_faked(capi):
