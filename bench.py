@@ -54,6 +54,19 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of an iterate kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+    path = ROOT / "profiles" / "batch_iterate_rolling500_traffic.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def oracle_seconds_per_iteration(lp, budget_s: float):
     """Restated reference PDHG (oracle, FP32, 1 thread): seconds per
     iteration on lp, setup subtracted, over a bounded run."""
@@ -96,6 +109,8 @@ def bench_rolling(args, P) -> dict:
     lps = [rh.window(w) for w in range(nw)]
     net = R.load(WEIGHTS)
     b = P.Batch(lps, tolerance=args.tol, max_iterations=args.max_iters, check_frequency=64)
+    if args.no_graph:
+        b.set_graph(False)
     log(f"{nw} windows resident in {time.perf_counter() - t0:.1f}s; spmv lanes (A, A^T) {b.spmv_variants()}")
     cpp, rpp = rh.cols_per_period, rh.rows_per_period
 
@@ -165,8 +180,10 @@ def bench_rolling(args, P) -> dict:
         "warm_starts_kept": int((acc == 1).sum()), "warm_starts_replaced_by_cold": int((acc == 0).sum()),
         "objective_rel_error_vs_highs": obj_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "kernel_us": ms * 1e3,
-                     "alg_bytes_per_launch": by},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
+                     "kernel_us": ms * 1e3, "alg_bytes_per_launch": by,
+                     "traffic_source": "profiles/batch_iterate_rolling500_traffic.md (rocprofv3 FETCH_SIZE + "
+                                       "WRITE_SIZE, calibrated on a 256 MiB copy)"},
         "batch_kernels": {k: {"us": v[0] * 1e3, "alg_bytes": v[1], "GBps": v[1] / v[0] / 1e6}
                           for k, v in kern.items()},
     }
@@ -235,6 +252,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lp1", action="store_true", help="skip the 500k x 1M SpMV measurement")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="issue the batch launches directly instead of replaying the HIP graph (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs every LP of the workload on a single device and has no "

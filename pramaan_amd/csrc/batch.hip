@@ -51,9 +51,10 @@ constexpr int kBB = 1024;
 // Largest lane count the iterate kernels are built for.
 constexpr int kMaxLanes = 32;
 
-// Lanes of an explicit SpMV variant (0 for PM_SPMV_AUTO and the variants
-// the iterate kernels do not offer, which are autotuned instead).
-inline int variant_lanes_or_zero(int v) {
+// Kernel of an explicit SpMV variant: lanes per row, 0 for the stream
+// kernel, -1 for PM_SPMV_AUTO and the variants the iterate kernels do not
+// offer (autotuned instead).
+inline int variant_kernel(int v) {
     switch (v) {
         case PM_SPMV_SCALAR: return 1;
         case PM_SPMV_VECTOR2: return 2;
@@ -61,18 +62,33 @@ inline int variant_lanes_or_zero(int v) {
         case PM_SPMV_VECTOR8: return 8;
         case PM_SPMV_VECTOR16: return 16;
         case PM_SPMV_VECTOR32: return 32;
-        default: return 0;
+        case PM_SPMV_STREAM: return 0;
+        default: return -1;
     }
 }
 
 inline int lanes_variant(int l) {
     switch (l) {
+        case 0: return PM_SPMV_STREAM;
         case 1: return PM_SPMV_SCALAR;
         case 2: return PM_SPMV_VECTOR2;
         case 4: return PM_SPMV_VECTOR4;
         case 16: return PM_SPMV_VECTOR16;
         case 32: return PM_SPMV_VECTOR32;
         default: return PM_SPMV_VECTOR8;
+    }
+}
+
+// Stream segments of one CSR: consecutive rows with at most kStreamNnz
+// nonzeros and kBlock rows together; a longer row forms a segment alone.
+inline void stream_segments(int nrows, const int* rp, std::vector<int2>& seg) {
+    seg.clear();
+    int first = 0;
+    while (first < nrows) {
+        int last = first + 1;
+        while (last < nrows && last - first < kBlock && rp[last + 1] - rp[first] <= kStreamNnz) ++last;
+        seg.push_back(make_int2(first, last - first));
+        first = last;
     }
 }
 
@@ -403,13 +419,15 @@ __global__ __launch_bounds__(kBB) void k_b_round(BatchArrays<VT> A, BatchState* 
     if (tid == 0) st.round_start = 0;
 }
 
-// Work list of the iterate kernels, rebuilt after every check: one item
-// {lp, first global row, rows} per group of G rows of every LP still
-// running (G = 256 / lanes), in LP order; counts in wcnt[0] (rows of A) and
+// Work list of the iterate kernels, rebuilt after every check: one
+// WorkItem per group of G rows of every LP still running (G = 256 / lanes;
+// G = 0 selects the LP's stream segments instead), in LP order, carrying
+// that LP's parameters for the chunk; counts in wcnt[0] (rows of A) and
 // wcnt[1] (rows of A^T).  One 1024-thread workgroup scans the LPs.
 template <class VT>
 __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int count, int ga, int gt,
-                                                     int4* __restrict__ wr, int4* __restrict__ wc,
+                                                     const int2* __restrict__ seg_a, const int2* __restrict__ seg_at,
+                                                     WorkItem* __restrict__ wr, WorkItem* __restrict__ wc,
                                                      int* __restrict__ wcnt) {
     __shared__ int sr[1024], sc[1024];
     __shared__ int carry_r, carry_c;
@@ -420,14 +438,16 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
         const int k = base + tid;
         int nr = 0, nc = 0, m = 0, n = 0;
         int64_t ro = 0, co = 0;
-        if (k < count && A.it[k].chunk > 0) {
+        LpIter P{};
+        if (k < count) P = A.it[k];
+        if (k < count && P.chunk > 0) {
             const BatchDesc& d = A.desc[k];
             m = d.m;
             n = d.n;
             ro = d.row_off;
             co = d.col_off;
-            nr = (m + ga - 1) / ga;
-            nc = (n + gt - 1) / gt;
+            nr = ga > 0 ? (m + ga - 1) / ga : d.sa_cnt;
+            nc = gt > 0 ? (n + gt - 1) / gt : d.st_cnt;
         }
         sr[tid] = nr;
         sc[tid] = nc;
@@ -440,10 +460,24 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
             __syncthreads();
         }
         const int offr = carry_r + sr[tid] - nr, offc = carry_c + sc[tid] - nc;
-        for (int g = 0; g < nr; ++g)
-            wr[offr + g] = make_int4(k, static_cast<int>(ro) + g * ga, min(ga, m - g * ga), 0);
-        for (int g = 0; g < nc; ++g)
-            wc[offc + g] = make_int4(k, static_cast<int>(co) + g * gt, min(gt, n - g * gt), 0);
+        for (int g = 0; g < nr; ++g) {
+            int first = g * ga, rows = min(ga, m - g * ga);
+            if (ga == 0) {
+                const int2 sg = seg_a[A.desc[k].sa_off + g];
+                first = sg.x;
+                rows = sg.y;
+            }
+            wr[offr + g] = WorkItem{static_cast<int>(ro) + first, rows, P.k0, P.chunk, P.use_g, P.col_off, P.sig, k};
+        }
+        for (int g = 0; g < nc; ++g) {
+            int first = g * gt, rows = min(gt, n - g * gt);
+            if (gt == 0) {
+                const int2 sg = seg_at[A.desc[k].st_off + g];
+                first = sg.x;
+                rows = sg.y;
+            }
+            wc[offc + g] = WorkItem{static_cast<int>(co) + first, rows, P.k0, P.chunk, 0, P.row_off, P.tau, k};
+        }
         __syncthreads();
         if (tid == 1023) {
             carry_r += sr[1023];
@@ -469,42 +503,122 @@ __device__ __forceinline__ void xcd_slice(int total, int& w0, int& w1, int& step
     step = static_cast<int>(gridDim.x) / nx;
 }
 
+// The next item is read while the current one is computed, so the only
+// dependent loads left per item are its row pointers, matrix entries and
+// gathered vector entries.  Items are read as two int4 (wave-uniform, so
+// scalar loads) and kept in registers.
+struct ItemRegs {
+    int4 a, b;  // a = {row0, cnt, k0, chunk}, b = {use_g, off, step bits, lp}
+};
+
+__device__ __forceinline__ ItemRegs load_item(const WorkItem* work, int w) {
+    const int4* p = reinterpret_cast<const int4*>(work + w);
+    return ItemRegs{p[0], p[1]};
+}
+
 template <int L, class VT>
-__global__ __launch_bounds__(kBlock) void k_s_primal(BatchArrays<VT> A, const int4* __restrict__ work,
+__global__ __launch_bounds__(kBlock) void k_s_primal(BatchArrays<VT> A, const WorkItem* __restrict__ work,
                                                      const int* __restrict__ wcnt, int i) {
+    static_assert(sizeof(WorkItem) == 32, "WorkItem is read as two int4");
     const int lane = static_cast<int>(threadIdx.x) & (L - 1), g = static_cast<int>(threadIdx.x) / L;
     int w0, w1, step;
     xcd_slice(wcnt[1], w0, w1, step);
+    if (w0 >= w1) return;
+    ItemRegs cur = load_item(work, w0);
     for (int w = w0; w < w1; w += step) {
-        const int4 t = work[w];
-        const LpIter P = A.it[t.x];
-        if (g >= t.z || i >= P.chunk) continue;
-        const int j = t.y + g;
-        const float aty = row_dot<L, false, float>(A.gcp[j], A.gcp[j + 1], lane, A.ri, A.vat, A.y + P.row_off);
-        if (lane == 0) {
-            const bool keep = i == P.chunk - 1;
-            EpiPrimalHal{P.tau, Control::lambda(P.k0 + i), 1.0f, A.c32, A.lx32, A.ux32, A.x0, A.x, A.xe,
-                         keep ? A.xt : nullptr}(j, aty);
+        ItemRegs nxt = cur;
+        if (w + step < w1) nxt = load_item(work, w + step);
+        const int chunk = cur.a.w;
+        if (g < cur.a.y && i < chunk) {
+            const int j = cur.a.x + g;
+            const float aty = row_dot<L, false, float>(A.gcp[j], A.gcp[j + 1], lane, A.ri, A.vat, A.y + cur.b.y);
+            if (lane == 0) {
+                const bool keep = i == chunk - 1;
+                EpiPrimalHal{__int_as_float(cur.b.z), Control::lambda(cur.a.z + i), 1.0f, A.c32, A.lx32, A.ux32,
+                             A.x0, A.x, A.xe, keep ? A.xt : nullptr}(j, aty);
+            }
         }
+        cur = nxt;
     }
 }
 
 template <int L, class VT>
-__global__ __launch_bounds__(kBlock) void k_s_dual(BatchArrays<VT> A, const int4* __restrict__ work,
+__global__ __launch_bounds__(kBlock) void k_s_dual(BatchArrays<VT> A, const WorkItem* __restrict__ work,
                                                    const int* __restrict__ wcnt, int i) {
     const int lane = static_cast<int>(threadIdx.x) & (L - 1), g = static_cast<int>(threadIdx.x) / L;
     int w0, w1, step;
     xcd_slice(wcnt[0], w0, w1, step);
+    if (w0 >= w1) return;
+    ItemRegs cur = load_item(work, w0);
     for (int w = w0; w < w1; w += step) {
-        const int4 t = work[w];
-        const LpIter P = A.it[t.x];
-        if (g >= t.z || i >= P.chunk) continue;
-        const int r = t.y + g;
-        const float ax = row_dot<L, false, float>(A.grp[r], A.grp[r + 1], lane, A.ci, A.va, A.xe + P.col_off);
-        if (lane == 0) {
-            const bool keep = i == P.chunk - 1;
-            EpiDualHal{P.sig, Control::lambda(P.k0 + i), 1.0f, A.lr32, A.ur32, P.use_g ? A.g32 : nullptr, A.y0,
-                       A.y, keep ? A.yt : nullptr}(r, ax);
+        ItemRegs nxt = cur;
+        if (w + step < w1) nxt = load_item(work, w + step);
+        const int chunk = cur.a.w;
+        if (g < cur.a.y && i < chunk) {
+            const int r = cur.a.x + g;
+            const float ax = row_dot<L, false, float>(A.grp[r], A.grp[r + 1], lane, A.ci, A.va, A.xe + cur.b.y);
+            if (lane == 0) {
+                const bool keep = i == chunk - 1;
+                EpiDualHal{__int_as_float(cur.b.z), Control::lambda(cur.a.z + i), 1.0f, A.lr32, A.ur32,
+                           cur.b.x ? A.g32 : nullptr, A.y0, A.y, keep ? A.yt : nullptr}(r, ax);
+            }
+        }
+        cur = nxt;
+    }
+}
+
+// Stream variant: a work item is a segment of consecutive rows holding at
+// most kStreamNnz nonzeros.  The workgroup reads the segment's nonzeros
+// contiguously (every load coalesced), stages the products in LDS, and one
+// thread per row sums its products in CSR order; a longer row (alone in its
+// segment) is reduced by the whole workgroup.  The epilogues are the same
+// functors as the vector kernels'.
+template <bool PRIMAL, class VT>
+__global__ __launch_bounds__(kBlock) void k_t_half(BatchArrays<VT> A, const WorkItem* __restrict__ work,
+                                                   const int* __restrict__ wcnt, int i) {
+    __shared__ float prod[kStreamNnz];
+    __shared__ float red[kBlock / 64];
+    const int t = static_cast<int>(threadIdx.x);
+    const int* rp = PRIMAL ? A.gcp : A.grp;
+    const int* idx = PRIMAL ? A.ri : A.ci;
+    const float* val = PRIMAL ? A.vat : A.va;
+    int w0, w1, step;
+    xcd_slice(wcnt[PRIMAL ? 1 : 0], w0, w1, step);
+    for (int w = w0; w < w1; w += step) {
+        const ItemRegs it = load_item(work, w);
+        const int row0 = it.a.x, cnt = it.a.y, chunk = it.a.w;
+        if (i >= chunk) continue;  // whole item: uniform over the workgroup
+        const float* xv = (PRIMAL ? A.y : A.xe) + it.b.y;
+        const int nz0 = rp[row0], nz1 = rp[row0 + cnt];
+        float s = 0.0f;
+        if (nz1 - nz0 <= kStreamNnz) {
+            for (int k = nz0 + t; k < nz1; k += kBlock) prod[k - nz0] = val[k] * xv[idx[k]];
+            __syncthreads();
+            if (t < cnt) {
+                const int b = rp[row0 + t] - nz0, e = rp[row0 + t + 1] - nz0;
+                for (int k = b; k < e; ++k) s += prod[k];
+            }
+            __syncthreads();
+        } else {  // one long row
+            float part = 0.0f;
+            for (int k = nz0 + t; k < nz1; k += kBlock) part = fmaf(val[k], xv[idx[k]], part);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+            if ((t & 63) == 0) red[t >> 6] = part;
+            __syncthreads();
+            if (t == 0)
+                for (int q = 0; q < kBlock / 64; ++q) s += red[q];
+            __syncthreads();
+        }
+        if (t < cnt) {
+            const int r = row0 + t;
+            const bool keep = i == chunk - 1;
+            const float stp = __int_as_float(it.b.z), lam = Control::lambda(it.a.z + i);
+            if constexpr (PRIMAL)
+                EpiPrimalHal{stp, lam, 1.0f, A.c32, A.lx32, A.ux32, A.x0, A.x, A.xe, keep ? A.xt : nullptr}(r, s);
+            else
+                EpiDualHal{stp, lam, 1.0f, A.lr32, A.ur32, it.b.x ? A.g32 : nullptr, A.y0, A.y,
+                           keep ? A.yt : nullptr}(r, s);
         }
     }
 }
@@ -677,6 +791,7 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     else { h_oa64.resize(Z); h_oat64.resize(Z); }
     std::vector<double> h_cmin(Cn), h_lx(Cn), h_ux(Cn), h_dc(Cn), h_lr(R), h_ur(R), h_dr(R);
     std::vector<int> h_grp(R + 1, 0), h_gcp(Cn + 1, 0);
+    std::vector<std::vector<int2>> seg_a_lp(count), seg_at_lp(count);
     h_grp[ro] = static_cast<int>(nzo);
     h_gcp[co] = static_cast<int>(nzo);
     b->c_model.assign(Cn, 0.0);
@@ -727,6 +842,8 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         }
         d.lanes_a = m > 0 ? lanes_for(static_cast<double>(nnz) / m) : 1;
         d.lanes_at = lanes_for(static_cast<double>(nnz) / n);
+        stream_segments(m, L.row_ptr, seg_a_lp[k]);
+        stream_segments(n, cp.data(), seg_at_lp[k]);
     };
     {
         const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -737,6 +854,23 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
                 for (int k = next++; k < count; k = next++) prep(k);
             });
         for (auto& t : th) t.join();
+    }
+    // Stream segments, concatenated in LP order.
+    {
+        std::vector<int2> sa, st;
+        for (int k = 0; k < count; ++k) {
+            BatchDesc& d = b->desc[k];
+            d.sa_off = static_cast<int32_t>(sa.size());
+            d.sa_cnt = static_cast<int32_t>(seg_a_lp[k].size());
+            d.st_off = static_cast<int32_t>(st.size());
+            d.st_cnt = static_cast<int32_t>(seg_at_lp[k].size());
+            sa.insert(sa.end(), seg_a_lp[k].begin(), seg_a_lp[k].end());
+            st.insert(st.end(), seg_at_lp[k].begin(), seg_at_lp[k].end());
+        }
+        if (sa.empty()) sa.push_back(make_int2(0, 0));
+        if (st.empty()) st.push_back(make_int2(0, 0));
+        b->seg_a.upload(sa, b->stream);
+        b->seg_at.upload(st, b->stream);
     }
     // Largest LP first.
     b->order.resize(count);
@@ -768,18 +902,21 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     // length), four chunks per graph unless that makes the graph large.
     b->lanes_a = ro > 0 ? lanes_for(static_cast<double>(nzo) / static_cast<double>(ro)) : 1;
     b->lanes_at = lanes_for(static_cast<double>(nzo) / static_cast<double>(co));
-    // Work-list capacity for the largest lane count (the smallest groups).
+    // Work-list capacity: the largest lane count (the smallest groups) or
+    // the stream segments, whichever gives more items.
     constexpr int kMinGroup = kBlock / kMaxLanes;
-    size_t items_r = 0, items_c = 0;
+    size_t items_r = 0, items_c = 0, segs_r = 0, segs_c = 0;
     for (const BatchDesc& d : b->desc) {
         items_r += (d.m + kMinGroup - 1) / kMinGroup;
         items_c += (d.n + kMinGroup - 1) / kMinGroup;
+        segs_r += d.sa_cnt;
+        segs_c += d.st_cnt;
     }
-    b->work_r.resize(std::max<size_t>(items_r, 1));
-    b->work_c.resize(std::max<size_t>(items_c, 1));
+    b->work_r.resize(std::max<size_t>(std::max(items_r, segs_r), 1));
+    b->work_c.resize(std::max<size_t>(std::max(items_c, segs_c), 1));
     b->wcnt.resize(2);
-    const int fixed = variant_lanes_or_zero(opts.spmv_variant);
-    if (fixed > 0) b->lanes_a = b->lanes_at = fixed;
+    const int fixed = variant_kernel(opts.spmv_variant);
+    if (fixed >= 0) b->lanes_a = b->lanes_at = fixed;
     b->chunks_per_graph = std::max(1, std::min(4, 512 / std::max(1, b->check_frequency)));
     for (auto* p : {&b->c32, &b->lx32, &b->ux32, &b->x, &b->xe, &b->x0, &b->xt, &b->xl}) p->resize(Cn);
     for (auto* p : {&b->lr32, &b->ur32, &b->g32, &b->y, &b->y0, &b->yt, &b->yl}) p->resize(R);
@@ -790,7 +927,7 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->counter.resize(1);
     b->h_counter.resize(1);
     PM_HIP(hipStreamSynchronize(s));
-    if (fixed == 0) batch_autotune(b);
+    if (fixed < 0) batch_autotune(b);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0)
         b->clock_khz = khz;
@@ -820,6 +957,32 @@ BatchArrays<VT> batch_arrays(pm_batch* b, const VT* oa, const VT* oat) {
     return A;
 }
 
+// One iterate launch: half `which` (PM_KERNEL_SPMV_AT primal, PM_KERNEL_SPMV_A
+// dual) of iteration i of the chunk, with the kernel for `lanes` (0 =
+// stream).  The work items partition the rows exactly only for a grid that
+// is a multiple of 8 (xcd_slice).
+template <class VT>
+void launch_iterate(pm_batch* b, const BatchArrays<VT>& A, int which, int i, int lanes) {
+    if (b->iter_grid % 8 != 0) fail(PM_ERR_INTERNAL, "iterate grid must be a multiple of 8");
+    hipStream_t s = b->stream;
+    const dim3 gi(b->iter_grid), bi(kBlock);
+    const bool primal = which == PM_KERNEL_SPMV_AT;
+    if (!primal && b->rows_total == 0) return;
+    WorkItem* work = primal ? b->work_c.get() : b->work_r.get();
+    if (lanes == 0) {
+        if (primal) hipLaunchKernelGGL((k_t_half<true, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+        else hipLaunchKernelGGL((k_t_half<false, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+    } else {
+        with_lanes(lanes, [&](auto LC) {
+            if (primal)
+                hipLaunchKernelGGL((k_s_primal<decltype(LC)::value, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+            else
+                hipLaunchKernelGGL((k_s_dual<decltype(LC)::value, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+        });
+    }
+    PM_LAUNCHED();
+}
+
 template <class VT>
 void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
     hipStream_t s = b->stream;
@@ -829,30 +992,16 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
     switch (phase) {
         case 0: hipLaunchKernelGGL(k_b_init<VT>, g, blk, 0, s, A, S); break;
         case 1: hipLaunchKernelGGL(k_b_round<VT>, g, blk, 0, s, A, S); break;
-        case 2: {
-            // xcd_range partitions the row groups exactly only for a grid
-            // that is a multiple of 8.
-            if (b->iter_grid % 8 != 0) fail(PM_ERR_INTERNAL, "iterate grid must be a multiple of 8");
-            const dim3 gi(b->iter_grid), bi(kBlock);
+        case 2:
             for (int i = 0; i < b->check_frequency; ++i) {
-                with_lanes(b->lanes_at, [&](auto LC) {
-                    hipLaunchKernelGGL((k_s_primal<decltype(LC)::value, VT>), gi, bi, 0, s, A, b->work_c.get(),
-                                       b->wcnt.get(), i);
-                });
-                PM_LAUNCHED();
-                if (b->rows_total > 0) {
-                    with_lanes(b->lanes_a, [&](auto LC) {
-                        hipLaunchKernelGGL((k_s_dual<decltype(LC)::value, VT>), gi, bi, 0, s, A, b->work_r.get(),
-                                           b->wcnt.get(), i);
-                    });
-                    PM_LAUNCHED();
-                }
+                launch_iterate(b, A, PM_KERNEL_SPMV_AT, i, b->lanes_at);
+                launch_iterate(b, A, PM_KERNEL_SPMV_A, i, b->lanes_a);
             }
             return;
-        }
         case 4:
-            hipLaunchKernelGGL(k_b_worklist<VT>, dim3(1), dim3(1024), 0, s, A, b->count, kBlock / b->lanes_a,
-                               kBlock / b->lanes_at, b->work_r.get(), b->work_c.get(), b->wcnt.get());
+            hipLaunchKernelGGL(k_b_worklist<VT>, dim3(1), dim3(1024), 0, s, A, b->count,
+                               b->lanes_a > 0 ? kBlock / b->lanes_a : 0, b->lanes_at > 0 ? kBlock / b->lanes_at : 0,
+                               b->seg_a.get(), b->seg_at.get(), b->work_r.get(), b->work_c.get(), b->wcnt.get());
             break;
         case 3: hipLaunchKernelGGL(k_b_check<VT>, g, blk, 0, s, A, S, b->counter.get()); break;
         default: fail(PM_ERR_INTERNAL, "unknown batch phase");
@@ -954,22 +1103,7 @@ void batch_time_kernel(pm_batch* b, int which, int reps, double* avg_ms, double*
     b->d_desc.upload(b->desc, s);
     phase(b, 0);  // every LP active, chunk = check_frequency
     phase(b, 4);
-    auto launch = [&](auto A) {
-        using VT = std::remove_cv_t<std::remove_pointer_t<decltype(A.oa)>>;
-        const dim3 gi(b->iter_grid), bi(kBlock);
-        if (which == PM_KERNEL_SPMV_AT) {
-            with_lanes(b->lanes_at, [&](auto LC) {
-                hipLaunchKernelGGL((k_s_primal<decltype(LC)::value, VT>), gi, bi, 0, s, A, b->work_c.get(),
-                                   b->wcnt.get(), 0);
-            });
-        } else if (b->rows_total > 0) {
-            with_lanes(b->lanes_a, [&](auto LC) {
-                hipLaunchKernelGGL((k_s_dual<decltype(LC)::value, VT>), gi, bi, 0, s, A, b->work_r.get(),
-                                   b->wcnt.get(), 0);
-            });
-        }
-        PM_LAUNCHED();
-    };
+    auto launch = [&](const auto& A) { launch_iterate(b, A, which, 0, which == PM_KERNEL_SPMV_A ? b->lanes_a : b->lanes_at); };
     auto one = [&] {
         if (b->exact) launch(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
         else launch(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
@@ -991,9 +1125,9 @@ void batch_time_kernel(pm_batch* b, int which, int reps, double* avg_ms, double*
     if (alg_bytes) *alg_bytes = bytes;
 }
 
-// Lanes per row of the two iterate kernels: each candidate timed over the
-// whole batch with every LP running (iterates zeroed, unit steps), fastest
-// kept.  Runs once at creation.
+// Kernel of each half of the iteration (stream, or 1..32 lanes per row):
+// each candidate timed over the whole batch with every LP running
+// (iterates zeroed, small steps), fastest kept.  Runs once at creation.
 void batch_autotune(pm_batch* b) {
     hipStream_t s = b->stream;
     for (auto* p : {&b->x, &b->xe, &b->x0, &b->xt, &b->xl, &b->c32, &b->lx32, &b->ux32})
@@ -1006,7 +1140,7 @@ void batch_autotune(pm_batch* b) {
                        static_cast<int32_t>(b->desc[k].col_off), 0};
     }
     b->iter.upload(it, s);
-    const int cand[] = {1, 2, 4, 8, 16, 32};
+    const int cand[] = {0, 1, 2, 4, 8, 16, 32};  // 0 = stream kernel
     for (int which : {PM_KERNEL_SPMV_AT, PM_KERNEL_SPMV_A}) {
         if (which == PM_KERNEL_SPMV_A && b->rows_total == 0) continue;
         int& lanes = which == PM_KERNEL_SPMV_A ? b->lanes_a : b->lanes_at;
@@ -1015,19 +1149,7 @@ void batch_autotune(pm_batch* b) {
         for (int l : cand) {
             lanes = l;
             phase(b, 4);
-            auto launch = [&](auto A) {
-                using VT = std::remove_cv_t<std::remove_pointer_t<decltype(A.oa)>>;
-                const dim3 gi(b->iter_grid), bi(kBlock);
-                with_lanes(l, [&](auto LC) {
-                    if (which == PM_KERNEL_SPMV_AT)
-                        hipLaunchKernelGGL((k_s_primal<decltype(LC)::value, VT>), gi, bi, 0, s, A,
-                                           b->work_c.get(), b->wcnt.get(), 0);
-                    else
-                        hipLaunchKernelGGL((k_s_dual<decltype(LC)::value, VT>), gi, bi, 0, s, A,
-                                           b->work_r.get(), b->wcnt.get(), 0);
-                });
-                PM_LAUNCHED();
-            };
+            auto launch = [&](const auto& A) { launch_iterate(b, A, which, 0, l); };
             auto one = [&] {
                 if (b->exact) launch(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
                 else launch(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));

@@ -20,6 +20,10 @@ struct BatchDesc {
     double eta;             // 0.99 / ||A_hat||_2, 0 until the first run measured it
     double start_kkt;       // FP64 relative KKT of the starting point that was used
     int64_t rp_off, cp_off, nz_off, row_off, col_off;
+    // row segments of the stream iterate kernels: consecutive rows with at
+    // most kStreamNnz nonzeros together (a longer row alone), in seg_a
+    // (rows of A) and seg_at (rows of A^T) from these offsets
+    int32_t sa_off, sa_cnt, st_off, st_cnt;
     // results
     int32_t status, iterations, restarts, refinements;
     double rel_kkt, rel_p, rel_d, rel_g, pobj, dobj, omega;
@@ -35,6 +39,24 @@ struct LpIter {
     int32_t use_g;   // dual shift of a refinement round present
     int32_t row_off, col_off;
     int32_t pad;
+};
+
+// Nonzeros one workgroup of the stream iterate kernels stages at a time.
+constexpr int kStreamNnz = 1024;
+
+// One item of the iterate kernels' work list: a group of consecutive rows
+// of one LP (rows of A for the dual half, of A^T for the primal half) with
+// everything the kernel needs of that LP for the current chunk, so a block
+// reads one 32-byte record per item and no per-LP table.
+struct WorkItem {
+    int32_t row0;   // first global row of the group
+    int32_t cnt;    // rows in the group
+    int32_t k0;     // in-round Halpern index of the chunk's first iteration
+    int32_t chunk;  // iterations in the chunk
+    int32_t use_g;  // dual shift of a refinement round present (dual half)
+    int32_t off;    // column offset (dual half) or row offset (primal half) of the LP
+    float step;     // sigma (dual half) or tau (primal half)
+    int32_t lp;
 };
 
 }  // namespace pm
@@ -64,9 +86,10 @@ struct pm_batch {
     pm::DevBuf<double> xtot, ytot, xbase, ybase;
     pm::DevBuf<pm::BatchDesc> d_desc;
     pm::DevBuf<int> grp, gcp, wcnt;
-    pm::DevBuf<int4> work_r, work_c;
+    pm::DevBuf<pm::WorkItem> work_r, work_c;
+    pm::DevBuf<int2> seg_a, seg_at;  // {local first row, rows} per segment
     pm::DevBuf<pm::LpIter> iter;
-    int lanes_a = 8, lanes_at = 8;  // lanes per row of the grid-wide iterate kernels
+    int lanes_a = 8, lanes_at = 8;  // lanes per row of the iterate kernels; 0 = stream kernel
     int iter_grid = 2048;           // workgroups of one iterate launch: 8 per CU, a multiple of 8
     int chunks_per_graph = 4;
     bool use_graph = true;
