@@ -199,6 +199,12 @@ int pm_ctx_solve(pm_ctx* ctx, pm_pdhg_result* res);
  * algorithmic bytes one launch moves (each array touched once). */
 int pm_ctx_time_kernel(pm_ctx* ctx, int32_t which, int32_t reps,
                        double* avg_ms, double* alg_bytes);
+/* Row and column scaling factors of the loaded LP (arrays of m and n):
+ * D_r, D_c of PM_ALG_PDLP (Ruiz infinity-norm passes on A, then
+ * Pock-Chambolle 1-norms, computed on the device; the reference applies
+ * its Ruiz scaling on the host, src/presolve/scaling.cpp:29-150), all ones
+ * for PM_ALG_REFERENCE. */
+int pm_ctx_scaling(pm_ctx* ctx, double* row_scale, double* col_scale);
 /* SpMV variant the context chose for A and A^T (PM_SPMV_*). */
 int pm_ctx_spmv_variants(pm_ctx* ctx, int32_t* variant_a,
                          int32_t* variant_at);

@@ -85,6 +85,7 @@ SIGNATURES = {
     "pm_ctx_load": (C.c_int, [C.c_void_p, C.POINTER(pm_lp), C.POINTER(pm_pdhg_options)]),
     "pm_ctx_solve": (C.c_int, [C.c_void_p, C.POINTER(pm_pdhg_result)]),
     "pm_ctx_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _d, _d]),
+    "pm_ctx_scaling": (C.c_int, [C.c_void_p, _d, _d]),
     "pm_ctx_spmv_variants": (C.c_int, [C.c_void_p, _i, _i]),
     "pm_batch_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(pm_lp), C.POINTER(pm_pdhg_options),
                                   C.POINTER(C.c_void_p)]),
@@ -324,6 +325,12 @@ class Context:
         ms, b = C.c_double(), C.c_double()
         _check(lib().pm_ctx_time_kernel(self._h, which, reps, C.byref(ms), C.byref(b)))
         return ms.value, b.value
+
+    def scaling(self) -> tuple[np.ndarray, np.ndarray]:
+        """Row and column scaling factors of the loaded LP."""
+        dr, dc = np.zeros(max(self.lp.m, 1)), np.zeros(self.lp.n)
+        _check(lib().pm_ctx_scaling(self._h, _p(dr, C.c_double), _p(dc, C.c_double)))
+        return dr[:self.lp.m], dc
 
     def spmv_variants(self) -> tuple[str, str]:
         a, t = C.c_int32(), C.c_int32()

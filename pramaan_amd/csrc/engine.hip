@@ -286,6 +286,67 @@ int autotune(pm_ctx* c, int nrows, int ncols, int nnz, const int* rp, const int*
 // ---------------------------------------------------------------------------
 // Load
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Ruiz (infinity-norm passes) + Pock-Chambolle (alpha = 1) scaling of A on
+// the device.  A is kept in both orders (CSR values v, CSC values vt) and
+// both are scaled every pass, so row and column norms are row reductions in
+// one order or the other; every norm is a sequential loop in CSR (rows) or
+// CSC (columns, rows ascending) order, and every update is the host's
+// expression, so the factors equal those of ruiz_pc_scale (prep.hpp).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_row_norm(int nrows, const int* __restrict__ rp,
+                                                     const double* __restrict__ v, bool sum,
+                                                     double* __restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nrows) return;
+    double a = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) a = sum ? a + fabs(v[k]) : fmax(a, fabs(v[k]));
+    out[i] = a > 0.0 ? 1.0 / sqrt(a) : 1.0;
+}
+
+// v[k] *= rs[row] * cs[col] over one CSR order; acc[row] *= rs[row].
+__global__ __launch_bounds__(kBlock) void k_row_scale(int nrows, const int* __restrict__ rp,
+                                                      const int* __restrict__ col, double* __restrict__ v,
+                                                      const double* __restrict__ rs, const double* __restrict__ cs,
+                                                      double* __restrict__ acc) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nrows) return;
+    const double r = rs[i];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) v[k] *= r * cs[col[k]];
+    if (acc) acc[i] *= r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) b[i] = static_cast<float>(a[i]);
+}
+
+// Scales the LP whose original values are in vs (CSR) and vst (CSC), both
+// overwritten with the scaled values; dr, dc receive the factors.
+void device_ruiz_pc(pm_ctx* c, int passes, DevBuf<double>& vs, DevBuf<double>& vst, DevBuf<double>& dr,
+                    DevBuf<double>& dc) {
+    hipStream_t s = c->stream;
+    const int m = c->m, n = c->n;
+    DevBuf<double> rn(std::max(m, 1)), cn(std::max(n, 1));
+    dr.resize(std::max(m, 1));
+    dc.resize(n);
+    launch_elem(m, FnFill<double>{dr.get(), 1.0}, s);
+    launch_elem(n, FnFill<double>{dc.get(), 1.0}, s);
+    const dim3 gm((m + kBlock - 1) / kBlock), gn((n + kBlock - 1) / kBlock), b(kBlock);
+    for (int pass = 0; pass <= passes; ++pass) {
+        const bool sum = pass == passes;  // the last pass is Pock-Chambolle
+        if (m > 0) hipLaunchKernelGGL(k_row_norm, gm, b, 0, s, m, c->rp.get(), vs.get(), sum, rn.get());
+        hipLaunchKernelGGL(k_row_norm, gn, b, 0, s, n, c->cp.get(), vst.get(), sum, cn.get());
+        if (m > 0) {
+            hipLaunchKernelGGL(k_row_scale, gm, b, 0, s, m, c->rp.get(), c->ci.get(), vs.get(), rn.get(), cn.get(),
+                               dr.get());
+            hipLaunchKernelGGL(k_row_scale, gn, b, 0, s, n, c->cp.get(), c->ri.get(), vst.get(), cn.get(), rn.get(),
+                               dc.get());
+        }
+        PM_LAUNCHED();
+    }
+}
+
 void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
     check_lp(lp);
     pm_pdhg_options opts;
@@ -399,50 +460,10 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
         c->ux32.upload(ux32, s);
     } else {
         // Ruiz equilibration of A (inf-norm, opts.ruiz_iterations passes)
-        // followed by Pock-Chambolle alpha = 1 (1-norms).
-        std::vector<double> dr(m, 1.0), dc(n, 1.0), vs(lp->values, lp->values + nnz);
-        std::vector<double> rnorm(m), cnorm(n);
-        for (int pass = 0; pass < opts.ruiz_iterations; ++pass) {
-            std::fill(rnorm.begin(), rnorm.end(), 0.0);
-            std::fill(cnorm.begin(), cnorm.end(), 0.0);
-            for (int i = 0; i < m; ++i)
-                for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) {
-                    const double a = std::abs(vs[k]);
-                    rnorm[i] = std::max(rnorm[i], a);
-                    cnorm[lp->col_idx[k]] = std::max(cnorm[lp->col_idx[k]], a);
-                }
-            for (int i = 0; i < m; ++i) rnorm[i] = rnorm[i] > 0.0 ? 1.0 / std::sqrt(rnorm[i]) : 1.0;
-            for (int j = 0; j < n; ++j) cnorm[j] = cnorm[j] > 0.0 ? 1.0 / std::sqrt(cnorm[j]) : 1.0;
-            for (int i = 0; i < m; ++i)
-                for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) vs[k] *= rnorm[i] * cnorm[lp->col_idx[k]];
-            for (int i = 0; i < m; ++i) dr[i] *= rnorm[i];
-            for (int j = 0; j < n; ++j) dc[j] *= cnorm[j];
-        }
-        std::fill(rnorm.begin(), rnorm.end(), 0.0);
-        std::fill(cnorm.begin(), cnorm.end(), 0.0);
-        for (int i = 0; i < m; ++i)
-            for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) {
-                const double a = std::abs(vs[k]);
-                rnorm[i] += a;
-                cnorm[lp->col_idx[k]] += a;
-            }
-        for (int i = 0; i < m; ++i) rnorm[i] = rnorm[i] > 0.0 ? 1.0 / std::sqrt(rnorm[i]) : 1.0;
-        for (int j = 0; j < n; ++j) cnorm[j] = cnorm[j] > 0.0 ? 1.0 / std::sqrt(cnorm[j]) : 1.0;
-        for (int i = 0; i < m; ++i)
-            for (int k = lp->row_ptr[i]; k < lp->row_ptr[i + 1]; ++k) vs[k] *= rnorm[i] * cnorm[lp->col_idx[k]];
-        for (int i = 0; i < m; ++i) dr[i] *= rnorm[i];
-        for (int j = 0; j < n; ++j) dc[j] *= cnorm[j];
-
-        std::vector<float> v32(nnz), vt32(nnz);
-        for (int k = 0; k < nnz; ++k) v32[k] = static_cast<float>(vs[k]);
-        {
-            std::vector<int> cp2, ri2;
-            std::vector<float> vt;
-            host_transpose<float>(m, n, lp->row_ptr, lp->col_idx, v32.data(), cp2, ri2, vt, nullptr);
-            vt32.swap(vt);
-        }
-        c->va.upload(v32, s);
-        c->vat.upload(vt32, s);
+        // followed by Pock-Chambolle alpha = 1 (1-norms), on the device.
+        DevBuf<double> vs, vst;
+        vs.upload(lp->values, nnz, s);
+        vst.upload(vt64, s);
         if (c->exact_f32) {
             std::vector<float> o(nnz), ot(nnz);
             for (int k = 0; k < nnz; ++k) o[k] = static_cast<float>(lp->values[k]);
@@ -450,11 +471,24 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
             c->oa32.upload(o, s);
             c->oat32.upload(ot, s);
         } else {
-            c->oa64.upload(lp->values, nnz, s);
-            c->oat64.upload(vt64, s);
+            c->oa64.resize(nnz);
+            c->oat64.resize(nnz);
+            PM_HIP(hipMemcpyAsync(c->oa64.get(), vs.get(), sizeof(double) * nnz, hipMemcpyDeviceToDevice, s));
+            PM_HIP(hipMemcpyAsync(c->oat64.get(), vst.get(), sizeof(double) * nnz, hipMemcpyDeviceToDevice, s));
         }
-        up64(c->d_dr, dr);
-        up64(c->d_dc, dc);
+        device_ruiz_pc(c, opts.ruiz_iterations, vs, vst, c->d_dr, c->d_dc);
+        c->va.resize(std::max(nnz, 1));
+        c->vat.resize(std::max(nnz, 1));
+        if (nnz > 0) {
+            const dim3 gz((nnz + kBlock - 1) / kBlock), b(kBlock);
+            hipLaunchKernelGGL(k_to_f32, gz, b, 0, s, static_cast<int64_t>(nnz), vs.get(), c->va.get());
+            hipLaunchKernelGGL(k_to_f32, gz, b, 0, s, static_cast<int64_t>(nnz), vst.get(), c->vat.get());
+            PM_LAUNCHED();
+        }
+        std::vector<double> dr(std::max(m, 1)), dc(n);
+        if (m > 0) c->d_dr.download(dr.data(), m, s);
+        c->d_dc.download(dc.data(), n, s);
+        sync(c);
         for (auto* b : {&c->c32, &c->lx32, &c->ux32}) b->resize(n);
         for (auto* b : {&c->lr32, &c->ur32, &c->g32}) b->resize(std::max(m, 1));
 
@@ -995,6 +1029,17 @@ int pm_pdhg_solve(const pm_lp* lp, const pm_pdhg_options* opts, pm_pdhg_result* 
     pm_ctx_destroy(c);
     if (rc != PM_OK) pm::set_last_error(err);
     return rc;
+}
+
+int pm_ctx_scaling(pm_ctx* c, double* row_scale, double* col_scale) {
+    return pm::guarded([&] {
+        if (!c || !row_scale || !col_scale) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
+        if (!c->loaded || c->n == 0 || c->nan_cost) pm::fail(PM_ERR_INVALID_ARG, "no LP loaded");
+        PM_HIP(hipSetDevice(c->device));
+        if (c->m > 0) c->d_dr.download(row_scale, c->m, c->stream);
+        c->d_dc.download(col_scale, c->n, c->stream);
+        PM_HIP(hipStreamSynchronize(c->stream));
+    });
 }
 
 int pm_ctx_spmv_variants(pm_ctx* c, int32_t* va, int32_t* vat) {

@@ -75,8 +75,9 @@ def test_batch_iterate_kernels_agree(capi, variant):
     solves the batch, including a row longer than one stream segment, to the
     closed-form optimum."""
     lps = small_batch(capi, count=6, seed0=1300)
-    dense = capi.LP.from_dense(np.vstack([np.ones(1500), np.eye(1500)[:3]]), np.linspace(1.0, 2.0, 1500),
-                               [10.0, 0.0, 0.0, 0.0], [10.0, 5.0, 5.0, 5.0], np.zeros(1500), np.full(1500, 1e30))
+    nd = 1500  # one row longer than a stream segment (1024 nonzeros)
+    dense = capi.LP.from_dense(np.vstack([np.ones(nd), np.eye(nd)[:3]]), np.linspace(1.0, 2.0, nd),
+                               [10.0, 0.0, 0.0, 0.0], [10.0, 5.0, 5.0, 5.0], np.zeros(nd), np.full(nd, 1e30))
     lps.append(dense)
     with capi.Batch(lps, tolerance=1e-7, max_iterations=60000, check_frequency=64, spmv_variant=variant) as b:
         assert b.spmv_variants() == (capi.SPMV[variant], capi.SPMV[variant])
@@ -85,7 +86,7 @@ def test_batch_iterate_kernels_agree(capi, variant):
         assert r.status == "OPTIMAL", (variant, lp.m, r.iterations, r.rel_kkt)
         assert abs(r.objective - lp.p_star) <= 1e-5 * (1 + abs(lp.p_star))
     # min c.x, sum x = 10, x_0, x_1, x_2 <= 5, x >= 0, c increasing: x_0 = x_1 = 5
-    c = np.linspace(1.0, 2.0, 1500)
+    c = np.linspace(1.0, 2.0, nd)
     assert res[-1].status == "OPTIMAL" and abs(res[-1].objective - 5 * (c[0] + c[1])) <= 1e-4
 
 
