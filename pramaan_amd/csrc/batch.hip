@@ -568,17 +568,26 @@ __global__ __launch_bounds__(kBlock) void k_s_dual(BatchArrays<VT> A, const Work
 }
 
 // Stream variant: a work item is a segment of consecutive rows holding at
-// most kStreamNnz nonzeros.  The workgroup reads the segment's nonzeros
-// contiguously (every load coalesced), stages the products in LDS, and one
-// thread per row sums its products in CSR order; a longer row (alone in its
-// segment) is reduced by the whole workgroup.  The epilogues are the same
-// functors as the vector kernels'.
+// most kStreamNnz nonzeros (a longer row alone), split over the
+// workgroup's four wavefronts.  Each wavefront reads its rows' nonzeros as
+// one contiguous range, 64 consecutive entries per load (every load fully
+// coalesced, unlike L-lanes-per-row, where a wavefront touches a line per
+// row per step), stages the products in its own slice of LDS, and one lane
+// per row sums that row in CSR order.  No workgroup barrier: wavefronts
+// synchronise only among their own lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool PRIMAL, class VT>
 __global__ __launch_bounds__(kBlock) void k_t_half(BatchArrays<VT> A, const WorkItem* __restrict__ work,
                                                    const int* __restrict__ wcnt, int i) {
-    __shared__ float prod[kStreamNnz];
-    __shared__ float red[kBlock / 64];
-    const int t = static_cast<int>(threadIdx.x);
+    constexpr int kWaves = kBlock / 64, kWaveBuf = 512;
+    __shared__ float buf[kWaves][kWaveBuf];
+    const int lane = static_cast<int>(threadIdx.x) & 63, wv = static_cast<int>(threadIdx.x) >> 6;
+    float* lb = buf[wv];
     const int* rp = PRIMAL ? A.gcp : A.grp;
     const int* idx = PRIMAL ? A.ri : A.ci;
     const float* val = PRIMAL ? A.vat : A.va;
@@ -586,32 +595,32 @@ __global__ __launch_bounds__(kBlock) void k_t_half(BatchArrays<VT> A, const Work
     xcd_slice(wcnt[PRIMAL ? 1 : 0], w0, w1, step);
     for (int w = w0; w < w1; w += step) {
         const ItemRegs it = load_item(work, w);
-        const int row0 = it.a.x, cnt = it.a.y, chunk = it.a.w;
-        if (i >= chunk) continue;  // whole item: uniform over the workgroup
+        const int chunk = it.a.w;
+        if (i >= chunk) continue;
+        const int per = (it.a.y + kWaves - 1) / kWaves;  // rows per wavefront, <= 64
+        const int r0 = it.a.x + wv * per, r1 = min(it.a.x + it.a.y, r0 + per);
+        if (r0 >= r1) continue;  // uniform over the wavefront
         const float* xv = (PRIMAL ? A.y : A.xe) + it.b.y;
-        const int nz0 = rp[row0], nz1 = rp[row0 + cnt];
-        float s = 0.0f;
-        if (nz1 - nz0 <= kStreamNnz) {
-            for (int k = nz0 + t; k < nz1; k += kBlock) prod[k - nz0] = val[k] * xv[idx[k]];
-            __syncthreads();
-            if (t < cnt) {
-                const int b = rp[row0 + t] - nz0, e = rp[row0 + t + 1] - nz0;
-                for (int k = b; k < e; ++k) s += prod[k];
-            }
-            __syncthreads();
-        } else {  // one long row
-            float part = 0.0f;
-            for (int k = nz0 + t; k < nz1; k += kBlock) part = fmaf(val[k], xv[idx[k]], part);
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
-            if ((t & 63) == 0) red[t >> 6] = part;
-            __syncthreads();
-            if (t == 0)
-                for (int q = 0; q < kBlock / 64; ++q) s += red[q];
-            __syncthreads();
+        const int nz0 = rp[r0], nz1 = rp[r1];
+        const bool has = lane < r1 - r0;
+        int rb = 0, re = 0;
+        if (has) {
+            rb = rp[r0 + lane];
+            re = rp[r0 + lane + 1];
         }
-        if (t < cnt) {
-            const int r = row0 + t;
+        float s = 0.0f;
+        for (int base = nz0; base < nz1; base += kWaveBuf) {
+            const int top = min(nz1, base + kWaveBuf);
+            for (int k = base + lane; k < top; k += 64) lb[k - base] = val[k] * xv[idx[k]];
+            wave_sync();
+            if (has) {
+                const int a = max(rb, base), e = min(re, top);
+                for (int k = a; k < e; ++k) s += lb[k - base];
+            }
+            wave_sync();
+        }
+        if (has) {
+            const int r = r0 + lane;
             const bool keep = i == chunk - 1;
             const float stp = __int_as_float(it.b.z), lam = Control::lambda(it.a.z + i);
             if constexpr (PRIMAL)
