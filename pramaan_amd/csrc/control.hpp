@@ -19,7 +19,11 @@
 //            all iterations passed since (PDLP's adaptive restart rule)
 //   CONTINUE otherwise.
 // After RESTART the caller measures ||dx||, ||dy|| moved since the previous
-// restart and calls on_restart() (primal weight update, theta = 0.5);
+// restart and calls on_restart() (primal weight update, theta = 0.5, kept
+// within a factor kOmegaSpan = 100 of the initial weight: when x stops
+// moving the update alone drives the weight up without bound -- 160 to
+// 4.6e5 on a generated 3709 x 9273 LP whose gap then stalled at 1.0 -- and
+// the bound solved it in 3k iterations with no cost on the others tried);
 // after REFINE it calls nothing: on_check already set the new scale
 // factors DP = DD = min(1/rel, 1e4 * previous) and reset the round state.
 #pragma once
@@ -34,14 +38,14 @@ struct Control {
     enum Action { CONTINUE = 0, OPTIMAL = 1, LIMIT = 2, FAIL = 3, REFINE = 4, RESTART = 5 };
     static constexpr int kStall = 20;
     static constexpr double kSuff = 0.2, kNec = 0.8, kArt = 0.36, kTheta = 0.5, kRoundGain = 1e-3,
-                            kStallGain = 0.5;
+                            kStallGain = 0.5, kOmegaSpan = 100.0;
 
     // parameters
     double tol;
     int max_iterations;
     int max_refinements;
     // state
-    double DP, DD, omega;
+    double DP, DD, omega, omega_init;
     int total_it, k, restarts, rounds, it_since;
     double k_round_start, k_last, k_prev;
     double ring[kStall];
@@ -54,7 +58,7 @@ struct Control {
         max_iterations = max_it;
         max_refinements = max_ref;
         DP = DD = 1.0;
-        omega = omega0;
+        omega = omega_init = omega0;
         total_it = k = restarts = rounds = it_since = 0;
         use_g = warm_dual;
         new_round();
@@ -128,7 +132,11 @@ struct Control {
     }
 
     __host__ __device__ void on_restart(double dx, double dy) {
-        if (dx > 1e-10 && dy > 1e-10) omega = exp(kTheta * log(dy / dx) + (1.0 - kTheta) * log(omega));
+        if (dx > 1e-10 && dy > 1e-10) {
+            omega = exp(kTheta * log(dy / dx) + (1.0 - kTheta) * log(omega));
+            const double lo = omega_init / kOmegaSpan, hi = omega_init * kOmegaSpan;
+            omega = omega < lo ? lo : (omega > hi ? hi : omega);
+        }
     }
 
     // Halpern weight of the iteration with in-round index kk.

@@ -136,3 +136,18 @@ def test_batch_rejects_bad_input(capi):
     lp.col_idx[0] = 10**6
     with pytest.raises(capi.PramaanError):
         capi.Batch([lp])
+
+
+def test_primal_weight_stays_bounded(capi):
+    """Generated LP seed 2007 of the 4,096-LP workload (3709 x 9273): with an
+    unbounded primal-weight update the weight ran to 4.6e5 and the gap
+    stalled at 1.0 for 200k iterations; bounded within 100x of its initial
+    value, both engines solve it."""
+    lp = capi.generate(3, seed=2007)
+    with capi.Batch([lp], tolerance=1e-6, max_iterations=50000, check_frequency=64) as b:
+        (r,), _, _ = b.run()
+    assert r.status == "OPTIMAL", (r.iterations, r.rel_kkt, r.primal_weight)
+    assert abs(r.objective - lp.p_star) <= 1e-4 * (1 + abs(lp.p_star))
+    s = capi.solve(lp, algorithm="pdlp", termination="relative_kkt", tolerance=1e-6, max_iterations=50000,
+                   check_frequency=64)
+    assert s.status == "OPTIMAL" and abs(s.objective - r.objective) <= 1e-5 * (1 + abs(r.objective))
