@@ -81,7 +81,12 @@ extern "C" {
 #define PM_SPMV_VECTOR32    7
 #define PM_SPMV_VECTOR64    8
 #define PM_SPMV_STREAM      9  /* LDS-staged row segments (CSR-stream)        */
-#define PM_SPMV_NUM_VARIANTS 10
+#define PM_SPMV_SLICED      10 /* sliced ELL (batch engine): 64-row slices,
+                                  rows sorted by length within 256, one lane
+                                  per row, every load coalesced              */
+#define PM_SPMV_STAGED      11 /* sliced ELL with the gathered vector of each
+                                  LP staged in LDS (batch engine)            */
+#define PM_SPMV_NUM_VARIANTS 12
 
 /* ---- the LP, in the layout of ModelIR (include/pramaan/ir.hpp:62-119) ---
  * optimize sense: c^T x + offset  s.t.  row_lower <= A x <= row_upper,

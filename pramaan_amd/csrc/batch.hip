@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -51,9 +52,10 @@ constexpr int kBB = 1024;
 // Largest lane count the iterate kernels are built for.
 constexpr int kMaxLanes = 32;
 
-// Kernel of an explicit SpMV variant: lanes per row, 0 for the stream
-// kernel, -1 for PM_SPMV_AUTO and the variants the iterate kernels do not
-// offer (autotuned instead).
+// Kernel of an explicit SpMV variant: lanes per row, kStreamKernel,
+// kSlicedKernel, a staged code, or kAutoKernel for PM_SPMV_AUTO and the variants the
+// iterate kernels do not offer (autotuned instead).
+constexpr int kAutoKernel = -100;
 inline int variant_kernel(int v) {
     switch (v) {
         case PM_SPMV_SCALAR: return 1;
@@ -62,20 +64,87 @@ inline int variant_kernel(int v) {
         case PM_SPMV_VECTOR8: return 8;
         case PM_SPMV_VECTOR16: return 16;
         case PM_SPMV_VECTOR32: return 32;
-        case PM_SPMV_STREAM: return 0;
-        default: return -1;
+        case PM_SPMV_STREAM: return kStreamKernel;
+        case PM_SPMV_SLICED: return kSlicedKernel;
+        case PM_SPMV_STAGED: return kStagedKernel;
+        default: return kAutoKernel;
     }
 }
 
 inline int lanes_variant(int l) {
     switch (l) {
-        case 0: return PM_SPMV_STREAM;
+        case kStreamKernel: return PM_SPMV_STREAM;
+        case kSlicedKernel: return PM_SPMV_SLICED;
+        case kStagedKernel: return PM_SPMV_STAGED;
         case 1: return PM_SPMV_SCALAR;
         case 2: return PM_SPMV_VECTOR2;
         case 4: return PM_SPMV_VECTOR4;
         case 16: return PM_SPMV_VECTOR16;
         case 32: return PM_SPMV_VECTOR32;
         default: return PM_SPMV_VECTOR8;
+    }
+}
+
+// Sliced ELL form of one CSR (local indices), appended to the host arrays:
+// groups of kSliceGroup rows, rows sorted by length (longest first, ties in
+// row order) inside each group, slices of kSliceRows sorted rows; each
+// row's entries stay in CSR order.
+struct SlicedHost {
+    std::vector<float> val;
+    std::vector<int> idx, len, pos;
+    std::vector<int64_t> ptr;
+    std::vector<int2> win;
+};
+
+// Padding entries hold value 0 and the row's last column (the window's
+// first column for an empty row), so that they read an element the group
+// reads anyway.
+inline void build_sliced(int nrows, const int* rp, const int* ci, const float* v, SlicedHost& out) {
+    std::vector<int> order;
+    for (int g0 = 0; g0 < nrows; g0 += kSliceGroup) {
+        const int gn = std::min(kSliceGroup, nrows - g0);
+        int lo = INT_MAX, hi = -1;
+        for (int k = rp[g0]; k < rp[g0 + gn]; ++k) {
+            lo = std::min(lo, ci[k]);
+            hi = std::max(hi, ci[k]);
+        }
+        if (hi < 0) lo = 0, hi = -1;
+        const int2 win = make_int2(lo, hi - lo + 1);
+        order.resize(gn);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return rp[g0 + a + 1] - rp[g0 + a] > rp[g0 + b + 1] - rp[g0 + b];
+        });
+        for (int q = 0; q * kSliceRows < gn; ++q) {
+            int len = 0;
+            for (int l = 0; l < kSliceRows && q * kSliceRows + l < gn; ++l) {
+                const int r = g0 + order[q * kSliceRows + l];
+                len = std::max(len, rp[r + 1] - rp[r]);
+            }
+            out.ptr.push_back(static_cast<int64_t>(out.val.size()));
+            out.len.push_back(len);
+            out.win.push_back(win);
+            for (int l = 0; l < kSliceRows; ++l) {
+                const int p = q * kSliceRows + l;
+                out.pos.push_back(p < gn ? order[p] : -1);
+            }
+            const size_t base = out.val.size();
+            out.val.resize(base + static_cast<size_t>(len) * kSliceRows, 0.0f);
+            out.idx.resize(base + static_cast<size_t>(len) * kSliceRows, 0);
+            for (int l = 0; l < kSliceRows; ++l) {
+                const int p = q * kSliceRows + l;
+                if (p >= gn) continue;
+                const int r = g0 + order[p];
+                for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                    const size_t e = base + static_cast<size_t>(k - rp[r]) * kSliceRows + l;
+                    out.val[e] = v[k];
+                    out.idx[e] = ci[k] - lo;
+                }
+                const int last = rp[r + 1] > rp[r] ? ci[rp[r + 1] - 1] - lo : 0;
+                for (int j = rp[r + 1] - rp[r]; j < len; ++j)
+                    out.idx[base + static_cast<size_t>(j) * kSliceRows + l] = last;
+            }
+        }
     }
 }
 
@@ -112,6 +181,10 @@ struct BatchArrays {
     const int* grp;     // global row pointer of A   (nz offsets of the concatenation)
     const int* gcp;     // global row pointer of A^T
     LpIter* it;
+    // sliced A (sa_*) and A^T (st_*)
+    const float *sa_val, *st_val;
+    const int *sa_idx, *sa_ptr, *sa_len, *sa_pos, *st_idx, *st_ptr, *st_len, *st_pos;
+    const int2 *sa_win, *st_win;
     double tol;
     int max_iterations, check_frequency, max_refinements;
 };
@@ -432,6 +505,9 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
     __shared__ int sr[1024], sc[1024];
     __shared__ int carry_r, carry_c;
     const int tid = static_cast<int>(threadIdx.x);
+    // ga, gt: rows per item of the lanes kernels, 0 for stream segments,
+    // negative for groups of the sliced layout.
+    constexpr int ra = kSliceGroup, rt = kSliceGroup;
     if (tid == 0) carry_r = carry_c = 0;
     __syncthreads();
     for (int base = 0; base < count; base += 1024) {
@@ -446,8 +522,8 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
             n = d.n;
             ro = d.row_off;
             co = d.col_off;
-            nr = ga > 0 ? (m + ga - 1) / ga : d.sa_cnt;
-            nc = gt > 0 ? (n + gt - 1) / gt : d.st_cnt;
+            nr = ga > 0 ? (m + ga - 1) / ga : (ga == 0 ? d.sa_cnt : (m + ra - 1) / ra);
+            nc = gt > 0 ? (n + gt - 1) / gt : (gt == 0 ? d.st_cnt : (n + rt - 1) / rt);
         }
         sr[tid] = nr;
         sc[tid] = nc;
@@ -460,23 +536,33 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
             __syncthreads();
         }
         const int offr = carry_r + sr[tid] - nr, offc = carry_c + sc[tid] - nc;
+        // The last field carries the LP, or for the sliced and staged
+        // kernels the item's first slice.
         for (int g = 0; g < nr; ++g) {
-            int first = g * ga, rows = min(ga, m - g * ga);
+            int first = g * ga, rows = min(ga, m - g * ga), tag = k;
             if (ga == 0) {
                 const int2 sg = seg_a[A.desc[k].sa_off + g];
                 first = sg.x;
                 rows = sg.y;
+            } else if (ga < 0) {
+                first = g * ra;
+                rows = min(ra, m - first);
+                tag = A.desc[k].la_off + first / kSliceRows;
             }
-            wr[offr + g] = WorkItem{static_cast<int>(ro) + first, rows, P.k0, P.chunk, P.use_g, P.col_off, P.sig, k};
+            wr[offr + g] = WorkItem{static_cast<int>(ro) + first, rows, P.k0, P.chunk, P.use_g, P.col_off, P.sig, tag};
         }
         for (int g = 0; g < nc; ++g) {
-            int first = g * gt, rows = min(gt, n - g * gt);
+            int first = g * gt, rows = min(gt, n - g * gt), tag = k;
             if (gt == 0) {
                 const int2 sg = seg_at[A.desc[k].st_off + g];
                 first = sg.x;
                 rows = sg.y;
+            } else if (gt < 0) {
+                first = g * rt;
+                rows = min(rt, n - first);
+                tag = A.desc[k].lt_off + first / kSliceRows;
             }
-            wc[offc + g] = WorkItem{static_cast<int>(co) + first, rows, P.k0, P.chunk, 0, P.row_off, P.tau, k};
+            wc[offc + g] = WorkItem{static_cast<int>(co) + first, rows, P.k0, P.chunk, 0, P.row_off, P.tau, tag};
         }
         __syncthreads();
         if (tid == 1023) {
@@ -629,6 +715,85 @@ __global__ __launch_bounds__(kBlock) void k_t_half(BatchArrays<VT> A, const Work
                 EpiDualHal{stp, lam, 1.0f, A.lr32, A.ur32, it.b.x ? A.g32 : nullptr, A.y0, A.y,
                            keep ? A.yt : nullptr}(r, s);
         }
+    }
+}
+
+// Sliced kernels: a work item is a group of up to kSliceGroup rows of one
+// LP in the sliced layout.  The four wavefronts take the group's slices in
+// turn: lane l walks its row's entries j = 0 .. len-1 at ptr + j * 64 + l,
+// so each value and index load of 64 lanes is one contiguous 256-byte run.
+// STAGED first copies the group's column window of the gathered vector (y
+// for A^T, xe for A) into LDS with coalesced loads and gathers from there;
+// otherwise the gathers go to global memory, up to 64 distinct cache lines
+// per wavefront load.  The sums go to LDS at the row's place in the group,
+// and the epilogue runs over the group's rows in their own order
+// (coalesced vector accesses).
+template <bool PRIMAL, bool STAGED, class VT>
+__global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const WorkItem* __restrict__ work,
+                                                   const int* __restrict__ wcnt, int i) {
+    extern __shared__ float xs[];
+    __shared__ float sums[kSliceGroup];
+    const int t = static_cast<int>(threadIdx.x);
+    const int lane = t & 63, wv = t >> 6;
+    constexpr int kWaves = kBlock / 64;
+    const float* val = PRIMAL ? A.st_val : A.sa_val;
+    const int* idx = PRIMAL ? A.st_idx : A.sa_idx;
+    const int* ptr = PRIMAL ? A.st_ptr : A.sa_ptr;
+    const int* len = PRIMAL ? A.st_len : A.sa_len;
+    const int* pos = PRIMAL ? A.st_pos : A.sa_pos;
+    const int2* win = PRIMAL ? A.st_win : A.sa_win;
+    int w0, w1, step;
+    xcd_slice(wcnt[PRIMAL ? 1 : 0], w0, w1, step);
+    for (int w = w0; w < w1; w += step) {
+        const ItemRegs it = load_item(work, w);
+        const int row0 = it.a.x, cnt = it.a.y, chunk = it.a.w, first = it.b.w;
+        if (i >= chunk) continue;  // whole item: uniform over the workgroup
+        const int2 wn = win[first];
+        const float* xg = (PRIMAL ? A.y : A.xe) + it.b.y + wn.x;
+        const float* xv = xg;
+        if constexpr (STAGED) {
+            for (int q = t; q < wn.y; q += kBlock) xs[q] = xg[q];
+            __syncthreads();
+            xv = xs;
+        }
+        const int nsl = (cnt + kSliceRows - 1) / kSliceRows;
+        for (int q = wv; q < nsl; q += kWaves) {
+            const int sl = first + q;
+            const float* vp = val + ptr[sl] + lane;
+            const int* ip = idx + ptr[sl] + lane;
+            const int n = len[sl];
+            float s0 = 0.0f, s1 = 0.0f;
+            int j = 0;
+            for (; j + 4 <= n; j += 4) {
+                const float v0 = vp[0], v1 = vp[kSliceRows], v2 = vp[2 * kSliceRows], v3 = vp[3 * kSliceRows];
+                const int c0 = ip[0], c1 = ip[kSliceRows], c2 = ip[2 * kSliceRows], c3 = ip[3 * kSliceRows];
+                s0 = fmaf(v0, xv[c0], s0);
+                s1 = fmaf(v1, xv[c1], s1);
+                s0 = fmaf(v2, xv[c2], s0);
+                s1 = fmaf(v3, xv[c3], s1);
+                vp += 4 * kSliceRows;
+                ip += 4 * kSliceRows;
+            }
+            for (; j < n; ++j) {
+                s0 = fmaf(vp[0], xv[ip[0]], s0);
+                vp += kSliceRows;
+                ip += kSliceRows;
+            }
+            const int p = pos[sl * kSliceRows + lane];
+            if (p >= 0) sums[p] = s0 + s1;
+        }
+        __syncthreads();
+        const bool keep = i == chunk - 1;
+        const float stp = __int_as_float(it.b.z), lam = Control::lambda(it.a.z + i);
+        for (int r = t; r < cnt; r += kBlock) {
+            if constexpr (PRIMAL)
+                EpiPrimalHal{stp, lam, 1.0f, A.c32, A.lx32, A.ux32, A.x0, A.x, A.xe, keep ? A.xt : nullptr}(
+                    row0 + r, sums[r]);
+            else
+                EpiDualHal{stp, lam, 1.0f, A.lr32, A.ur32, it.b.x ? A.g32 : nullptr, A.y0, A.y,
+                           keep ? A.yt : nullptr}(row0 + r, sums[r]);
+        }
+        __syncthreads();
     }
 }
 
@@ -801,6 +966,7 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     std::vector<double> h_cmin(Cn), h_lx(Cn), h_ux(Cn), h_dc(Cn), h_lr(R), h_ur(R), h_dr(R);
     std::vector<int> h_grp(R + 1, 0), h_gcp(Cn + 1, 0);
     std::vector<std::vector<int2>> seg_a_lp(count), seg_at_lp(count);
+    std::vector<SlicedHost> sl_a_lp(count), sl_at_lp(count);
     h_grp[ro] = static_cast<int>(nzo);
     h_gcp[co] = static_cast<int>(nzo);
     b->c_model.assign(Cn, 0.0);
@@ -853,6 +1019,8 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         d.lanes_at = lanes_for(static_cast<double>(nnz) / n);
         stream_segments(m, L.row_ptr, seg_a_lp[k]);
         stream_segments(n, cp.data(), seg_at_lp[k]);
+        build_sliced(m, L.row_ptr, L.col_idx, h_va.data() + d.nz_off, sl_a_lp[k]);
+        build_sliced(n, cp.data(), ri.data(), h_vat.data() + d.nz_off, sl_at_lp[k]);
     };
     {
         const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -881,6 +1049,40 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         b->seg_a.upload(sa, b->stream);
         b->seg_at.upload(st, b->stream);
     }
+    // Sliced copies, concatenated in LP order with global entry offsets.
+    auto concat = [&](std::vector<SlicedHost>& parts, bool a_side, SlicedCsr& dst) {
+        SlicedHost all;
+        for (int k = 0; k < count; ++k) {
+            SlicedHost& p = parts[k];
+            (a_side ? b->desc[k].la_off : b->desc[k].lt_off) = static_cast<int32_t>(all.len.size());
+            const int64_t shift = static_cast<int64_t>(all.val.size());
+            for (int64_t q : p.ptr) all.ptr.push_back(q + shift);
+            all.len.insert(all.len.end(), p.len.begin(), p.len.end());
+            all.pos.insert(all.pos.end(), p.pos.begin(), p.pos.end());
+            all.win.insert(all.win.end(), p.win.begin(), p.win.end());
+            all.val.insert(all.val.end(), p.val.begin(), p.val.end());
+            all.idx.insert(all.idx.end(), p.idx.begin(), p.idx.end());
+            SlicedHost().val.swap(p.val);
+            SlicedHost().idx.swap(p.idx);
+        }
+        if (all.val.size() >= static_cast<size_t>(INT32_MAX))
+            fail(PM_ERR_INVALID_ARG, "batch too large for the sliced layout");
+        std::vector<int> ptr32(all.ptr.begin(), all.ptr.end());
+        int& widest = a_side ? b->win_a : b->win_at;
+        for (const int2& w : all.win) widest = std::max(widest, w.y);
+        if (ptr32.empty()) { ptr32.push_back(0); all.len.push_back(0); all.win.push_back(make_int2(0, 0)); }
+        if (all.pos.empty()) all.pos.assign(kSliceRows, -1);
+        if (all.val.empty()) { all.val.push_back(0.0f); all.idx.push_back(0); }
+        dst.val.upload(all.val, b->stream);
+        dst.idx.upload(all.idx, b->stream);
+        dst.ptr.upload(ptr32, b->stream);
+        dst.len.upload(all.len, b->stream);
+        dst.pos.upload(all.pos, b->stream);
+        dst.win.upload(all.win, b->stream);
+        PM_HIP(hipStreamSynchronize(b->stream));  // the host vectors die here
+    };
+    concat(sl_a_lp, true, b->sl_a);
+    concat(sl_at_lp, false, b->sl_at);
     // Largest LP first.
     b->order.resize(count);
     std::iota(b->order.begin(), b->order.end(), 0);
@@ -925,7 +1127,10 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->work_c.resize(std::max<size_t>(std::max(items_c, segs_c), 1));
     b->wcnt.resize(2);
     const int fixed = variant_kernel(opts.spmv_variant);
-    if (fixed >= 0) b->lanes_a = b->lanes_at = fixed;
+    if (fixed != kAutoKernel) b->lanes_a = b->lanes_at = fixed;
+    if (fixed == kStagedKernel && std::max(b->win_a, b->win_at) > kStagedMaxVec)
+        fail(PM_ERR_INVALID_ARG, "PM_SPMV_STAGED holds each group's column window in LDS: windows must stay within " +
+                                     std::to_string(kStagedMaxVec) + " columns");
     b->chunks_per_graph = std::max(1, std::min(4, 512 / std::max(1, b->check_frequency)));
     for (auto* p : {&b->c32, &b->lx32, &b->ux32, &b->x, &b->xe, &b->x0, &b->xt, &b->xl}) p->resize(Cn);
     for (auto* p : {&b->lr32, &b->ur32, &b->g32, &b->y, &b->y0, &b->yt, &b->yl}) p->resize(R);
@@ -936,7 +1141,7 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->counter.resize(1);
     b->h_counter.resize(1);
     PM_HIP(hipStreamSynchronize(s));
-    if (fixed < 0) batch_autotune(b);
+    if (fixed == kAutoKernel) batch_autotune(b);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0)
         b->clock_khz = khz;
@@ -959,6 +1164,11 @@ BatchArrays<VT> batch_arrays(pm_batch* b, const VT* oa, const VT* oat) {
     A.grp = b->grp.get();
     A.gcp = b->gcp.get();
     A.it = b->iter.get();
+    A.sa_val = b->sl_a.val.get(); A.sa_idx = b->sl_a.idx.get(); A.sa_ptr = b->sl_a.ptr.get();
+    A.sa_len = b->sl_a.len.get(); A.sa_pos = b->sl_a.pos.get();
+    A.st_val = b->sl_at.val.get(); A.st_idx = b->sl_at.idx.get(); A.st_ptr = b->sl_at.ptr.get();
+    A.st_len = b->sl_at.len.get(); A.st_pos = b->sl_at.pos.get();
+    A.sa_win = b->sl_a.win.get(); A.st_win = b->sl_at.win.get();
     A.tol = b->tol;
     A.max_iterations = b->max_iterations;
     A.check_frequency = b->check_frequency;
@@ -978,9 +1188,18 @@ void launch_iterate(pm_batch* b, const BatchArrays<VT>& A, int which, int i, int
     const bool primal = which == PM_KERNEL_SPMV_AT;
     if (!primal && b->rows_total == 0) return;
     WorkItem* work = primal ? b->work_c.get() : b->work_r.get();
-    if (lanes == 0) {
+    if (lanes == kStreamKernel) {
         if (primal) hipLaunchKernelGGL((k_t_half<true, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
         else hipLaunchKernelGGL((k_t_half<false, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+    } else if (lanes == kSlicedKernel) {
+        if (primal) hipLaunchKernelGGL((k_e_half<true, false, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+        else hipLaunchKernelGGL((k_e_half<false, false, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+    } else if (lanes == kStagedKernel) {
+        const int v = primal ? b->win_at : b->win_a;
+        if (v > kStagedMaxVec) fail(PM_ERR_INVALID_ARG, "a column window is too wide for the staged iterate kernel");
+        const size_t lds = sizeof(float) * static_cast<size_t>(std::max(v, 1));
+        if (primal) hipLaunchKernelGGL((k_e_half<true, true, VT>), gi, bi, lds, s, A, work, b->wcnt.get(), i);
+        else hipLaunchKernelGGL((k_e_half<false, true, VT>), gi, bi, lds, s, A, work, b->wcnt.get(), i);
     } else {
         with_lanes(lanes, [&](auto LC) {
             if (primal)
@@ -1009,7 +1228,8 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
             return;
         case 4:
             hipLaunchKernelGGL(k_b_worklist<VT>, dim3(1), dim3(1024), 0, s, A, b->count,
-                               b->lanes_a > 0 ? kBlock / b->lanes_a : 0, b->lanes_at > 0 ? kBlock / b->lanes_at : 0,
+                               b->lanes_a > 0 ? kBlock / b->lanes_a : b->lanes_a,
+                               b->lanes_at > 0 ? kBlock / b->lanes_at : b->lanes_at,  // rows per item, or a code
                                b->seg_a.get(), b->seg_at.get(), b->work_r.get(), b->work_c.get(), b->wcnt.get());
             break;
         case 3: hipLaunchKernelGGL(k_b_check<VT>, g, blk, 0, s, A, S, b->counter.get()); break;
@@ -1134,7 +1354,8 @@ void batch_time_kernel(pm_batch* b, int which, int reps, double* avg_ms, double*
     if (alg_bytes) *alg_bytes = bytes;
 }
 
-// Kernel of each half of the iteration (stream, or 1..32 lanes per row):
+// Kernel of each half of the iteration (staged, sliced, stream, or 1..32
+// lanes per row):
 // each candidate timed over the whole batch with every LP running
 // (iterates zeroed, small steps), fastest kept.  Runs once at creation.
 void batch_autotune(pm_batch* b) {
@@ -1149,13 +1370,15 @@ void batch_autotune(pm_batch* b) {
                        static_cast<int32_t>(b->desc[k].col_off), 0};
     }
     b->iter.upload(it, s);
-    const int cand[] = {0, 1, 2, 4, 8, 16, 32};  // 0 = stream kernel
+    const int cand[] = {kStagedKernel, kSlicedKernel, kStreamKernel, 1, 2, 4, 8, 16, 32};
     for (int which : {PM_KERNEL_SPMV_AT, PM_KERNEL_SPMV_A}) {
         if (which == PM_KERNEL_SPMV_A && b->rows_total == 0) continue;
         int& lanes = which == PM_KERNEL_SPMV_A ? b->lanes_a : b->lanes_at;
+        const bool staged_fits = (which == PM_KERNEL_SPMV_A ? b->win_a : b->win_at) <= kStagedMaxVec;
         double best = INFINITY;
         int best_l = lanes;
         for (int l : cand) {
+            if (l == kStagedKernel && !staged_fits) continue;
             lanes = l;
             phase(b, 4);
             auto launch = [&](const auto& A) { launch_iterate(b, A, which, 0, l); };

@@ -24,6 +24,10 @@ struct BatchDesc {
     // most kStreamNnz nonzeros together (a longer row alone), in seg_a
     // (rows of A) and seg_at (rows of A^T) from these offsets
     int32_t sa_off, sa_cnt, st_off, st_cnt;
+    // slices of the sliced iterate kernels (64 rows each, rows sorted by
+    // length inside every group of 256), from these offsets in the sliced
+    // arrays of A and A^T
+    int32_t la_off, lt_off;
     // results
     int32_t status, iterations, restarts, refinements;
     double rel_kkt, rel_p, rel_d, rel_g, pobj, dobj, omega;
@@ -43,6 +47,28 @@ struct LpIter {
 
 // Nonzeros one workgroup of the stream iterate kernels stages at a time.
 constexpr int kStreamNnz = 1024;
+
+// Kernel codes of the iterate halves besides 1..32 lanes per row: the
+// stream kernel, and the two kernels over the sliced layout, gathering
+// from global memory (sliced) or from a copy in LDS (staged).
+constexpr int kStreamKernel = 0, kSlicedKernel = -1, kStagedKernel = -2;
+// Sliced layout: groups of kSliceGroup consecutive rows, one group per
+// work item; inside a group the rows are sorted by length and cut into
+// slices of kSliceRows (one row per lane of a wavefront).
+constexpr int kSliceRows = 64, kSliceGroup = 1024;
+// Largest column window (floats) the staged kernel holds in LDS.
+constexpr int kStagedMaxVec = 12 * 1024;
+
+// A CSR in sliced ELL form.  Slice s holds len[s] columns of 64 lanes:
+// entry (j, lane) at ptr[s] + j * 64 + lane; pos[s * 64 + lane] is the
+// lane's row as an offset inside its group (-1 for a padding lane).  The
+// column indexes are relative to the group's window win[s] = {first
+// column, columns} (every column the group's rows hold lies inside it).
+struct SlicedCsr {
+    DevBuf<float> val;
+    DevBuf<int> idx, ptr, len, pos;
+    DevBuf<int2> win;
+};
 
 // One item of the iterate kernels' work list: a group of consecutive rows
 // of one LP (rows of A for the dual half, of A^T for the primal half) with
@@ -88,6 +114,8 @@ struct pm_batch {
     pm::DevBuf<int> grp, gcp, wcnt;
     pm::DevBuf<pm::WorkItem> work_r, work_c;
     pm::DevBuf<int2> seg_a, seg_at;  // {local first row, rows} per segment
+    pm::SlicedCsr sl_a, sl_at;       // sliced copies of A and A^T
+    int win_a = 0, win_at = 0;       // widest group window of A and A^T (staged kernel's LDS)
     pm::DevBuf<pm::LpIter> iter;
     int lanes_a = 8, lanes_at = 8;  // lanes per row of the iterate kernels; 0 = stream kernel
     int iter_grid = 2048;           // workgroups of one iterate launch: 8 per CU, a multiple of 8

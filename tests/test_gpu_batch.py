@@ -69,9 +69,10 @@ def test_batch_one_lp_and_mixed_sizes(capi):
             assert abs(r.objective - lp.p_star) <= 1e-5 * (1 + abs(lp.p_star))
 
 
-@pytest.mark.parametrize("variant", ["stream", "scalar", "vector4", "vector32"])
+@pytest.mark.parametrize("variant", ["stream", "sliced", "staged", "scalar", "vector4", "vector32"])
 def test_batch_iterate_kernels_agree(capi, variant):
-    """Every iterate kernel (LDS-staged stream segments, 1..32 lanes per row)
+    """Every iterate kernel (LDS-staged stream segments, sliced ELL, sliced ELL
+    gathering from LDS, 1..32 lanes per row)
     solves the batch, including a row longer than one stream segment, to the
     closed-form optimum."""
     lps = small_batch(capi, count=6, seed0=1300)
