@@ -51,6 +51,10 @@ constexpr int kBB = 1024;
 
 // Largest lane count the iterate kernels are built for.
 constexpr int kMaxLanes = 32;
+// Nonzeros of the running LPs below which the batch switches to the tail
+// kernels (lanes per row by mean row length): about where a sliced launch,
+// a few workgroups per LP, stops being throughput-bound.
+constexpr int64_t kTailNnz = int64_t(3) << 20;
 
 // Kernel of an explicit SpMV variant: lanes per row, kStreamKernel,
 // kSlicedKernel, a staged code, or kAutoKernel for PM_SPMV_AUTO and the variants the
@@ -718,10 +722,35 @@ __global__ __launch_bounds__(kBlock) void k_t_half(BatchArrays<VT> A, const Work
     }
 }
 
+// Steps j0 .. j1-1 of NA slices at once (one row per lane in each): the
+// 2 NA loads of a step are independent, so a wavefront keeps them in
+// flight together.  Each row is summed in CSR order with FMA, as the
+// one-lane-per-row kernels sum it.
+template <int NA>
+__device__ __forceinline__ void slice_steps(const float* __restrict__ val, const int* __restrict__ idx,
+                                            const float* xv, const int (&base)[4], int j0, int j1,
+                                            float (&s)[4]) {
+#pragma unroll 2
+    for (int j = j0; j < j1; ++j) {
+        float v[NA];
+        int c[NA];
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            v[u] = val[base[u] + j * kSliceRows];
+            c[u] = idx[base[u] + j * kSliceRows];
+        }
+#pragma unroll
+        for (int u = 0; u < NA; ++u) s[u] = fmaf(v[u], xv[c[u]], s[u]);
+    }
+}
+
 // Sliced kernels: a work item is a group of up to kSliceGroup rows of one
-// LP in the sliced layout.  The four wavefronts take the group's slices in
-// turn: lane l walks its row's entries j = 0 .. len-1 at ptr + j * 64 + l,
-// so each value and index load of 64 lanes is one contiguous 256-byte run.
+// LP in the sliced layout.  Wavefront w takes slices w, w + 4, w + 8 and
+// w + 12 of the group and walks them together: lane l reads entry j of its
+// row in slice q at ptr[q] + j * 64 + l, so each value and index load of
+// 64 lanes is one contiguous 256-byte run.  Slice lengths do not grow
+// along the group (its rows are sorted by length), so the walk runs all
+// four slices up to the shortest one's length, then three, two, one.
 // STAGED first copies the group's column window of the gathered vector (y
 // for A^T, xe for A) into LDS with coalesced loads and gathers from there;
 // otherwise the gathers go to global memory, up to 64 distinct cache lines
@@ -731,11 +760,12 @@ __global__ __launch_bounds__(kBlock) void k_t_half(BatchArrays<VT> A, const Work
 template <bool PRIMAL, bool STAGED, class VT>
 __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const WorkItem* __restrict__ work,
                                                    const int* __restrict__ wcnt, int i) {
+    constexpr int kWaves = kBlock / 64, kPer = kSliceGroup / kSliceRows / kWaves;
+    static_assert(kPer == 4, "four slices per wavefront");
     extern __shared__ float xs[];
     __shared__ float sums[kSliceGroup];
     const int t = static_cast<int>(threadIdx.x);
     const int lane = t & 63, wv = t >> 6;
-    constexpr int kWaves = kBlock / 64;
     const float* val = PRIMAL ? A.st_val : A.sa_val;
     const int* idx = PRIMAL ? A.st_idx : A.sa_idx;
     const int* ptr = PRIMAL ? A.st_ptr : A.sa_ptr;
@@ -748,40 +778,32 @@ __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const Work
         const ItemRegs it = load_item(work, w);
         const int row0 = it.a.x, cnt = it.a.y, chunk = it.a.w, first = it.b.w;
         if (i >= chunk) continue;  // whole item: uniform over the workgroup
+        const int nsl = (cnt + kSliceRows - 1) / kSliceRows;
+        int base[kPer], ln[kPer], ps[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int q = wv + kWaves * u;
+            const bool on = q < nsl;
+            base[u] = on ? ptr[first + q] + lane : 0;
+            ln[u] = on ? len[first + q] : 0;
+            ps[u] = on ? pos[(first + q) * kSliceRows + lane] : -1;
+        }
         const int2 wn = win[first];
-        const float* xg = (PRIMAL ? A.y : A.xe) + it.b.y + wn.x;
-        const float* xv = xg;
+        const float* xv = (PRIMAL ? A.y : A.xe) + it.b.y + wn.x;
         if constexpr (STAGED) {
-            for (int q = t; q < wn.y; q += kBlock) xs[q] = xg[q];
+#pragma unroll 4
+            for (int q = t; q < wn.y; q += kBlock) xs[q] = xv[q];
             __syncthreads();
             xv = xs;
         }
-        const int nsl = (cnt + kSliceRows - 1) / kSliceRows;
-        for (int q = wv; q < nsl; q += kWaves) {
-            const int sl = first + q;
-            const float* vp = val + ptr[sl] + lane;
-            const int* ip = idx + ptr[sl] + lane;
-            const int n = len[sl];
-            float s0 = 0.0f, s1 = 0.0f;
-            int j = 0;
-            for (; j + 4 <= n; j += 4) {
-                const float v0 = vp[0], v1 = vp[kSliceRows], v2 = vp[2 * kSliceRows], v3 = vp[3 * kSliceRows];
-                const int c0 = ip[0], c1 = ip[kSliceRows], c2 = ip[2 * kSliceRows], c3 = ip[3 * kSliceRows];
-                s0 = fmaf(v0, xv[c0], s0);
-                s1 = fmaf(v1, xv[c1], s1);
-                s0 = fmaf(v2, xv[c2], s0);
-                s1 = fmaf(v3, xv[c3], s1);
-                vp += 4 * kSliceRows;
-                ip += 4 * kSliceRows;
-            }
-            for (; j < n; ++j) {
-                s0 = fmaf(vp[0], xv[ip[0]], s0);
-                vp += kSliceRows;
-                ip += kSliceRows;
-            }
-            const int p = pos[sl * kSliceRows + lane];
-            if (p >= 0) sums[p] = s0 + s1;
-        }
+        float sm[kPer] = {0.0f, 0.0f, 0.0f, 0.0f};
+        slice_steps<4>(val, idx, xv, base, 0, ln[3], sm);
+        slice_steps<3>(val, idx, xv, base, ln[3], ln[2], sm);
+        slice_steps<2>(val, idx, xv, base, ln[2], ln[1], sm);
+        slice_steps<1>(val, idx, xv, base, ln[1], ln[0], sm);
+#pragma unroll
+        for (int u = 0; u < kPer; ++u)
+            if (ps[u] >= 0) sums[ps[u]] = sm[u];
         __syncthreads();
         const bool keep = i == chunk - 1;
         const float stp = __int_as_float(it.b.z), lam = Control::lambda(it.a.z + i);
@@ -901,6 +923,7 @@ namespace pm {
 namespace {
 
 void batch_autotune(pm_batch* b);
+void set_mode(pm_batch* b, bool tail);
 
 void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_options* o) {
     if (count <= 0) fail(PM_ERR_INVALID_ARG, "batch count must be positive");
@@ -1141,7 +1164,13 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->counter.resize(1);
     b->h_counter.resize(1);
     PM_HIP(hipStreamSynchronize(s));
-    if (fixed == kAutoKernel) batch_autotune(b);
+    if (fixed == kAutoKernel) {
+        b->tail_a = b->lanes_a;
+        b->tail_at = b->lanes_at;
+        b->tail_nnz = kTailNnz;
+        batch_autotune(b);
+    }
+    set_mode(b, false);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0)
         b->clock_khz = khz;
@@ -1222,14 +1251,14 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
         case 1: hipLaunchKernelGGL(k_b_round<VT>, g, blk, 0, s, A, S); break;
         case 2:
             for (int i = 0; i < b->check_frequency; ++i) {
-                launch_iterate(b, A, PM_KERNEL_SPMV_AT, i, b->lanes_at);
-                launch_iterate(b, A, PM_KERNEL_SPMV_A, i, b->lanes_a);
+                launch_iterate(b, A, PM_KERNEL_SPMV_AT, i, b->mode_at);
+                launch_iterate(b, A, PM_KERNEL_SPMV_A, i, b->mode_a);
             }
             return;
         case 4:
             hipLaunchKernelGGL(k_b_worklist<VT>, dim3(1), dim3(1024), 0, s, A, b->count,
-                               b->lanes_a > 0 ? kBlock / b->lanes_a : b->lanes_a,
-                               b->lanes_at > 0 ? kBlock / b->lanes_at : b->lanes_at,  // rows per item, or a code
+                               b->mode_a > 0 ? kBlock / b->mode_a : b->mode_a,
+                               b->mode_at > 0 ? kBlock / b->mode_at : b->mode_at,  // rows per item, or a code
                                b->seg_a.get(), b->seg_at.get(), b->work_r.get(), b->work_c.get(), b->wcnt.get());
             break;
         case 3: hipLaunchKernelGGL(k_b_check<VT>, g, blk, 0, s, A, S, b->counter.get()); break;
@@ -1241,6 +1270,12 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
 void phase(pm_batch* b, int p) {
     if (b->exact) launch_phase<float>(b, p, b->oa32.get(), b->oat32.get());
     else launch_phase<double>(b, p, b->oa64.get(), b->oat64.get());
+}
+
+// Kernels of the launches issued next: the tuned ones, or the tail ones.
+void set_mode(pm_batch* b, bool tail) {
+    b->mode_a = tail ? b->tail_a : b->lanes_a;
+    b->mode_at = tail ? b->tail_at : b->lanes_at;
 }
 
 // One chunk: clear the active counter, round data, K iterations, check.
@@ -1255,28 +1290,50 @@ void chunk_launches(pm_batch* b) {
 
 void batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
     hipStream_t s = b->stream;
+    // Tail kernels: taken once the running LPs hold at most tail_nnz
+    // nonzeros (estimated from the running count and the mean LP), by a
+    // second graph that first rebuilds the work list for them.
+    const bool can_tail = b->tail_nnz > 0 && (b->tail_a != b->lanes_a || b->tail_at != b->lanes_at);
+    const double mean_nnz = static_cast<double>(b->nnz_total) / b->count;
     if (b->use_graph && !b->exec) {
+        set_mode(b, false);
         PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int q = 0; q < b->chunks_per_graph; ++q) chunk_launches(b);
         PM_HIP(hipStreamEndCapture(s, &b->graph));
         PM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        if (can_tail) {
+            set_mode(b, true);
+            PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            phase(b, 4);
+            for (int q = 0; q < b->chunks_per_graph; ++q) chunk_launches(b);
+            PM_HIP(hipStreamEndCapture(s, &b->graph_tail));
+            PM_HIP(hipGraphInstantiate(&b->exec_tail, b->graph_tail, nullptr, nullptr, 0));
+        }
     }
     // Warm-start flags may have changed since the last run.
     b->d_desc.upload(b->desc, s);
     PM_HIP(hipEventRecord(b->ev0, s));
+    set_mode(b, false);
     phase(b, 0);
     phase(b, 4);
+    bool tail = false;
     const int max_launches = b->max_iterations / std::max(1, b->check_frequency) + 4 * b->max_refinements + 8;
     for (int q = 0; q < max_launches; q += b->chunks_per_graph) {
         if (b->use_graph) {
-            PM_HIP(hipGraphLaunch(b->exec, s));
+            PM_HIP(hipGraphLaunch(tail ? b->exec_tail : b->exec, s));
         } else {
+            if (tail) phase(b, 4);
             for (int c = 0; c < b->chunks_per_graph; ++c) chunk_launches(b);
         }
         PM_HIP(hipMemcpyAsync(b->h_counter.get(), b->counter.get(), sizeof(int), hipMemcpyDeviceToHost, s));
         PM_HIP(hipStreamSynchronize(s));
         if (b->h_counter[0] == 0) break;
+        if (can_tail && !tail && b->h_counter[0] * mean_nnz <= static_cast<double>(b->tail_nnz)) {
+            tail = true;
+            set_mode(b, true);
+        }
     }
+    set_mode(b, false);
     PM_HIP(hipEventRecord(b->ev1, s));
     PM_HIP(hipEventSynchronize(b->ev1));
     float ms = 0.0f;
@@ -1330,6 +1387,7 @@ void batch_time_kernel(pm_batch* b, int which, int reps, double* avg_ms, double*
     if (reps <= 0) fail(PM_ERR_INVALID_ARG, "reps must be positive");
     hipStream_t s = b->stream;
     b->d_desc.upload(b->desc, s);
+    set_mode(b, false);
     phase(b, 0);  // every LP active, chunk = check_frequency
     phase(b, 4);
     auto launch = [&](const auto& A) { launch_iterate(b, A, which, 0, which == PM_KERNEL_SPMV_A ? b->lanes_a : b->lanes_at); };
@@ -1380,6 +1438,7 @@ void batch_autotune(pm_batch* b) {
         for (int l : cand) {
             if (l == kStagedKernel && !staged_fits) continue;
             lanes = l;
+            set_mode(b, false);
             phase(b, 4);
             auto launch = [&](const auto& A) { launch_iterate(b, A, which, 0, l); };
             auto one = [&] {
@@ -1400,6 +1459,7 @@ void batch_autotune(pm_batch* b) {
         }
         lanes = best_l;
     }
+    set_mode(b, false);
 }
 
 }  // namespace
@@ -1498,6 +1558,8 @@ int pm_batch_destroy(pm_batch* b) {
         if (b->stream) (void)hipStreamSynchronize(b->stream);
         if (b->exec) (void)hipGraphExecDestroy(b->exec);
         if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->exec_tail) (void)hipGraphExecDestroy(b->exec_tail);
+        if (b->graph_tail) (void)hipGraphDestroy(b->graph_tail);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
         hipStream_t s = b->stream;

@@ -91,8 +91,8 @@ struct pm_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr, graph_tail = nullptr;
+    hipGraphExec_t exec = nullptr, exec_tail = nullptr;
     int count = 0;
     bool exact = true;
     double tol = 1e-6;
@@ -117,7 +117,13 @@ struct pm_batch {
     pm::SlicedCsr sl_a, sl_at;       // sliced copies of A and A^T
     int win_a = 0, win_at = 0;       // widest group window of A and A^T (staged kernel's LDS)
     pm::DevBuf<pm::LpIter> iter;
-    int lanes_a = 8, lanes_at = 8;  // lanes per row of the iterate kernels; 0 = stream kernel
+    int lanes_a = 8, lanes_at = 8;  // iterate kernels: lanes per row, or a kernel code
+    // Kernels once the running LPs hold at most tail_nnz nonzeros (0 =
+    // never switch): lanes per row, many small work items per LP, for the
+    // latency of a launch rather than its throughput.
+    int tail_a = 8, tail_at = 8;
+    int64_t tail_nnz = 0;
+    int mode_a = 8, mode_at = 8;    // kernels the launches being issued use
     int iter_grid = 2048;           // workgroups of one iterate launch: 8 per CU, a multiple of 8
     int chunks_per_graph = 4;
     bool use_graph = true;

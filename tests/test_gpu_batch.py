@@ -56,6 +56,28 @@ def test_batch_direct_launches_match_graph(capi):
         assert a.iterations == c.iterations and np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y)
 
 
+def test_batch_sliced_kernels_match_one_lane_per_row(capi):
+    """The sliced kernels (global and LDS gathers) sum every row in CSR
+    order with FMA, as the one-lane-per-row kernel does, so whole solves
+    give the same bits; the batch mixes groups of 1024 rows with partial
+    groups and slices, an empty row and a 1500-entry row."""
+    lps = small_batch(capi, count=5, seed0=1700)
+    lps.append(capi.generate(None, seed=1705, m=2300, n=5750, k_min=4, k_max=16))
+    nd = 1500
+    dense = np.vstack([np.ones(nd), np.eye(nd)[:3], np.zeros(nd)])
+    lps.append(capi.LP.from_dense(dense, np.linspace(1.0, 2.0, nd), [10.0, 0.0, 0.0, 0.0, -1.0],
+                                  [10.0, 5.0, 5.0, 5.0, 1.0], np.zeros(nd), np.full(nd, 1e30)))
+    out = {}
+    for variant in ("scalar", "sliced", "staged"):
+        with capi.Batch(lps, tolerance=1e-7, max_iterations=60000, check_frequency=64, spmv_variant=variant) as b:
+            out[variant], _, _ = b.run(want_vectors=True)
+    for variant in ("sliced", "staged"):
+        for a, c in zip(out["scalar"], out[variant]):
+            assert a.status == c.status == "OPTIMAL"
+            assert a.iterations == c.iterations and a.objective == c.objective, variant
+            assert np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y), variant
+
+
 def test_batch_one_lp_and_mixed_sizes(capi):
     """A batch of one LP and a batch whose sizes differ 100x (the work list
     and the per-LP chunk lengths), against each LP's closed form."""
