@@ -49,6 +49,10 @@ namespace {
 // enough loads in flight to stream its data from HBM.
 constexpr int kBB = 1024;
 
+// Rows per block of the round and check passes, and doubles per block of
+// the check's partial sums.
+constexpr int kCheckRows = 128, kPart = 8;
+
 // Largest lane count the iterate kernels are built for.
 constexpr int kMaxLanes = 32;
 // Nonzeros of the running LPs below which the batch switches to the tail
@@ -185,6 +189,10 @@ struct BatchArrays {
     const int* grp;     // global row pointer of A   (nz offsets of the concatenation)
     const int* gcp;     // global row pointer of A^T
     LpIter* it;
+    int4* wrange;
+    int2 *ck_r, *ck_c;
+    double *part_r, *part_c;
+    const int* wcnt;
     // sliced A (sa_*) and A^T (st_*)
     const float *sa_val, *st_val;
     const int *sa_idx, *sa_ptr, *sa_len, *sa_pos, *st_idx, *st_ptr, *st_len, *st_pos;
@@ -195,9 +203,9 @@ struct BatchArrays {
 
 // ---- block helpers -------------------------------------------------------
 
-template <int NS, int NM>
+template <int NS, int NM, int NT = kBB>
 __device__ void block_reduce_all(double (&acc)[NS + NM], double* out) {
-    __shared__ double tmp[kBB / 64][16];
+    __shared__ double tmp[NT / 64][16];
 #pragma unroll
     for (int j = 0; j < NS + NM; ++j) {
         double a = acc[j];
@@ -217,7 +225,7 @@ __device__ void block_reduce_all(double (&acc)[NS + NM], double* out) {
     if (threadIdx.x < NS + NM) {
         const int j = threadIdx.x;
         double a = tmp[0][j];
-        for (int w = 1; w < kBB / 64; ++w) a = (j < NS) ? a + tmp[w][j] : (tmp[w][j] > a ? tmp[w][j] : a);
+        for (int w = 1; w < NT / 64; ++w) a = (j < NS) ? a + tmp[w][j] : (tmp[w][j] > a ? tmp[w][j] : a);
         out[j] = a;
     }
     __syncthreads();
@@ -357,6 +365,20 @@ __device__ __forceinline__ void publish_iter(LpIter* it, int lp, const BatchStat
 
 // FP64 relative KKT of (xtot, ytot) into the descriptor; returns rel on
 // thread 0 (every thread must call).
+// Relative KKT error of a total iterate from the reduced row terms kr
+// (EpiKktRow) and column terms kc (EpiKktCol); stored into d.
+__device__ inline double kkt_finish(const double* kr, const double* kc, BatchDesc& d) {
+    const double p = kc[4], dd2 = kr[5] + kc[3];
+    const double rp = sqrt(kr[0]) / (1.0 + sqrt(kr[2]));
+    const double rdd = sqrt(kr[3] + kc[0]) / (1.0 + sqrt(kc[2]));
+    const double rg = fabs(p - dd2) / (1.0 + fabs(p) + fabs(dd2));
+    double rel = rp > rdd ? rp : rdd;
+    rel = rel > rg ? rel : rg;
+    if (!(rel == rel)) rel = INFINITY;
+    d.rel_kkt = rel; d.rel_p = rp; d.rel_d = rdd; d.rel_g = rg; d.pobj = p; d.dobj = dd2;
+    return rel;
+}
+
 template <class VT>
 __device__ double batch_kkt(const View<VT>& v, BatchDesc& d, double* red, double* keep) {
     brows_red_any(v.la, v.m, v.rp, v.ci, v.oa, v.xtot, EpiKktRow{v.lr, v.ur, v.ytot, v.dr}, red);
@@ -364,16 +386,7 @@ __device__ double batch_kkt(const View<VT>& v, BatchDesc& d, double* red, double
     __syncthreads();
     brows_red_any(v.lat, v.n, v.cp, v.ri, v.oat, v.ytot, EpiKktCol{v.cmin, v.lx, v.ux, v.xtot, v.dc}, red);
     double rel = 0.0;
-    if (threadIdx.x == 0) {
-        const double p = red[4], dd2 = keep[5] + red[3];
-        const double rp = sqrt(keep[0]) / (1.0 + sqrt(keep[2]));
-        const double rdd = sqrt(keep[3] + red[0]) / (1.0 + sqrt(red[2]));
-        const double rg = fabs(p - dd2) / (1.0 + fabs(p) + fabs(dd2));
-        rel = rp > rdd ? rp : rdd;
-        rel = rel > rg ? rel : rg;
-        if (!(rel == rel)) rel = INFINITY;
-        d.rel_kkt = rel; d.rel_p = rp; d.rel_d = rdd; d.rel_g = rg; d.pobj = p; d.dobj = dd2;
-    }
+    if (threadIdx.x == 0) rel = kkt_finish(keep, red, d);
     __syncthreads();
     return rel;
 }
@@ -470,53 +483,31 @@ __global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S
     }
 }
 
-// Phase 1: correction-LP data of a new refinement round, iterates reset.
-template <class VT>
-__global__ __launch_bounds__(kBB) void k_b_round(BatchArrays<VT> A, BatchState* S) {
-    const int lp = A.order[blockIdx.x];
-    BatchState& st = S[lp];
-    if (!st.active || !st.round_start) return;
-    const BatchDesc& d = A.desc[lp];
-    const View<VT> v = view_of(A, d);
-    const int m = v.m, n = v.n, tid = threadIdx.x;
-    const double dp = st.ctl.DP, dd = st.ctl.DD;
-    brows_any<double>(v.lat, n, v.cp, v.ri, v.oat, v.ybase,
-                      EpiRoundCol{v.cmin, v.lx, v.ux, v.xbase, v.dc, dp, dd, v.c32, v.lx32, v.ux32});
-    brows_any<double>(v.la, m, v.rp, v.ci, v.oa, v.xbase,
-                      EpiRoundRow{v.lr, v.ur, v.ybase, v.dr, dp, dd, v.lr32, v.ur32, v.g32});
-    __syncthreads();
-    for (int j = tid; j < n; j += kBB) {
-        const float x0v = fmaxf(v.lx32[j], fminf(0.0f, v.ux32[j]));
-        v.x[j] = x0v;
-        v.x0[j] = x0v;
-        v.xl[j] = x0v;
-    }
-    for (int i = tid; i < m; i += kBB) v.y[i] = v.y0[i] = v.yl[i] = 0.0f;
-    __syncthreads();
-    if (tid == 0) st.round_start = 0;
-}
 
 // Work list of the iterate kernels, rebuilt after every check: one
 // WorkItem per group of G rows of every LP still running (G = 256 / lanes;
-// G = 0 selects the LP's stream segments instead), in LP order, carrying
-// that LP's parameters for the chunk; counts in wcnt[0] (rows of A) and
-// wcnt[1] (rows of A^T).  One 1024-thread workgroup scans the LPs.
+// G = 0 selects the LP's stream segments, G < 0 the groups of the sliced
+// layout), in LP order, carrying that LP's parameters for the chunk;
+// counts in wcnt[0] (rows of A) and wcnt[1] (rows of A^T).  Also the
+// blocks of the round and check passes (ck_r, ck_c, counts in wcnt[2],
+// wcnt[3], each LP's range in wrange).  One 1024-thread workgroup scans
+// the LPs.
 template <class VT>
 __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int count, int ga, int gt,
                                                      const int2* __restrict__ seg_a, const int2* __restrict__ seg_at,
                                                      WorkItem* __restrict__ wr, WorkItem* __restrict__ wc,
                                                      int* __restrict__ wcnt) {
-    __shared__ int sr[1024], sc[1024];
-    __shared__ int carry_r, carry_c;
+    __shared__ int sr[1024], sc[1024], kr[1024], kc[1024];
+    __shared__ int carry_r, carry_c, carry_kr, carry_kc;
     const int tid = static_cast<int>(threadIdx.x);
     // ga, gt: rows per item of the lanes kernels, 0 for stream segments,
     // negative for groups of the sliced layout.
     constexpr int ra = kSliceGroup, rt = kSliceGroup;
-    if (tid == 0) carry_r = carry_c = 0;
+    if (tid == 0) carry_r = carry_c = carry_kr = carry_kc = 0;
     __syncthreads();
     for (int base = 0; base < count; base += 1024) {
         const int k = base + tid;
-        int nr = 0, nc = 0, m = 0, n = 0;
+        int nr = 0, nc = 0, m = 0, n = 0, br = 0, bc = 0;
         int64_t ro = 0, co = 0;
         LpIter P{};
         if (k < count) P = A.it[k];
@@ -528,18 +519,29 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
             co = d.col_off;
             nr = ga > 0 ? (m + ga - 1) / ga : (ga == 0 ? d.sa_cnt : (m + ra - 1) / ra);
             nc = gt > 0 ? (n + gt - 1) / gt : (gt == 0 ? d.st_cnt : (n + rt - 1) / rt);
+            br = (m + kCheckRows - 1) / kCheckRows;
+            bc = (n + kCheckRows - 1) / kCheckRows;
         }
         sr[tid] = nr;
         sc[tid] = nc;
+        kr[tid] = br;
+        kc[tid] = bc;
         __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan (Hillis-Steele)
+        for (int o = 1; o < 1024; o <<= 1) {  // inclusive scans (Hillis-Steele)
             const int ar = tid >= o ? sr[tid - o] : 0, ac = tid >= o ? sc[tid - o] : 0;
+            const int akr = tid >= o ? kr[tid - o] : 0, akc = tid >= o ? kc[tid - o] : 0;
             __syncthreads();
             sr[tid] += ar;
             sc[tid] += ac;
+            kr[tid] += akr;
+            kc[tid] += akc;
             __syncthreads();
         }
         const int offr = carry_r + sr[tid] - nr, offc = carry_c + sc[tid] - nc;
+        const int offkr = carry_kr + kr[tid] - br, offkc = carry_kc + kc[tid] - bc;
+        for (int g = 0; g < br; ++g) A.ck_r[offkr + g] = make_int2(k, g);
+        for (int g = 0; g < bc; ++g) A.ck_c[offkc + g] = make_int2(k, g);
+        if (k < count) A.wrange[k] = make_int4(offkr, br, offkc, bc);
         // The last field carries the LP, or for the sliced and staged
         // kernels the item's first slice.
         for (int g = 0; g < nr; ++g) {
@@ -572,12 +574,16 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
         if (tid == 1023) {
             carry_r += sr[1023];
             carry_c += sc[1023];
+            carry_kr += kr[1023];
+            carry_kc += kc[1023];
         }
         __syncthreads();
     }
     if (tid == 0) {
         wcnt[0] = carry_r;
         wcnt[1] = carry_c;
+        wcnt[2] = carry_kr;
+        wcnt[3] = carry_kc;
     }
 }
 
@@ -821,6 +827,145 @@ __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const Work
 
 // Phase 3: candidate -> FP64 KKT -> Control decision -> restart / refine /
 // finish.  Counts the LPs still active.
+// ---- round and check phases over the work items --------------------------
+//
+// The FP64 products of a refinement round's data and of the check's KKT
+// error run over blocks of kCheckRows rows of every LP with a chunk in
+// flight, so an LP's products are spread over the chip however few LPs
+// still run, and the check's sums do not depend on the iterate kernels.
+// Rows take lanes_for(mean row length) lanes each.
+
+enum ItemPass { kPassRound, kPassTotals, kPassKkt };
+
+// f(row in the item, lane, lanes as an integral constant) for the item's
+// rows, L lanes per row.
+template <int L, class F>
+__device__ __forceinline__ void item_rows(int cnt, const F& f) {
+    constexpr int G = kBlock / L;
+    const int g = static_cast<int>(threadIdx.x) / L, lane = static_cast<int>(threadIdx.x) & (L - 1);
+    for (int r0 = 0; r0 < cnt; r0 += G)
+        if (r0 + g < cnt) f(r0 + g, lane, std::integral_constant<int, L>{});
+}
+
+template <class F>
+__device__ __forceinline__ void item_rows_any(int L, int cnt, const F& f) {
+    switch (L) {
+        case 1: item_rows<1>(cnt, f); break;
+        case 2: item_rows<2>(cnt, f); break;
+        case 4: item_rows<4>(cnt, f); break;
+        case 8: item_rows<8>(cnt, f); break;
+        case 16: item_rows<16>(cnt, f); break;
+        default: item_rows<32>(cnt, f); break;
+    }
+}
+
+// One pass over the blocks of both lists (A rows first, then A^T rows):
+//  kPassRound:  for LPs starting a refinement round, the correction-LP data
+//               from FP64 A x_base and A^T y_base, iterates reset;
+//  kPassTotals: the total iterates x_tot, y_tot of the chunk's last
+//               iterate (base point plus the scaled correction);
+//  kPassKkt:    FP64 A x_tot and A^T y_tot with the relative-KKT terms,
+//               reduced per block into part_r / part_c (kPart doubles each).
+template <int PASS, class VT>
+__global__ __launch_bounds__(kBlock) void k_b_items(BatchArrays<VT> A, const BatchState* __restrict__ S) {
+    __shared__ double red[16];
+    const int tid = static_cast<int>(threadIdx.x);
+    const int nr = A.wcnt[2], total = nr + A.wcnt[3];
+    int w0, w1, step;
+    xcd_slice(total, w0, w1, step);
+    for (int w = w0; w < w1; w += step) {
+        const bool rows = w < nr;
+        const int wi = rows ? w : w - nr;
+        const int2 ck = rows ? A.ck_r[wi] : A.ck_c[wi];
+        const int lp = ck.x;
+        const BatchDesc& d = A.desc[lp];
+        const View<VT> v = view_of(A, d);
+        const int first = ck.y * kCheckRows;
+        const int cnt = min(kCheckRows, (rows ? d.m : d.n) - first);
+        if constexpr (PASS == kPassRound) {
+            if (!S[lp].round_start) continue;  // uniform over the workgroup
+            const double dp = S[lp].ctl.DP, dd = S[lp].ctl.DD;
+            if (rows) {
+                const EpiRoundRow epi{v.lr, v.ur, v.ybase, v.dr, dp, dd, v.lr32, v.ur32, v.g32};
+                item_rows_any(v.la, cnt, [&](int r, int lane, auto LC) {
+                    const int i = first + r;
+                    const double ax =
+                        row_dot<decltype(LC)::value, false, double>(v.rp[i], v.rp[i + 1], lane, v.ci, v.oa, v.xbase);
+                    if (lane == 0) {
+                        epi(i, ax);
+                        v.y[i] = v.y0[i] = v.yl[i] = 0.0f;
+                    }
+                });
+            } else {
+                const EpiRoundCol epi{v.cmin, v.lx, v.ux, v.xbase, v.dc, dp, dd, v.c32, v.lx32, v.ux32};
+                item_rows_any(v.lat, cnt, [&](int r, int lane, auto LC) {
+                    const int j = first + r;
+                    const double aty =
+                        row_dot<decltype(LC)::value, false, double>(v.cp[j], v.cp[j + 1], lane, v.ri, v.oat, v.ybase);
+                    if (lane == 0) {
+                        epi(j, aty);
+                        const float x0v = fmaxf(v.lx32[j], fminf(0.0f, v.ux32[j]));
+                        v.x[j] = v.x0[j] = v.xl[j] = x0v;
+                    }
+                });
+            }
+        } else if constexpr (PASS == kPassTotals) {
+            if (rows) {
+                const double invd = 1.0 / S[lp].ctl.DD;
+                for (int r = tid; r < cnt; r += kBlock) {
+                    const int i = first + r;
+                    double t = v.ybase[i] + v.dr[i] * static_cast<double>(v.yt[i]) * invd;
+                    if (!(v.ur[i] < kInf) && t > 0.0) t = 0.0;
+                    if (!(v.lr[i] > -kInf) && t < 0.0) t = 0.0;
+                    v.ytot[i] = t;
+                }
+            } else {
+                const double invp = 1.0 / S[lp].ctl.DP;
+                for (int r = tid; r < cnt; r += kBlock) {
+                    const int j = first + r;
+                    double t = v.xbase[j] + v.dc[j] * static_cast<double>(v.xt[j]) * invp;
+                    t = t < v.lx[j] ? v.lx[j] : (t > v.ux[j] ? v.ux[j] : t);
+                    v.xtot[j] = t;
+                }
+            }
+        } else {
+            if (rows) {
+                const EpiKktRow epi{v.lr, v.ur, v.ytot, v.dr};
+                double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                item_rows_any(v.la, cnt, [&](int r, int lane, auto LC) {
+                    const int i = first + r;
+                    const double ax =
+                        row_dot<decltype(LC)::value, false, double>(v.rp[i], v.rp[i + 1], lane, v.ci, v.oa, v.xtot);
+                    if (lane == 0) {
+                        double t[7];
+                        epi(i, ax, t);
+#pragma unroll
+                        for (int q = 0; q < 7; ++q) acc[q] = q < 6 ? acc[q] + t[q] : (t[q] > acc[q] ? t[q] : acc[q]);
+                    }
+                });
+                block_reduce_all<6, 1, kBlock>(acc, red);
+                if (tid < 7) A.part_r[static_cast<size_t>(wi) * kPart + tid] = red[tid];
+            } else {
+                const EpiKktCol epi{v.cmin, v.lx, v.ux, v.xtot, v.dc};
+                double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                item_rows_any(v.lat, cnt, [&](int r, int lane, auto LC) {
+                    const int j = first + r;
+                    const double aty =
+                        row_dot<decltype(LC)::value, false, double>(v.cp[j], v.cp[j + 1], lane, v.ri, v.oat, v.ytot);
+                    if (lane == 0) {
+                        double t[6];
+                        epi(j, aty, t);
+#pragma unroll
+                        for (int q = 0; q < 6; ++q) acc[q] = q < 5 ? acc[q] + t[q] : (t[q] > acc[q] ? t[q] : acc[q]);
+                    }
+                });
+                block_reduce_all<5, 1, kBlock>(acc, red);
+                if (tid < 6) A.part_c[static_cast<size_t>(wi) * kPart + tid] = red[tid];
+            }
+        }
+    }
+}
+
 template <class VT>
 __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* S, int* active_count) {
     const int lp = A.order[blockIdx.x];
@@ -835,22 +980,31 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
     if (tid == 0) ctl = st.ctl;
     __syncthreads();
     const int chunk = A.it[lp].chunk;
+    if (tid == 0) st.round_start = 0;  // the round pass of this chunk has run
     int act;
     if (chunk > 0) {
-        const double invp = 1.0 / ctl.DP, invd = 1.0 / ctl.DD;
-        for (int j = tid; j < n; j += kBB) {
-            double t = v.xbase[j] + v.dc[j] * static_cast<double>(v.xt[j]) * invp;
-            t = t < v.lx[j] ? v.lx[j] : (t > v.ux[j] ? v.ux[j] : t);
-            v.xtot[j] = t;
-        }
-        for (int i = tid; i < m; i += kBB) {
-            double t = v.ybase[i] + v.dr[i] * static_cast<double>(v.yt[i]) * invd;
-            if (!(v.ur[i] < kInf) && t > 0.0) t = 0.0;
-            if (!(v.lr[i] > -kInf) && t < 0.0) t = 0.0;
-            v.ytot[i] = t;
+        // The KKT terms of x_tot, y_tot, reduced per block by the check
+        // pass, summed over the LP's blocks in order.
+        const int4 wr = A.wrange[lp];
+        if (tid < 7) {
+            double a = 0.0;
+            for (int q = 0; q < wr.y; ++q) {
+                const double t = A.part_r[static_cast<size_t>(wr.x + q) * kPart + tid];
+                a = tid < 6 ? a + t : (t > a ? t : a);
+            }
+            keep[tid] = a;
+        } else if (tid >= 64 && tid < 70) {
+            const int j = tid - 64;
+            double a = 0.0;
+            for (int q = 0; q < wr.w; ++q) {
+                const double t = A.part_c[static_cast<size_t>(wr.z + q) * kPart + j];
+                a = j < 5 ? a + t : (t > a ? t : a);
+            }
+            red[j] = a;
         }
         __syncthreads();
-        const double rel = batch_kkt(v, d, red, keep);
+        double rel = 0.0;
+        if (tid == 0) rel = kkt_finish(keep, red, d);
         if (tid == 0) {
             ctl.advance(chunk);
             act_s = ctl.on_check(rel);
@@ -1148,7 +1302,17 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     }
     b->work_r.resize(std::max<size_t>(std::max(items_r, segs_r), 1));
     b->work_c.resize(std::max<size_t>(std::max(items_c, segs_c), 1));
-    b->wcnt.resize(2);
+    size_t blocks_r = 0, blocks_c = 0;
+    for (const BatchDesc& d : b->desc) {
+        blocks_r += (d.m + kCheckRows - 1) / kCheckRows;
+        blocks_c += (d.n + kCheckRows - 1) / kCheckRows;
+    }
+    b->ck_r.resize(std::max<size_t>(blocks_r, 1));
+    b->ck_c.resize(std::max<size_t>(blocks_c, 1));
+    b->part_r.resize(b->ck_r.size() * kPart);
+    b->part_c.resize(b->ck_c.size() * kPart);
+    b->wrange.resize(static_cast<size_t>(count));
+    b->wcnt.resize(4);
     const int fixed = variant_kernel(opts.spmv_variant);
     if (fixed != kAutoKernel) b->lanes_a = b->lanes_at = fixed;
     if (fixed == kStagedKernel && std::max(b->win_a, b->win_at) > kStagedMaxVec)
@@ -1193,6 +1357,8 @@ BatchArrays<VT> batch_arrays(pm_batch* b, const VT* oa, const VT* oat) {
     A.grp = b->grp.get();
     A.gcp = b->gcp.get();
     A.it = b->iter.get();
+    A.wrange = b->wrange.get(); A.ck_r = b->ck_r.get(); A.ck_c = b->ck_c.get();
+    A.part_r = b->part_r.get(); A.part_c = b->part_c.get(); A.wcnt = b->wcnt.get();
     A.sa_val = b->sl_a.val.get(); A.sa_idx = b->sl_a.idx.get(); A.sa_ptr = b->sl_a.ptr.get();
     A.sa_len = b->sl_a.len.get(); A.sa_pos = b->sl_a.pos.get();
     A.st_val = b->sl_at.val.get(); A.st_idx = b->sl_at.idx.get(); A.st_ptr = b->sl_at.ptr.get();
@@ -1248,7 +1414,9 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
     const dim3 g(b->count), blk(kBB);
     switch (phase) {
         case 0: hipLaunchKernelGGL(k_b_init<VT>, g, blk, 0, s, A, S); break;
-        case 1: hipLaunchKernelGGL(k_b_round<VT>, g, blk, 0, s, A, S); break;
+        case 1:
+            hipLaunchKernelGGL((k_b_items<kPassRound, VT>), dim3(b->iter_grid), dim3(kBlock), 0, s, A, S);
+            break;
         case 2:
             for (int i = 0; i < b->check_frequency; ++i) {
                 launch_iterate(b, A, PM_KERNEL_SPMV_AT, i, b->mode_at);
@@ -1261,7 +1429,13 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
                                b->mode_at > 0 ? kBlock / b->mode_at : b->mode_at,  // rows per item, or a code
                                b->seg_a.get(), b->seg_at.get(), b->work_r.get(), b->work_c.get(), b->wcnt.get());
             break;
-        case 3: hipLaunchKernelGGL(k_b_check<VT>, g, blk, 0, s, A, S, b->counter.get()); break;
+        case 3:
+            hipLaunchKernelGGL((k_b_items<kPassTotals, VT>), dim3(b->iter_grid), dim3(kBlock), 0, s, A, S);
+            PM_LAUNCHED();
+            hipLaunchKernelGGL((k_b_items<kPassKkt, VT>), dim3(b->iter_grid), dim3(kBlock), 0, s, A, S);
+            PM_LAUNCHED();
+            hipLaunchKernelGGL(k_b_check<VT>, g, blk, 0, s, A, S, b->counter.get());
+            break;
         default: fail(PM_ERR_INTERNAL, "unknown batch phase");
     }
     PM_LAUNCHED();

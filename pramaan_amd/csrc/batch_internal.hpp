@@ -113,6 +113,14 @@ struct pm_batch {
     pm::DevBuf<pm::BatchDesc> d_desc;
     pm::DevBuf<int> grp, gcp, wcnt;
     pm::DevBuf<pm::WorkItem> work_r, work_c;
+    // Blocks of the round and check passes: {LP, block} per block of
+    // kCheckRows rows of A (ck_r) and of A^T (ck_c) of every LP with a
+    // chunk in flight, the same whichever iterate kernel runs; per LP
+    // {first A block, A blocks, first A^T block, A^T blocks}; per-block
+    // partial sums of the check.
+    pm::DevBuf<int2> ck_r, ck_c;
+    pm::DevBuf<int4> wrange;
+    pm::DevBuf<double> part_r, part_c;
     pm::DevBuf<int2> seg_a, seg_at;  // {local first row, rows} per segment
     pm::SlicedCsr sl_a, sl_at;       // sliced copies of A and A^T
     int win_a = 0, win_at = 0;       // widest group window of A and A^T (staged kernel's LDS)
