@@ -240,6 +240,11 @@ int pm_batch_set_warm_start(pm_batch* b, int32_t i, const double* x0,
  * Ruiz / Pock-Chambolle scaled problem, as pm_pdhg_result.primal_weight
  * reports it); omega <= 0 restores the default ||c_hat||_2 / ||b_hat||_2. */
 int pm_batch_set_primal_weight(pm_batch* b, int32_t i, double omega);
+/* Row and column scaling factors of LP i (Ruiz inf-norm passes then
+ * Pock-Chambolle 1-norms, computed on the device for the whole batch; the
+ * reference scales on the host, src/presolve/scaling.cpp:29-150);
+ * row_scale[m], col_scale[n]. */
+int pm_batch_scaling(pm_batch* b, int32_t i, double* row_scale, double* col_scale);
 /* Solves every LP of the batch; results[i].x/.y filled when non-NULL.
  * A warm start is judged first: its FP64 relative KKT error is compared
  * with that of the cold start (x = 0 clipped to the bounds, y = 0), and

@@ -98,6 +98,7 @@ SIGNATURES = {
     "pm_batch_get_warm_start": (C.c_int, [C.c_void_p, C.c_int32, _d, _d]),
     "pm_batch_warm_start_verdicts": (C.c_int, [C.c_void_p, _i, _d]),
     "pm_batch_set_primal_weight": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "pm_batch_scaling": (C.c_int, [C.c_void_p, C.c_int32, _d, _d]),
     "pm_batch_spmv_variants": (C.c_int, [C.c_void_p, _i, _i]),
     "pm_batch_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _d, _d]),
     "pm_batch_solve": (C.c_int, [C.c_int32, C.POINTER(pm_lp), C.POINTER(_d), C.POINTER(_d),
@@ -410,6 +411,13 @@ class Batch:
 
     def set_primal_weight(self, i: int, omega: float) -> None:
         _check(lib().pm_batch_set_primal_weight(self._h, i, float(omega)))
+
+    def scaling(self, i: int) -> tuple[np.ndarray, np.ndarray]:
+        """Row and column scaling factors of LP i."""
+        lp = self.lps[i]
+        dr, dc = np.zeros(max(lp.m, 1)), np.zeros(lp.n)
+        _check(lib().pm_batch_scaling(self._h, i, _p(dr, C.c_double), _p(dc, C.c_double)))
+        return dr[:lp.m], dc
 
     def spmv_variants(self) -> tuple[int, int]:
         a, at = C.c_int32(), C.c_int32()

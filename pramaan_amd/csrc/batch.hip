@@ -41,6 +41,7 @@
 #include "control.hpp"
 #include "dispatch.hpp"
 #include "prep.hpp"
+#include "scaling.hpp"
 
 namespace pm {
 namespace {
@@ -98,16 +99,18 @@ inline int lanes_variant(int l) {
 // row order) inside each group, slices of kSliceRows sorted rows; each
 // row's entries stay in CSR order.
 struct SlicedHost {
-    std::vector<float> val;
-    std::vector<int> idx, len, pos;
+    std::vector<int> src;  // entry of the CSR value array each slot takes, -1 for padding
+    std::vector<uint16_t> idx;
+    std::vector<int> len, pos;
     std::vector<int64_t> ptr;
     std::vector<int2> win;
 };
 
 // Padding entries hold value 0 and the row's last column (the window's
 // first column for an empty row), so that they read an element the group
-// reads anyway.
-inline void build_sliced(int nrows, const int* rp, const int* ci, const float* v, SlicedHost& out) {
+// reads anyway.  The values are filled on the device from the scaled CSR
+// values: slot e takes value[src[e]] (global entry indexes from nz0).
+inline void build_sliced(int nrows, const int* rp, const int* ci, int64_t nz0, SlicedHost& out) {
     std::vector<int> order;
     for (int g0 = 0; g0 < nrows; g0 += kSliceGroup) {
         const int gn = std::min(kSliceGroup, nrows - g0);
@@ -129,15 +132,15 @@ inline void build_sliced(int nrows, const int* rp, const int* ci, const float* v
                 const int r = g0 + order[q * kSliceRows + l];
                 len = std::max(len, rp[r + 1] - rp[r]);
             }
-            out.ptr.push_back(static_cast<int64_t>(out.val.size()));
+            out.ptr.push_back(static_cast<int64_t>(out.src.size()));
             out.len.push_back(len);
             out.win.push_back(win);
             for (int l = 0; l < kSliceRows; ++l) {
                 const int p = q * kSliceRows + l;
                 out.pos.push_back(p < gn ? order[p] : -1);
             }
-            const size_t base = out.val.size();
-            out.val.resize(base + static_cast<size_t>(len) * kSliceRows, 0.0f);
+            const size_t base = out.src.size();
+            out.src.resize(base + static_cast<size_t>(len) * kSliceRows, -1);
             out.idx.resize(base + static_cast<size_t>(len) * kSliceRows, 0);
             for (int l = 0; l < kSliceRows; ++l) {
                 const int p = q * kSliceRows + l;
@@ -145,12 +148,12 @@ inline void build_sliced(int nrows, const int* rp, const int* ci, const float* v
                 const int r = g0 + order[p];
                 for (int k = rp[r]; k < rp[r + 1]; ++k) {
                     const size_t e = base + static_cast<size_t>(k - rp[r]) * kSliceRows + l;
-                    out.val[e] = v[k];
-                    out.idx[e] = ci[k] - lo;
+                    out.src[e] = static_cast<int>(nz0 + k);
+                    out.idx[e] = static_cast<uint16_t>(ci[k] - lo);  // exact when the window fits
                 }
                 const int last = rp[r + 1] > rp[r] ? ci[rp[r + 1] - 1] - lo : 0;
                 for (int j = rp[r + 1] - rp[r]; j < len; ++j)
-                    out.idx[base + static_cast<size_t>(j) * kSliceRows + l] = last;
+                    out.idx[base + static_cast<size_t>(j) * kSliceRows + l] = static_cast<uint16_t>(last);
             }
         }
     }
@@ -195,7 +198,8 @@ struct BatchArrays {
     const int* wcnt;
     // sliced A (sa_*) and A^T (st_*)
     const float *sa_val, *st_val;
-    const int *sa_idx, *sa_ptr, *sa_len, *sa_pos, *st_idx, *st_ptr, *st_len, *st_pos;
+    const uint16_t *sa_idx, *st_idx;
+    const int *sa_ptr, *sa_len, *sa_pos, *st_ptr, *st_len, *st_pos;
     const int2 *sa_win, *st_win;
     double tol;
     int max_iterations, check_frequency, max_refinements;
@@ -733,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_t_half(BatchArrays<VT> A, const Work
 // flight together.  Each row is summed in CSR order with FMA, as the
 // one-lane-per-row kernels sum it.
 template <int NA>
-__device__ __forceinline__ void slice_steps(const float* __restrict__ val, const int* __restrict__ idx,
+__device__ __forceinline__ void slice_steps(const float* __restrict__ val, const uint16_t* __restrict__ idx,
                                             const float* xv, const int (&base)[4], int j0, int j1,
                                             float (&s)[4]) {
 #pragma unroll 2
@@ -773,7 +777,7 @@ __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const Work
     const int t = static_cast<int>(threadIdx.x);
     const int lane = t & 63, wv = t >> 6;
     const float* val = PRIMAL ? A.st_val : A.sa_val;
-    const int* idx = PRIMAL ? A.st_idx : A.sa_idx;
+    const uint16_t* idx = PRIMAL ? A.st_idx : A.sa_idx;
     const int* ptr = PRIMAL ? A.st_ptr : A.sa_ptr;
     const int* len = PRIMAL ? A.st_len : A.sa_len;
     const int* pos = PRIMAL ? A.st_pos : A.sa_pos;
@@ -1068,6 +1072,19 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
+__global__ __launch_bounds__(kBlock) void k_f32_to_f64(int64_t n, const float* __restrict__ a,
+                                                       double* __restrict__ b) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) b[i] = static_cast<double>(a[i]);
+}
+
+// out[e] = v[src[e]], 0 where src[e] < 0.
+__global__ __launch_bounds__(kBlock) void k_take_f32(int64_t n, const int* __restrict__ src,
+                                                     const float* __restrict__ v, float* __restrict__ out) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (e < n) out[e] = src[e] >= 0 ? v[src[e]] : 0.0f;
+}
+
 }  // namespace
 }  // namespace pm
 
@@ -1136,18 +1153,20 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->nnz_total = nzo;
     const size_t R = std::max<int64_t>(ro, 1), Cn = std::max<int64_t>(co, 1), Z = std::max<int64_t>(nzo, 1);
     std::vector<int> h_rp(rpo), h_cp(cpo), h_ci(Z), h_ri(Z);
-    std::vector<float> h_va(Z), h_vat(Z), h_oa32, h_oat32;
+    std::vector<int> h_gci(Z), h_gri(Z);  // global column / row index of each entry
+    std::vector<float> h_oa32, h_oat32;
     std::vector<double> h_oa64, h_oat64;
     if (b->exact) { h_oa32.resize(Z); h_oat32.resize(Z); }
     else { h_oa64.resize(Z); h_oat64.resize(Z); }
-    std::vector<double> h_cmin(Cn), h_lx(Cn), h_ux(Cn), h_dc(Cn), h_lr(R), h_ur(R), h_dr(R);
+    std::vector<double> h_cmin(Cn), h_lx(Cn), h_ux(Cn), h_lr(R), h_ur(R);
     std::vector<int> h_grp(R + 1, 0), h_gcp(Cn + 1, 0);
     std::vector<std::vector<int2>> seg_a_lp(count), seg_at_lp(count);
     std::vector<SlicedHost> sl_a_lp(count), sl_at_lp(count);
     h_grp[ro] = static_cast<int>(nzo);
     h_gcp[co] = static_cast<int>(nzo);
     b->c_model.assign(Cn, 0.0);
-    // Per-LP preparation (transpose, Ruiz + PC scaling) on host threads.
+    // Per-LP preparation (transpose, layouts) on host threads; the scaling
+    // runs on the device afterwards.
     auto prep = [&](int k) {
         const pm_lp& L = lps[k];
         BatchDesc& d = b->desc[k];
@@ -1155,9 +1174,6 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         std::vector<int> cp, ri;
         std::vector<int64_t> perm;
         csr_transpose(m, n, L.row_ptr, L.col_idx, cp, ri, perm);
-        std::vector<double> dr, dc;
-        const std::vector<double> vs =
-            ruiz_pc_scale(m, n, L.row_ptr, L.col_idx, L.values, opts.ruiz_iterations, dr, dc);
         std::copy(L.row_ptr, L.row_ptr + m + 1, h_rp.begin() + d.rp_off);
         std::copy(cp.begin(), cp.end(), h_cp.begin() + d.cp_off);
         for (int i = 0; i < m; ++i) {
@@ -1169,8 +1185,8 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         for (int q = 0; q < nnz; ++q) {
             h_ci[d.nz_off + q] = L.col_idx[q];
             h_ri[d.nz_off + q] = ri[q];
-            h_va[d.nz_off + q] = static_cast<float>(vs[q]);
-            h_vat[d.nz_off + q] = static_cast<float>(vs[perm[q]]);
+            h_gci[d.nz_off + q] = static_cast<int>(d.col_off + L.col_idx[q]);
+            h_gri[d.nz_off + q] = static_cast<int>(d.row_off + ri[q]);
             if (b->exact) {
                 h_oa32[d.nz_off + q] = static_cast<float>(L.values[q]);
                 h_oat32[d.nz_off + q] = static_cast<float>(L.values[perm[q]]);
@@ -1185,19 +1201,17 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
             b->c_model[d.col_off + j] = L.c[j];
             h_lx[d.col_off + j] = L.var_lower[j];
             h_ux[d.col_off + j] = L.var_upper[j];
-            h_dc[d.col_off + j] = dc[j];
         }
         for (int i = 0; i < m; ++i) {
             h_lr[d.row_off + i] = L.row_lower[i];
             h_ur[d.row_off + i] = L.row_upper[i];
-            h_dr[d.row_off + i] = dr[i];
         }
         d.lanes_a = m > 0 ? lanes_for(static_cast<double>(nnz) / m) : 1;
         d.lanes_at = lanes_for(static_cast<double>(nnz) / n);
         stream_segments(m, L.row_ptr, seg_a_lp[k]);
         stream_segments(n, cp.data(), seg_at_lp[k]);
-        build_sliced(m, L.row_ptr, L.col_idx, h_va.data() + d.nz_off, sl_a_lp[k]);
-        build_sliced(n, cp.data(), ri.data(), h_vat.data() + d.nz_off, sl_at_lp[k]);
+        build_sliced(m, L.row_ptr, L.col_idx, d.nz_off, sl_a_lp[k]);
+        build_sliced(n, cp.data(), ri.data(), d.nz_off, sl_at_lp[k]);
     };
     {
         const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -1227,30 +1241,32 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         b->seg_at.upload(st, b->stream);
     }
     // Sliced copies, concatenated in LP order with global entry offsets.
-    auto concat = [&](std::vector<SlicedHost>& parts, bool a_side, SlicedCsr& dst) {
+    DevBuf<int> src_a, src_at;  // slot -> CSR entry, until the values are filled
+    auto concat = [&](std::vector<SlicedHost>& parts, bool a_side, SlicedCsr& dst, DevBuf<int>& src) {
         SlicedHost all;
         for (int k = 0; k < count; ++k) {
             SlicedHost& p = parts[k];
             (a_side ? b->desc[k].la_off : b->desc[k].lt_off) = static_cast<int32_t>(all.len.size());
-            const int64_t shift = static_cast<int64_t>(all.val.size());
+            const int64_t shift = static_cast<int64_t>(all.src.size());
             for (int64_t q : p.ptr) all.ptr.push_back(q + shift);
             all.len.insert(all.len.end(), p.len.begin(), p.len.end());
             all.pos.insert(all.pos.end(), p.pos.begin(), p.pos.end());
             all.win.insert(all.win.end(), p.win.begin(), p.win.end());
-            all.val.insert(all.val.end(), p.val.begin(), p.val.end());
+            all.src.insert(all.src.end(), p.src.begin(), p.src.end());
             all.idx.insert(all.idx.end(), p.idx.begin(), p.idx.end());
-            SlicedHost().val.swap(p.val);
+            SlicedHost().src.swap(p.src);
             SlicedHost().idx.swap(p.idx);
         }
-        if (all.val.size() >= static_cast<size_t>(INT32_MAX))
+        if (all.src.size() >= static_cast<size_t>(INT32_MAX))
             fail(PM_ERR_INVALID_ARG, "batch too large for the sliced layout");
         std::vector<int> ptr32(all.ptr.begin(), all.ptr.end());
         int& widest = a_side ? b->win_a : b->win_at;
         for (const int2& w : all.win) widest = std::max(widest, w.y);
         if (ptr32.empty()) { ptr32.push_back(0); all.len.push_back(0); all.win.push_back(make_int2(0, 0)); }
         if (all.pos.empty()) all.pos.assign(kSliceRows, -1);
-        if (all.val.empty()) { all.val.push_back(0.0f); all.idx.push_back(0); }
-        dst.val.upload(all.val, b->stream);
+        if (all.src.empty()) { all.src.push_back(-1); all.idx.push_back(0); }
+        src.upload(all.src, b->stream);
+        dst.val.resize(all.src.size());
         dst.idx.upload(all.idx, b->stream);
         dst.ptr.upload(ptr32, b->stream);
         dst.len.upload(all.len, b->stream);
@@ -1258,8 +1274,8 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         dst.win.upload(all.win, b->stream);
         PM_HIP(hipStreamSynchronize(b->stream));  // the host vectors die here
     };
-    concat(sl_a_lp, true, b->sl_a);
-    concat(sl_at_lp, false, b->sl_at);
+    concat(sl_a_lp, true, b->sl_a, src_a);
+    concat(sl_at_lp, false, b->sl_at, src_at);
     // Largest LP first.
     b->order.resize(count);
     std::iota(b->order.begin(), b->order.end(), 0);
@@ -1271,20 +1287,46 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->cp.upload(h_cp, s);
     b->ci.upload(h_ci, s);
     b->ri.upload(h_ri, s);
-    b->va.upload(h_va, s);
-    b->vat.upload(h_vat, s);
     if (b->exact) { b->oa32.upload(h_oa32, s); b->oat32.upload(h_oat32, s); }
     else { b->oa64.upload(h_oa64, s); b->oat64.upload(h_oat64, s); }
     b->cmin.upload(h_cmin, s);
     b->lx.upload(h_lx, s);
     b->ux.upload(h_ux, s);
-    b->dc.upload(h_dc, s);
     b->lr.upload(h_lr, s);
     b->ur.upload(h_ur, s);
-    b->dr.upload(h_dr, s);
     b->d_order.upload(b->order, s);
     b->grp.upload(h_grp, s);
     b->gcp.upload(h_gcp, s);
+    // Ruiz + Pock-Chambolle scaling of the block-diagonal concatenation on
+    // the device (each LP's factors are those of the LP alone), the FP32
+    // iterate values, and the values of the sliced layouts.
+    {
+        DevBuf<double> vs(Z), vst(Z), rn(R), cn(Cn);
+        DevBuf<int> gci, gri;
+        gci.upload(h_gci, s);
+        gri.upload(h_gri, s);
+        const dim3 gz = grid_for(static_cast<int64_t>(Z)), blk(kBlock);
+        if (b->exact) {
+            hipLaunchKernelGGL(k_f32_to_f64, gz, blk, 0, s, static_cast<int64_t>(Z), b->oa32.get(), vs.get());
+            hipLaunchKernelGGL(k_f32_to_f64, gz, blk, 0, s, static_cast<int64_t>(Z), b->oat32.get(), vst.get());
+        } else {
+            PM_HIP(hipMemcpyAsync(vs.get(), b->oa64.get(), sizeof(double) * Z, hipMemcpyDeviceToDevice, s));
+            PM_HIP(hipMemcpyAsync(vst.get(), b->oat64.get(), sizeof(double) * Z, hipMemcpyDeviceToDevice, s));
+        }
+        b->dr.resize(R);
+        b->dc.resize(Cn);
+        ruiz_pc_device(s, opts.ruiz_iterations, static_cast<int>(ro), static_cast<int>(co), b->grp.get(), gci.get(),
+                       b->gcp.get(), gri.get(), vs.get(), vst.get(), b->dr.get(), b->dc.get(), rn.get(), cn.get());
+        b->va.resize(Z);
+        b->vat.resize(Z);
+        hipLaunchKernelGGL(k_to_f32, gz, blk, 0, s, static_cast<int64_t>(Z), vs.get(), b->va.get());
+        hipLaunchKernelGGL(k_to_f32, gz, blk, 0, s, static_cast<int64_t>(Z), vst.get(), b->vat.get());
+        const int64_t na = static_cast<int64_t>(b->sl_a.val.size()), nat = static_cast<int64_t>(b->sl_at.val.size());
+        hipLaunchKernelGGL(k_take_f32, grid_for(na), blk, 0, s, na, src_a.get(), b->va.get(), b->sl_a.val.get());
+        hipLaunchKernelGGL(k_take_f32, grid_for(nat), blk, 0, s, nat, src_at.get(), b->vat.get(), b->sl_at.val.get());
+        PM_LAUNCHED();
+        PM_HIP(hipStreamSynchronize(s));
+    }
     b->iter.resize(static_cast<size_t>(count));
     // One lane count per product for the whole concatenation (its mean row
     // length), four chunks per graph unless that makes the graph large.
@@ -1318,6 +1360,9 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     if (fixed == kStagedKernel && std::max(b->win_a, b->win_at) > kStagedMaxVec)
         fail(PM_ERR_INVALID_ARG, "PM_SPMV_STAGED holds each group's column window in LDS: windows must stay within " +
                                      std::to_string(kStagedMaxVec) + " columns");
+    if (fixed == kSlicedKernel && std::max(b->win_a, b->win_at) > kSlicedMaxWin)
+        fail(PM_ERR_INVALID_ARG, "PM_SPMV_SLICED: a group's column window is wider than " +
+                                     std::to_string(kSlicedMaxWin) + " columns");
     b->chunks_per_graph = std::max(1, std::min(4, 512 / std::max(1, b->check_frequency)));
     for (auto* p : {&b->c32, &b->lx32, &b->ux32, &b->x, &b->xe, &b->x0, &b->xt, &b->xl}) p->resize(Cn);
     for (auto* p : {&b->lr32, &b->ur32, &b->g32, &b->y, &b->y0, &b->yt, &b->yl}) p->resize(R);
@@ -1606,11 +1651,12 @@ void batch_autotune(pm_batch* b) {
     for (int which : {PM_KERNEL_SPMV_AT, PM_KERNEL_SPMV_A}) {
         if (which == PM_KERNEL_SPMV_A && b->rows_total == 0) continue;
         int& lanes = which == PM_KERNEL_SPMV_A ? b->lanes_a : b->lanes_at;
-        const bool staged_fits = (which == PM_KERNEL_SPMV_A ? b->win_a : b->win_at) <= kStagedMaxVec;
+        const int widest = which == PM_KERNEL_SPMV_A ? b->win_a : b->win_at;
         double best = INFINITY;
         int best_l = lanes;
         for (int l : cand) {
-            if (l == kStagedKernel && !staged_fits) continue;
+            if ((l == kStagedKernel && widest > kStagedMaxVec) || (l == kSlicedKernel && widest > kSlicedMaxWin))
+                continue;
             lanes = l;
             set_mode(b, false);
             phase(b, 4);
@@ -1714,6 +1760,21 @@ int pm_batch_set_primal_weight(pm_batch* b, int32_t i, double omega) {
         if (!b || i < 0 || i >= b->count) pm::fail(PM_ERR_INVALID_ARG, "bad batch or index");
         if (!std::isfinite(omega)) pm::fail(PM_ERR_INVALID_ARG, "primal weight must be finite");
         b->desc[i].omega0 = omega;
+    });
+}
+
+int pm_batch_scaling(pm_batch* b, int32_t i, double* row_scale, double* col_scale) {
+    return pm::guarded([&] {
+        if (!b || i < 0 || i >= b->count) pm::fail(PM_ERR_INVALID_ARG, "bad batch or index");
+        if (!col_scale || (!row_scale && b->desc[i].m > 0)) pm::fail(PM_ERR_INVALID_ARG, "NULL output");
+        PM_HIP(hipSetDevice(b->device));
+        const pm::BatchDesc& d = b->desc[i];
+        if (d.m > 0)
+            PM_HIP(hipMemcpyAsync(row_scale, b->dr.get() + d.row_off, sizeof(double) * d.m, hipMemcpyDeviceToHost,
+                                  b->stream));
+        PM_HIP(hipMemcpyAsync(col_scale, b->dc.get() + d.col_off, sizeof(double) * d.n, hipMemcpyDeviceToHost,
+                              b->stream));
+        PM_HIP(hipStreamSynchronize(b->stream));
     });
 }
 

@@ -56,17 +56,20 @@ constexpr int kStreamKernel = 0, kSlicedKernel = -1, kStagedKernel = -2;
 // work item; inside a group the rows are sorted by length and cut into
 // slices of kSliceRows (one row per lane of a wavefront).
 constexpr int kSliceRows = 64, kSliceGroup = 1024;
-// Largest column window (floats) the staged kernel holds in LDS.
-constexpr int kStagedMaxVec = 12 * 1024;
+// Largest column window (floats) the staged kernel holds in LDS, and the
+// largest any sliced kernel takes (its indexes are 16-bit).
+constexpr int kStagedMaxVec = 12 * 1024, kSlicedMaxWin = 65536;
 
 // A CSR in sliced ELL form.  Slice s holds len[s] columns of 64 lanes:
 // entry (j, lane) at ptr[s] + j * 64 + lane; pos[s * 64 + lane] is the
 // lane's row as an offset inside its group (-1 for a padding lane).  The
-// column indexes are relative to the group's window win[s] = {first
-// column, columns} (every column the group's rows hold lies inside it).
+// column indexes are 16-bit, relative to the group's window win[s] =
+// {first column, columns} (every column the group's rows hold lies inside
+// it; a layout with a window wider than kSlicedMaxWin is not used).
 struct SlicedCsr {
     DevBuf<float> val;
-    DevBuf<int> idx, ptr, len, pos;
+    DevBuf<uint16_t> idx;
+    DevBuf<int> ptr, len, pos;
     DevBuf<int2> win;
 };
 

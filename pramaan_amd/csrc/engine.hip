@@ -29,6 +29,7 @@
 
 #include "control.hpp"
 #include "dispatch.hpp"
+#include "scaling.hpp"
 
 namespace pm {
 
@@ -286,65 +287,17 @@ int autotune(pm_ctx* c, int nrows, int ncols, int nnz, const int* rp, const int*
 // ---------------------------------------------------------------------------
 // Load
 // ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
-// Ruiz (infinity-norm passes) + Pock-Chambolle (alpha = 1) scaling of A on
-// the device.  A is kept in both orders (CSR values v, CSC values vt) and
-// both are scaled every pass, so row and column norms are row reductions in
-// one order or the other; every norm is a sequential loop in CSR (rows) or
-// CSC (columns, rows ascending) order, and every update is the host's
-// expression, so the factors equal those of ruiz_pc_scale (prep.hpp).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_row_norm(int nrows, const int* __restrict__ rp,
-                                                     const double* __restrict__ v, bool sum,
-                                                     double* __restrict__ out) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= nrows) return;
-    double a = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) a = sum ? a + fabs(v[k]) : fmax(a, fabs(v[k]));
-    out[i] = a > 0.0 ? 1.0 / sqrt(a) : 1.0;
-}
-
-// v[k] *= rs[row] * cs[col] over one CSR order; acc[row] *= rs[row].
-__global__ __launch_bounds__(kBlock) void k_row_scale(int nrows, const int* __restrict__ rp,
-                                                      const int* __restrict__ col, double* __restrict__ v,
-                                                      const double* __restrict__ rs, const double* __restrict__ cs,
-                                                      double* __restrict__ acc) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= nrows) return;
-    const double r = rs[i];
-    for (int k = rp[i]; k < rp[i + 1]; ++k) v[k] *= r * cs[col[k]];
-    if (acc) acc[i] *= r;
-}
-
-__global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (i < n) b[i] = static_cast<float>(a[i]);
-}
-
 // Scales the LP whose original values are in vs (CSR) and vst (CSC), both
 // overwritten with the scaled values; dr, dc receive the factors.
 void device_ruiz_pc(pm_ctx* c, int passes, DevBuf<double>& vs, DevBuf<double>& vst, DevBuf<double>& dr,
                     DevBuf<double>& dc) {
-    hipStream_t s = c->stream;
     const int m = c->m, n = c->n;
     DevBuf<double> rn(std::max(m, 1)), cn(std::max(n, 1));
     dr.resize(std::max(m, 1));
     dc.resize(n);
-    launch_elem(m, FnFill<double>{dr.get(), 1.0}, s);
-    launch_elem(n, FnFill<double>{dc.get(), 1.0}, s);
-    const dim3 gm((m + kBlock - 1) / kBlock), gn((n + kBlock - 1) / kBlock), b(kBlock);
-    for (int pass = 0; pass <= passes; ++pass) {
-        const bool sum = pass == passes;  // the last pass is Pock-Chambolle
-        if (m > 0) hipLaunchKernelGGL(k_row_norm, gm, b, 0, s, m, c->rp.get(), vs.get(), sum, rn.get());
-        hipLaunchKernelGGL(k_row_norm, gn, b, 0, s, n, c->cp.get(), vst.get(), sum, cn.get());
-        if (m > 0) {
-            hipLaunchKernelGGL(k_row_scale, gm, b, 0, s, m, c->rp.get(), c->ci.get(), vs.get(), rn.get(), cn.get(),
-                               dr.get());
-            hipLaunchKernelGGL(k_row_scale, gn, b, 0, s, n, c->cp.get(), c->ri.get(), vst.get(), cn.get(), rn.get(),
-                               dc.get());
-        }
-        PM_LAUNCHED();
-    }
+    ruiz_pc_device(c->stream, passes, m, n, c->rp.get(), c->ci.get(), c->cp.get(), c->ri.get(), vs.get(), vst.get(),
+                   dr.get(), dc.get(), rn.get(), cn.get());
+    PM_LAUNCHED();
 }
 
 void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {

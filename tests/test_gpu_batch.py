@@ -1,11 +1,15 @@
-"""Batched solver (pm_batch_*): one launch, one workgroup per LP.
+"""Batched solver (pm_batch_*): many LPs solved together in one graph-captured run.
 
 Every instance is checked against its own closed-form optimum and against
 the oracle's FP64 relative KKT of the returned point; the batch must agree
 with the single-LP engine, which runs the same decisions (pm::Control).
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
 
 pytestmark = pytest.mark.gpu
 
@@ -76,6 +80,25 @@ def test_batch_sliced_kernels_match_one_lane_per_row(capi):
             assert a.status == c.status == "OPTIMAL"
             assert a.iterations == c.iterations and a.objective == c.objective, variant
             assert np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y), variant
+
+
+def test_batch_device_scaling_matches_restatement(capi, ruiz_restated):
+    """The batch scales its block-diagonal concatenation on the device; each
+    LP's factors equal the plain-loop restatement of that LP alone, bit for
+    bit (an LP without rows, an LP with an empty row and column, mixed
+    sizes)."""
+    from pramaan_amd.mps import read_mps
+    lps = [capi.generate(None, seed=31, m=120, n=300, eq_frac=0.6, noise=0.05),
+           read_mps(GOLDEN / "afiro.mps"),
+           capi.LP.from_dense(np.zeros((0, 3)), [1.0, -1.0, 2.0], [], [], np.zeros(3), np.ones(3)),
+           capi.LP.from_dense([[2.0, 0.0, -4.0], [0.0, 0.0, 0.0], [0.5, 0.0, 8.0]], [1.0, 0.0, 1.0],
+                              [1.0, -1.0, 0.0], [4.0, 1.0, 9.0], np.zeros(3), np.full(3, 5.0)),
+           capi.generate(None, seed=32, m=700, n=1750, k_min=4, k_max=16)]
+    with capi.Batch(lps, tolerance=1e-6, check_frequency=64) as b:
+        for i, lp in enumerate(lps):
+            dr, dc = b.scaling(i)
+            wr, wc = ruiz_restated(lp)
+            assert np.array_equal(dr, wr) and np.array_equal(dc, wc), i
 
 
 def test_batch_one_lp_and_mixed_sizes(capi):

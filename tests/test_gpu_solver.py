@@ -141,37 +141,8 @@ def test_pdlp_warm_start_from_optimum(capi):
     assert warm.iterations <= cold.iterations
 
 
-def ruiz_pc_restated(lp, passes=10):
-    """Ruiz infinity-norm passes on A, then Pock-Chambolle 1-norms, as plain
-    loops in CSR order (rows) and in row order per column -- the order the
-    engine's host and device scalings both follow."""
-    m, n = lp.m, lp.n
-    rp, ci = lp.row_ptr.tolist(), lp.col_idx.tolist()
-    v = lp.values.astype(float).tolist()
-    dr, dc = [1.0] * m, [1.0] * n
-    for p in range(passes + 1):
-        rn, cn = [0.0] * m, [0.0] * n
-        for i in range(m):
-            for k in range(rp[i], rp[i + 1]):
-                a = abs(v[k])
-                if p < passes:
-                    rn[i] = max(rn[i], a)
-                    cn[ci[k]] = max(cn[ci[k]], a)
-                else:
-                    rn[i] += a
-                    cn[ci[k]] += a
-        rn = [1.0 / np.sqrt(a) if a > 0 else 1.0 for a in rn]
-        cn = [1.0 / np.sqrt(a) if a > 0 else 1.0 for a in cn]
-        for i in range(m):
-            for k in range(rp[i], rp[i + 1]):
-                v[k] *= rn[i] * cn[ci[k]]
-        dr = [d * r for d, r in zip(dr, rn)]
-        dc = [d * c for d, c in zip(dc, cn)]
-    return np.array(dr), np.array(dc)
-
-
 @pytest.mark.parametrize("which", ["generated", "afiro"])
-def test_device_scaling_matches_restatement(capi, which):
+def test_device_scaling_matches_restatement(capi, which, ruiz_restated):
     """The device Ruiz + Pock-Chambolle factors equal a plain-loop
     restatement bit for bit (row and column norms in the same order)."""
     if which == "afiro":
@@ -183,5 +154,5 @@ def test_device_scaling_matches_restatement(capi, which):
     ctx.load(lp, algorithm="pdlp", termination="relative_kkt", tolerance=1e-6)
     dr, dc = ctx.scaling()
     ctx.close()
-    wr, wc = ruiz_pc_restated(lp)
+    wr, wc = ruiz_restated(lp)
     assert np.array_equal(dr, wr) and np.array_equal(dc, wc), (np.abs(dr - wr).max(), np.abs(dc - wc).max())
