@@ -161,6 +161,7 @@ def bench_rolling(args, P) -> dict:
     dom = max(kern, key=lambda k: kern[k][0])
     ms, by = kern[dom]
     achieved = by / (ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom)
     b.close()
 
     out = {
@@ -181,8 +182,11 @@ def bench_rolling(args, P) -> dict:
         "warm_starts_kept": int((acc == 1).sum()), "warm_starts_replaced_by_cold": int((acc == 0).sum()),
         "objective_rel_error_vs_highs": obj_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "kernel_us": ms * 1e3, "alg_bytes_per_launch": by,
+                     "achieved_definition": "CSR bytes of the product and its fused update per launch / time",
+                     "traffic_GBps": None if traffic is None else traffic / (ms * 1e-3) / 1e9,
+                     "traffic_frac": None if traffic is None else traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic_source": "profiles/batch_iterate_rolling500_traffic.md (rocprofv3 FETCH_SIZE + "
                                        "WRITE_SIZE, calibrated on a 256 MiB copy)"},
         "batch_kernels": {k: {"us": v[0] * 1e3, "alg_bytes": v[1], "GBps": v[1] / v[0] / 1e6}
