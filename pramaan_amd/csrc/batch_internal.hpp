@@ -58,7 +58,7 @@ constexpr int kStreamKernel = 0, kSlicedKernel = -1, kStagedKernel = -2;
 constexpr int kSliceRows = 64, kSliceGroup = 1024;
 // Largest column window (floats) the staged kernel holds in LDS, and the
 // largest any sliced kernel takes (its indexes are 16-bit).
-constexpr int kStagedMaxVec = 12 * 1024, kSlicedMaxWin = 65536;
+constexpr int kStagedMaxVec = 15 * 1024, kSlicedMaxWin = 65536;
 
 // A CSR in sliced ELL form.  Slice s holds len[s] columns of 64 lanes:
 // entry (j, lane) at ptr[s] + j * 64 + lane; pos[s * 64 + lane] is the
