@@ -1103,6 +1103,9 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     pm_pdhg_options_default(&opts);
     if (o) opts = *o;
     if (opts.check_frequency <= 0) fail(PM_ERR_INVALID_ARG, "check_frequency must be positive");
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && ncu > 0)
+        b->num_cu = ncu;
     b->tol = opts.tolerance;
     b->max_iterations = opts.max_iterations;
     b->check_frequency = opts.check_frequency;
@@ -1416,36 +1419,56 @@ BatchArrays<VT> batch_arrays(pm_batch* b, const VT* oa, const VT* oat) {
     return A;
 }
 
+// Workgroups (of kBlock threads, `lds` dynamic LDS bytes) of a persistent
+// launch of `kernel`: as many as the device holds at once, a multiple of 8
+// (xcd_slice).  Work items are assigned statically, so the workgroups of a
+// larger grid that do not fit start only after others have run all of
+// theirs: with items of even cost that costs time, with uneven ones it
+// balances the load (the autotuner chooses, see mult_a / mult_at).
+template <class K>
+int resident_grid(pm_batch* b, K kernel, size_t lds) {
+    const std::pair<const void*, size_t> key(reinterpret_cast<const void*>(kernel), lds);
+    const auto it = b->grid_cache.find(key);
+    if (it != b->grid_cache.end()) return it->second;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds) != hipSuccess || per_cu <= 0)
+        per_cu = b->iter_grid / std::max(1, b->num_cu);
+    const int g = std::max(8, per_cu * b->num_cu / 8 * 8);
+    b->grid_cache.emplace(key, g);
+    return g;
+}
+
+template <class K, class... Args>
+void launch_resident(pm_batch* b, int mult, K kernel, size_t lds, Args... args) {
+    const int g = resident_grid(b, kernel, lds) * std::max(1, mult);
+    hipLaunchKernelGGL(kernel, dim3(g), dim3(kBlock), lds, b->stream, args...);
+}
+
 // One iterate launch: half `which` (PM_KERNEL_SPMV_AT primal, PM_KERNEL_SPMV_A
-// dual) of iteration i of the chunk, with the kernel for `lanes` (0 =
-// stream).  The work items partition the rows exactly only for a grid that
-// is a multiple of 8 (xcd_slice).
+// dual) of iteration i of the chunk, with the kernel for `lanes` (a lanes
+// count or a kernel code), on `mult` resident grids.
 template <class VT>
-void launch_iterate(pm_batch* b, const BatchArrays<VT>& A, int which, int i, int lanes) {
-    if (b->iter_grid % 8 != 0) fail(PM_ERR_INTERNAL, "iterate grid must be a multiple of 8");
-    hipStream_t s = b->stream;
-    const dim3 gi(b->iter_grid), bi(kBlock);
+void launch_iterate(pm_batch* b, const BatchArrays<VT>& A, int which, int i, int lanes, int mult) {
     const bool primal = which == PM_KERNEL_SPMV_AT;
     if (!primal && b->rows_total == 0) return;
     WorkItem* work = primal ? b->work_c.get() : b->work_r.get();
+    int* wc = b->wcnt.get();
     if (lanes == kStreamKernel) {
-        if (primal) hipLaunchKernelGGL((k_t_half<true, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
-        else hipLaunchKernelGGL((k_t_half<false, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+        if (primal) launch_resident(b, mult, k_t_half<true, VT>, 0, A, work, wc, i);
+        else launch_resident(b, mult, k_t_half<false, VT>, 0, A, work, wc, i);
     } else if (lanes == kSlicedKernel) {
-        if (primal) hipLaunchKernelGGL((k_e_half<true, false, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
-        else hipLaunchKernelGGL((k_e_half<false, false, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+        if (primal) launch_resident(b, mult, k_e_half<true, false, VT>, 0, A, work, wc, i);
+        else launch_resident(b, mult, k_e_half<false, false, VT>, 0, A, work, wc, i);
     } else if (lanes == kStagedKernel) {
         const int v = primal ? b->win_at : b->win_a;
         if (v > kStagedMaxVec) fail(PM_ERR_INVALID_ARG, "a column window is too wide for the staged iterate kernel");
         const size_t lds = sizeof(float) * static_cast<size_t>(std::max(v, 1));
-        if (primal) hipLaunchKernelGGL((k_e_half<true, true, VT>), gi, bi, lds, s, A, work, b->wcnt.get(), i);
-        else hipLaunchKernelGGL((k_e_half<false, true, VT>), gi, bi, lds, s, A, work, b->wcnt.get(), i);
+        if (primal) launch_resident(b, mult, k_e_half<true, true, VT>, lds, A, work, wc, i);
+        else launch_resident(b, mult, k_e_half<false, true, VT>, lds, A, work, wc, i);
     } else {
         with_lanes(lanes, [&](auto LC) {
-            if (primal)
-                hipLaunchKernelGGL((k_s_primal<decltype(LC)::value, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
-            else
-                hipLaunchKernelGGL((k_s_dual<decltype(LC)::value, VT>), gi, bi, 0, s, A, work, b->wcnt.get(), i);
+            if (primal) launch_resident(b, mult, k_s_primal<decltype(LC)::value, VT>, 0, A, work, wc, i);
+            else launch_resident(b, mult, k_s_dual<decltype(LC)::value, VT>, 0, A, work, wc, i);
         });
     }
     PM_LAUNCHED();
@@ -1460,12 +1483,12 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
     switch (phase) {
         case 0: hipLaunchKernelGGL(k_b_init<VT>, g, blk, 0, s, A, S); break;
         case 1:
-            hipLaunchKernelGGL((k_b_items<kPassRound, VT>), dim3(b->iter_grid), dim3(kBlock), 0, s, A, S);
+            launch_resident(b, 1, k_b_items<kPassRound, VT>, 0, A, S);
             break;
         case 2:
             for (int i = 0; i < b->check_frequency; ++i) {
-                launch_iterate(b, A, PM_KERNEL_SPMV_AT, i, b->mode_at);
-                launch_iterate(b, A, PM_KERNEL_SPMV_A, i, b->mode_a);
+                launch_iterate(b, A, PM_KERNEL_SPMV_AT, i, b->mode_at, b->mode_mult_at);
+                launch_iterate(b, A, PM_KERNEL_SPMV_A, i, b->mode_a, b->mode_mult_a);
             }
             return;
         case 4:
@@ -1475,9 +1498,9 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
                                b->seg_a.get(), b->seg_at.get(), b->work_r.get(), b->work_c.get(), b->wcnt.get());
             break;
         case 3:
-            hipLaunchKernelGGL((k_b_items<kPassTotals, VT>), dim3(b->iter_grid), dim3(kBlock), 0, s, A, S);
+            launch_resident(b, 1, k_b_items<kPassTotals, VT>, 0, A, S);
             PM_LAUNCHED();
-            hipLaunchKernelGGL((k_b_items<kPassKkt, VT>), dim3(b->iter_grid), dim3(kBlock), 0, s, A, S);
+            launch_resident(b, 1, k_b_items<kPassKkt, VT>, 0, A, S);
             PM_LAUNCHED();
             hipLaunchKernelGGL(k_b_check<VT>, g, blk, 0, s, A, S, b->counter.get());
             break;
@@ -1495,6 +1518,8 @@ void phase(pm_batch* b, int p) {
 void set_mode(pm_batch* b, bool tail) {
     b->mode_a = tail ? b->tail_a : b->lanes_a;
     b->mode_at = tail ? b->tail_at : b->lanes_at;
+    b->mode_mult_a = tail ? 1 : b->mult_a;
+    b->mode_mult_at = tail ? 1 : b->mult_at;
 }
 
 // One chunk: clear the active counter, round data, K iterations, check.
@@ -1609,7 +1634,10 @@ void batch_time_kernel(pm_batch* b, int which, int reps, double* avg_ms, double*
     set_mode(b, false);
     phase(b, 0);  // every LP active, chunk = check_frequency
     phase(b, 4);
-    auto launch = [&](const auto& A) { launch_iterate(b, A, which, 0, which == PM_KERNEL_SPMV_A ? b->lanes_a : b->lanes_at); };
+    const bool dual = which == PM_KERNEL_SPMV_A;
+    auto launch = [&](const auto& A) {
+        launch_iterate(b, A, which, 0, dual ? b->lanes_a : b->lanes_at, dual ? b->mult_a : b->mult_at);
+    };
     auto one = [&] {
         if (b->exact) launch(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
         else launch(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
@@ -1632,9 +1660,9 @@ void batch_time_kernel(pm_batch* b, int which, int reps, double* avg_ms, double*
 }
 
 // Kernel of each half of the iteration (staged, sliced, stream, or 1..32
-// lanes per row):
-// each candidate timed over the whole batch with every LP running
-// (iterates zeroed, small steps), fastest kept.  Runs once at creation.
+// lanes per row; the sliced ones on one or four resident grids): each
+// candidate timed over the whole batch with every LP running (iterates
+// zeroed, small steps), fastest kept.  Runs once at creation.
 void batch_autotune(pm_batch* b) {
     hipStream_t s = b->stream;
     for (auto* p : {&b->x, &b->xe, &b->x0, &b->xt, &b->xl, &b->c32, &b->lx32, &b->ux32})
@@ -1651,32 +1679,39 @@ void batch_autotune(pm_batch* b) {
     for (int which : {PM_KERNEL_SPMV_AT, PM_KERNEL_SPMV_A}) {
         if (which == PM_KERNEL_SPMV_A && b->rows_total == 0) continue;
         int& lanes = which == PM_KERNEL_SPMV_A ? b->lanes_a : b->lanes_at;
+        int& mult = which == PM_KERNEL_SPMV_A ? b->mult_a : b->mult_at;
         const int widest = which == PM_KERNEL_SPMV_A ? b->win_a : b->win_at;
         double best = INFINITY;
-        int best_l = lanes;
+        int best_l = lanes, best_g = 1;
         for (int l : cand) {
             if ((l == kStagedKernel && widest > kStagedMaxVec) || (l == kSlicedKernel && widest > kSlicedMaxWin))
                 continue;
-            lanes = l;
-            set_mode(b, false);
-            phase(b, 4);
-            auto launch = [&](const auto& A) { launch_iterate(b, A, which, 0, l); };
-            auto one = [&] {
-                if (b->exact) launch(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
-                else launch(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
-            };
-            one();
-            PM_HIP(hipEventRecord(b->ev0, s));
-            for (int r = 0; r < 3; ++r) one();
-            PM_HIP(hipEventRecord(b->ev1, s));
-            PM_HIP(hipEventSynchronize(b->ev1));
-            float ms = 0.0f;
-            PM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            if (ms < best) {
-                best = ms;
-                best_l = l;
+            for (int g : {1, 4}) {
+                if (g > 1 && l != kStagedKernel && l != kSlicedKernel) continue;
+                lanes = l;
+                mult = g;
+                set_mode(b, false);
+                phase(b, 4);
+                auto launch = [&](const auto& A) { launch_iterate(b, A, which, 0, l, g); };
+                auto one = [&] {
+                    if (b->exact) launch(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
+                    else launch(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
+                };
+                one();
+                PM_HIP(hipEventRecord(b->ev0, s));
+                for (int r = 0; r < 3; ++r) one();
+                PM_HIP(hipEventRecord(b->ev1, s));
+                PM_HIP(hipEventSynchronize(b->ev1));
+                float ms = 0.0f;
+                PM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                if (ms < best) {
+                    best = ms;
+                    best_l = l;
+                    best_g = g;
+                }
             }
         }
+        mult = best_g;
         lanes = best_l;
     }
     set_mode(b, false);

@@ -3,6 +3,8 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -135,7 +137,15 @@ struct pm_batch {
     int tail_a = 8, tail_at = 8;
     int64_t tail_nnz = 0;
     int mode_a = 8, mode_at = 8;    // kernels the launches being issued use
-    int iter_grid = 2048;           // workgroups of one iterate launch: 8 per CU, a multiple of 8
+    int iter_grid = 2048;           // workgroups of a launch when occupancy is unknown (8 per CU)
+    int num_cu = 256;
+    // Workgroups of a persistent launch per (kernel, dynamic LDS bytes):
+    // as many as the device holds at once.
+    std::map<std::pair<const void*, size_t>, int> grid_cache;
+    // Grid of the iterate kernels in resident grids (1, or 4 when the
+    // autotuner finds it faster: items of uneven cost then balance as the
+    // hardware starts the queued workgroups); the tail kernels use 1.
+    int mult_a = 1, mult_at = 1, mode_mult_a = 1, mode_mult_at = 1;
     int chunks_per_graph = 4;
     bool use_graph = true;
     pm::DevBuf<char> state;
