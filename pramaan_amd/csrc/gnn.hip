@@ -156,12 +156,16 @@ __global__ __launch_bounds__(256) void k_sum_partials(int nb, int width, const f
     out[e] = s;
 }
 
-// Out[i][:] (+)= sum_k val[k] H[idx[k]][:], one wavefront per row.
+// Out[i][:] (+)= sum_k val[k] H[idx[k]][:], one wavefront per row.  The
+// grid is 8 x per blocks; block b runs on XCD b % 8, which takes the
+// (b % 8)-th eighth of the rows in order, so the rows of a window (and the
+// H rows they gather) stay in one XCD's L2.
 template <bool ACC>
-__global__ __launch_bounds__(256) void k_spmm(int nrows, const int* __restrict__ rp, const int* __restrict__ idx,
-                                              const float* __restrict__ val, const float* __restrict__ Hin,
-                                              float* __restrict__ Out) {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), f = threadIdx.x & 63;
+__global__ __launch_bounds__(256) void k_spmm(int nrows, int per, const int* __restrict__ rp,
+                                              const int* __restrict__ idx, const float* __restrict__ val,
+                                              const float* __restrict__ Hin, float* __restrict__ Out) {
+    const int blk = (static_cast<int>(blockIdx.x) & 7) * per + (static_cast<int>(blockIdx.x) >> 3);
+    const int row = blk * 4 + (threadIdx.x >> 6), f = threadIdx.x & 63;
     if (row >= nrows) return;
     float s = 0.0f;
     for (int k = rp[row]; k < rp[row + 1]; ++k) s = __builtin_fmaf(val[k], Hin[static_cast<size_t>(idx[k]) * kH + f], s);
@@ -366,8 +370,10 @@ void dense(pm_gnn* g, int N, const float* A1, const short* B1, const float* A2, 
 void spmm(pm_gnn* g, int nrows, const int* rp, const int* idx, const float* val, const float* Hin, float* out,
           bool acc) {
     if (nrows <= 0) return;
-    if (acc) hipLaunchKernelGGL(k_spmm<true>, dim3(cdiv(nrows, 4)), dim3(256), 0, g->s, nrows, rp, idx, val, Hin, out);
-    else hipLaunchKernelGGL(k_spmm<false>, dim3(cdiv(nrows, 4)), dim3(256), 0, g->s, nrows, rp, idx, val, Hin, out);
+    const int per = cdiv(cdiv(nrows, 4), 8);
+    const dim3 grid(8 * per);
+    if (acc) hipLaunchKernelGGL(k_spmm<true>, grid, dim3(256), 0, g->s, nrows, per, rp, idx, val, Hin, out);
+    else hipLaunchKernelGGL(k_spmm<false>, grid, dim3(256), 0, g->s, nrows, per, rp, idx, val, Hin, out);
     PM_LAUNCHED();
 }
 
