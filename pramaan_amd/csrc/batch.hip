@@ -631,12 +631,13 @@ __global__ __launch_bounds__(kBlock) void k_s_primal(BatchArrays<VT> A, const Wo
         const int chunk = cur.a.w;
         if (g < cur.a.y && i < chunk) {
             const int j = cur.a.x + g;
+            const bool keep = i == chunk - 1;
+            const EpiPrimalHal epi{__int_as_float(cur.b.z), Control::lambda(cur.a.z + i), 1.0f, A.c32, A.lx32,
+                                   A.ux32, A.x0, A.x, A.xe, keep ? A.xt : nullptr};
+            EpiPrimalHal::Pre pre{};
+            if (lane == 0) pre = epi.load(j);  // in flight with the product's loads
             const float aty = row_dot<L, false, float>(A.gcp[j], A.gcp[j + 1], lane, A.ri, A.vat, A.y + cur.b.y);
-            if (lane == 0) {
-                const bool keep = i == chunk - 1;
-                EpiPrimalHal{__int_as_float(cur.b.z), Control::lambda(cur.a.z + i), 1.0f, A.c32, A.lx32, A.ux32,
-                             A.x0, A.x, A.xe, keep ? A.xt : nullptr}(j, aty);
-            }
+            if (lane == 0) epi.apply(j, aty, pre);
         }
         cur = nxt;
     }
@@ -656,12 +657,13 @@ __global__ __launch_bounds__(kBlock) void k_s_dual(BatchArrays<VT> A, const Work
         const int chunk = cur.a.w;
         if (g < cur.a.y && i < chunk) {
             const int r = cur.a.x + g;
+            const bool keep = i == chunk - 1;
+            const EpiDualHal epi{__int_as_float(cur.b.z), Control::lambda(cur.a.z + i), 1.0f, A.lr32, A.ur32,
+                                 cur.b.x ? A.g32 : nullptr, A.y0, A.y, keep ? A.yt : nullptr};
+            EpiDualHal::Pre pre{};
+            if (lane == 0) pre = epi.load(r);  // in flight with the product's loads
             const float ax = row_dot<L, false, float>(A.grp[r], A.grp[r + 1], lane, A.ci, A.va, A.xe + cur.b.y);
-            if (lane == 0) {
-                const bool keep = i == chunk - 1;
-                EpiDualHal{__int_as_float(cur.b.z), Control::lambda(cur.a.z + i), 1.0f, A.lr32, A.ur32,
-                           cur.b.x ? A.g32 : nullptr, A.y0, A.y, keep ? A.yt : nullptr}(r, ax);
-            }
+            if (lane == 0) epi.apply(r, ax, pre);
         }
         cur = nxt;
     }

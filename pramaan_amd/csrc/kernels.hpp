@@ -256,13 +256,18 @@ struct EpiPrimalHal {
     float* x;
     float* xe;
     float* xt_out;
-    __device__ void operator()(int j, float aty) const {
-        const float xk = x[j];
-        const float xt = fminf(fmaxf(xk - tau * (c[j] + aty), lo[j]), hi[j]);
-        xe[j] = 2.0f * xt - xk;
-        x[j] = lam * ((1.0f + rho) * xt - rho * xk) + (1.0f - lam) * x0[j];
+    // The operands of column j, loadable before its product is known.
+    struct Pre {
+        float xk, c, lo, hi, x0;
+    };
+    __device__ Pre load(int j) const { return Pre{x[j], c[j], lo[j], hi[j], x0[j]}; }
+    __device__ void apply(int j, float aty, const Pre& p) const {
+        const float xt = fminf(fmaxf(p.xk - tau * (p.c + aty), p.lo), p.hi);
+        xe[j] = 2.0f * xt - p.xk;
+        x[j] = lam * ((1.0f + rho) * xt - rho * p.xk) + (1.0f - lam) * p.x0;
         if (xt_out) xt_out[j] = xt;
     }
+    __device__ void operator()(int j, float aty) const { apply(j, aty, load(j)); }
 };
 
 // Reflected Halpern PDHG, dual half (fused into A xe).  The row prox is
@@ -278,14 +283,19 @@ struct EpiDualHal {
     const float* y0;
     float* y;
     float* yt_out;
-    __device__ void operator()(int i, float ax) const {
-        const float yk = y[i];
-        const float w = yk / sigma + ax;
-        const float sh = g ? w - g[i] / sigma : w;
-        const float yt = sigma * (w - fminf(fmaxf(sh, lo[i]), hi[i]));
-        y[i] = lam * ((1.0f + rho) * yt - rho * yk) + (1.0f - lam) * y0[i];
+    // The operands of row i, loadable before its product is known.
+    struct Pre {
+        float yk, lo, hi, g, y0;
+    };
+    __device__ Pre load(int i) const { return Pre{y[i], lo[i], hi[i], g ? g[i] : 0.0f, y0[i]}; }
+    __device__ void apply(int i, float ax, const Pre& p) const {
+        const float w = p.yk / sigma + ax;
+        const float sh = g ? w - p.g / sigma : w;
+        const float yt = sigma * (w - fminf(fmaxf(sh, p.lo), p.hi));
+        y[i] = lam * ((1.0f + rho) * yt - rho * p.yk) + (1.0f - lam) * p.y0;
         if (yt_out) yt_out[i] = yt;
     }
+    __device__ void operator()(int i, float ax) const { apply(i, ax, load(i)); }
 };
 
 // ---------------------------------------------------------------------------
