@@ -501,7 +501,7 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
                                                      const int2* __restrict__ seg_a, const int2* __restrict__ seg_at,
                                                      WorkItem* __restrict__ wr, WorkItem* __restrict__ wc,
                                                      int* __restrict__ wcnt) {
-    __shared__ int sr[1024], sc[1024], kr[1024], kc[1024];
+    __shared__ int sr[1024], sc[1024], kr[1024], kc[1024], fa[1024], act[1024];
     __shared__ int carry_r, carry_c, carry_kr, carry_kc;
     const int tid = static_cast<int>(threadIdx.x);
     // ga, gt: rows per item of the lanes kernels, 0 for stream segments,
@@ -512,15 +512,12 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
     for (int base = 0; base < count; base += 1024) {
         const int k = base + tid;
         int nr = 0, nc = 0, m = 0, n = 0, br = 0, bc = 0;
-        int64_t ro = 0, co = 0;
         LpIter P{};
         if (k < count) P = A.it[k];
         if (k < count && P.chunk > 0) {
             const BatchDesc& d = A.desc[k];
             m = d.m;
             n = d.n;
-            ro = d.row_off;
-            co = d.col_off;
             nr = ga > 0 ? (m + ga - 1) / ga : (ga == 0 ? d.sa_cnt : (m + ra - 1) / ra);
             nc = gt > 0 ? (n + gt - 1) / gt : (gt == 0 ? d.st_cnt : (n + rt - 1) / rt);
             br = (m + kCheckRows - 1) / kCheckRows;
@@ -530,49 +527,70 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
         sc[tid] = nc;
         kr[tid] = br;
         kc[tid] = bc;
+        fa[tid] = nr + nc + br + bc > 0 ? 1 : 0;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {  // inclusive scans (Hillis-Steele)
             const int ar = tid >= o ? sr[tid - o] : 0, ac = tid >= o ? sc[tid - o] : 0;
             const int akr = tid >= o ? kr[tid - o] : 0, akc = tid >= o ? kc[tid - o] : 0;
+            const int af = tid >= o ? fa[tid - o] : 0;
             __syncthreads();
             sr[tid] += ar;
             sc[tid] += ac;
             kr[tid] += akr;
             kc[tid] += akc;
+            fa[tid] += af;
             __syncthreads();
         }
-        const int offr = carry_r + sr[tid] - nr, offc = carry_c + sc[tid] - nc;
-        const int offkr = carry_kr + kr[tid] - br, offkc = carry_kc + kc[tid] - bc;
-        for (int g = 0; g < br; ++g) A.ck_r[offkr + g] = make_int2(k, g);
-        for (int g = 0; g < bc; ++g) A.ck_c[offkc + g] = make_int2(k, g);
-        if (k < count) A.wrange[k] = make_int4(offkr, br, offkc, bc);
-        // The last field carries the LP, or for the sliced and staged
-        // kernels the item's first slice.
-        for (int g = 0; g < nr; ++g) {
-            int first = g * ga, rows = min(ga, m - g * ga), tag = k;
-            if (ga == 0) {
-                const int2 sg = seg_a[A.desc[k].sa_off + g];
-                first = sg.x;
-                rows = sg.y;
-            } else if (ga < 0) {
-                first = g * ra;
-                rows = min(ra, m - first);
-                tag = A.desc[k].la_off + first / kSliceRows;
+        if (k < count)
+            A.wrange[k] = make_int4(carry_kr + kr[tid] - br, br, carry_kc + kc[tid] - bc, bc);
+        // The LPs of this chunk with anything to write, compacted; each
+        // gets `per` threads, which write its items and blocks interleaved
+        // (a few LPs left running spread over the whole workgroup).
+        if (nr + nc + br + bc > 0) act[fa[tid] - 1] = tid;
+        __syncthreads();
+        const int na = fa[1023];
+        const int per = na > 0 ? max(1, 1024 / na) : 1024;
+        const int slot = tid / per, sub = tid - slot * per;
+        if (slot < na) {
+            const int q = act[slot], kk = base + q;
+            const int nrk = sr[q] - (q ? sr[q - 1] : 0), nck = sc[q] - (q ? sc[q - 1] : 0);
+            const int brk = kr[q] - (q ? kr[q - 1] : 0), bck = kc[q] - (q ? kc[q - 1] : 0);
+            const int offr = carry_r + sr[q] - nrk, offc = carry_c + sc[q] - nck;
+            const int offkr = carry_kr + kr[q] - brk, offkc = carry_kc + kc[q] - bck;
+            const LpIter Pk = A.it[kk];
+            const BatchDesc& d = A.desc[kk];
+            const int mk = d.m, nk = d.n;
+            const int rok = static_cast<int>(d.row_off), cok = static_cast<int>(d.col_off);
+            for (int g = sub; g < brk; g += per) A.ck_r[offkr + g] = make_int2(kk, g);
+            for (int g = sub; g < bck; g += per) A.ck_c[offkc + g] = make_int2(kk, g);
+            // The last field carries the LP, or for the sliced and staged
+            // kernels the item's first slice.
+            for (int g = sub; g < nrk; g += per) {
+                int first = g * ga, rows = min(ga, mk - g * ga), tag = kk;
+                if (ga == 0) {
+                    const int2 sg = seg_a[d.sa_off + g];
+                    first = sg.x;
+                    rows = sg.y;
+                } else if (ga < 0) {
+                    first = g * ra;
+                    rows = min(ra, mk - first);
+                    tag = d.la_off + first / kSliceRows;
+                }
+                wr[offr + g] = WorkItem{rok + first, rows, Pk.k0, Pk.chunk, Pk.use_g, Pk.col_off, Pk.sig, tag};
             }
-            wr[offr + g] = WorkItem{static_cast<int>(ro) + first, rows, P.k0, P.chunk, P.use_g, P.col_off, P.sig, tag};
-        }
-        for (int g = 0; g < nc; ++g) {
-            int first = g * gt, rows = min(gt, n - g * gt), tag = k;
-            if (gt == 0) {
-                const int2 sg = seg_at[A.desc[k].st_off + g];
-                first = sg.x;
-                rows = sg.y;
-            } else if (gt < 0) {
-                first = g * rt;
-                rows = min(rt, n - first);
-                tag = A.desc[k].lt_off + first / kSliceRows;
+            for (int g = sub; g < nck; g += per) {
+                int first = g * gt, rows = min(gt, nk - g * gt), tag = kk;
+                if (gt == 0) {
+                    const int2 sg = seg_at[d.st_off + g];
+                    first = sg.x;
+                    rows = sg.y;
+                } else if (gt < 0) {
+                    first = g * rt;
+                    rows = min(rt, nk - first);
+                    tag = d.lt_off + first / kSliceRows;
+                }
+                wc[offc + g] = WorkItem{cok + first, rows, Pk.k0, Pk.chunk, 0, Pk.row_off, Pk.tau, tag};
             }
-            wc[offc + g] = WorkItem{static_cast<int>(co) + first, rows, P.k0, P.chunk, 0, P.row_off, P.tau, tag};
         }
         __syncthreads();
         if (tid == 1023) {
