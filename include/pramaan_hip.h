@@ -219,11 +219,13 @@ int pm_ctx_spmv_variants(pm_ctx* ctx, int32_t* variant_a,
  * each running the PM_ALG_PDLP algorithm (power iteration, refinement
  * rounds, reflected Halpern iterations, FP64 KKT checks, restarts)
  * entirely on the device: every iteration is two grid-wide launches over
- * all LPs still running, and the per-LP checks and decisions run one
- * workgroup per LP.  The launch sequence is captured once in a HIP graph
- * and replayed by every pm_batch_run.  Only opts->tolerance,
- * max_iterations, check_frequency and max_refinements are used; the
- * algorithm is always PM_ALG_PDLP with PM_TERM_RELATIVE_KKT. */
+ * all LPs still running; the checks' FP64 products are grid-wide too, and
+ * the per-LP decisions run one workgroup per LP.  The launch sequence is
+ * captured once in a HIP graph (a second one for the tail, when few LPs
+ * remain) and replayed by every pm_batch_run.  Only opts->tolerance,
+ * max_iterations, check_frequency, max_refinements, ruiz_iterations and
+ * spmv_variant are used; the algorithm is always PM_ALG_PDLP with
+ * PM_TERM_RELATIVE_KKT. */
 typedef struct pm_batch pm_batch;
 typedef struct pm_batch_stats {
     double device_seconds;      /* wall time of the launch                   */
@@ -270,9 +272,10 @@ int pm_batch_set_graph(pm_batch* b, int32_t use_graph);
  * changed; the next pm_batch_run starts afresh. */
 int pm_batch_time_kernel(pm_batch* b, int32_t which, int32_t reps,
                          double* avg_ms, double* alg_bytes);
-/* Lanes per row the iterate kernels use for A and A^T, as PM_SPMV_SCALAR /
- * PM_SPMV_VECTORx: opts->spmv_variant when it names one of those, else
- * autotuned at pm_batch_create over the whole batch. */
+/* Iterate kernels used for A and A^T while many LPs run, as PM_SPMV_*
+ * (SCALAR / VECTORx: lanes per row; STREAM; SLICED; STAGED):
+ * opts->spmv_variant when it names one of those, else autotuned at
+ * pm_batch_create over the whole batch. */
 int pm_batch_spmv_variants(pm_batch* b, int32_t* variant_a,
                            int32_t* variant_at);
 /* create + run + destroy; x0/y0 may be NULL or arrays of count pointers. */

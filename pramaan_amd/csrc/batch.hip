@@ -1,26 +1,34 @@
-// batch.hip -- many independent LPs solved by one graph launch.
+// batch.hip -- many independent LPs solved together by one replayed graph.
 //
 // Layout: every LP's A and A^T (CSR, local indices) and vectors are packed
 // back to back in shared device arrays, i.e. the batch is one block-diagonal
 // LP; a per-LP descriptor holds the offsets, and global row / column pointer
-// arrays plus a row -> LP and column -> LP map address the concatenation.
+// arrays address the concatenation.  A sliced ELL copy of A and A^T (rows
+// sorted by length within groups of 1024, 64-row slices, 16-bit indexes
+// relative to each group's column window) feeds the sliced iterate kernels.
+//
+// Creation: host threads transpose each LP and build its layouts; the
+// device scales the whole concatenation (Ruiz + Pock-Chambolle act on one
+// row or column at a time, so each LP gets its own factors); the autotuner
+// times every iterate kernel on the full batch and keeps the fastest.
 //
 // Phases:
-//  init, round, check -- one 1024-thread workgroup per LP (largest first):
-//    power iteration for ||A_hat||_2 and the initial primal weight; the
-//    correction-LP data of a refinement round; the FP64 KKT of the candidate
-//    with deterministic block reductions, then the pm::Control decision (the
-//    code the single-LP host loop runs) and the restart / refinement it asks
-//    for.  These run once per check_frequency iterations or less.
+//  init -- one 1024-thread workgroup per LP (largest first): power
+//    iteration for ||A_hat||_2 (first run only), the initial primal weight,
+//    the judgement of a warm start by its FP64 KKT error.
 //  iterate -- the hot loop: each reflected-Halpern iteration is two
-//    grid-wide launches over the whole concatenation (A^T y -> x, A xe ->
-//    y), the single-LP engine's row kernels with per-LP step sizes, Halpern
-//    weights and chunk lengths read from a small per-LP record.  Rows of
-//    finished LPs are skipped.  Each XCD owns one contiguous eighth of the
-//    rows (so it gathers from the vectors of its own LPs only, which stay
-//    in its L2), and inside that eighth consecutive row groups go to
-//    different workgroups, so even the last LP still running is spread over
-//    a whole XCD instead of a single CU.
+//    grid-wide launches over a work list of the running LPs' row groups
+//    (A^T y -> x, A xe -> y) with per-LP step sizes, Halpern weights and
+//    chunk lengths carried by each item.  Each XCD owns one contiguous
+//    eighth of the list (its LPs' vectors stay in its L2).  Grids are sized
+//    to the kernel's occupancy.  Once few LPs are left a second graph
+//    switches to lanes-per-row kernels with many small items per LP.
+//  round / check -- grid-wide passes over 128-row blocks of every running
+//    LP: a refinement round's correction-LP data, the total iterates, the
+//    FP64 relative-KKT terms reduced per block; then one workgroup per LP
+//    sums its blocks, takes the pm::Control decision (the code the single-LP
+//    host loop runs) and applies the restart or refinement it asks for; the
+//    work list is rebuilt.
 // Per-LP solver state lives in device memory between phases.  A few
 // round/iterate/check chunks are captured once in a HIP graph; the host
 // replays it until a device counter reports no active LP.  Nothing is

@@ -27,8 +27,8 @@ struct BatchDesc {
     // (rows of A) and seg_at (rows of A^T) from these offsets
     int32_t sa_off, sa_cnt, st_off, st_cnt;
     // slices of the sliced iterate kernels (64 rows each, rows sorted by
-    // length inside every group of 256), from these offsets in the sliced
-    // arrays of A and A^T
+    // length inside every group of kSliceGroup), from these offsets in the
+    // sliced arrays of A and A^T
     int32_t la_off, lt_off;
     // results
     int32_t status, iterations, restarts, refinements;
