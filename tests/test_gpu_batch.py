@@ -101,6 +101,28 @@ def test_batch_device_scaling_matches_restatement(capi, ruiz_restated):
             assert np.array_equal(dr, wr) and np.array_equal(dc, wc), i
 
 
+def test_batch_wide_window_falls_back(capi):
+    """A row touching 20,000 columns makes A's column window wider than the
+    staged kernel's LDS holds: an explicit PM_SPMV_STAGED is rejected at
+    creation, and the autotuner picks another kernel for A and still solves
+    the batch (min c.x, sum x = 10, x_0..x_2 <= 5: x_0 = x_1 = 5)."""
+    nd = 20000
+    dense = np.vstack([np.ones(nd), np.eye(nd)[:3]])
+    c = np.linspace(1.0, 2.0, nd)
+    lp = capi.LP.from_dense(dense, c, [10.0, 0.0, 0.0, 0.0], [10.0, 5.0, 5.0, 5.0], np.zeros(nd),
+                            np.full(nd, 1e30))
+    lps = small_batch(capi, count=3, seed0=1900) + [lp]
+    with pytest.raises(capi.PramaanError, match="window"):
+        capi.Batch(lps, tolerance=1e-6, spmv_variant="staged")
+    with capi.Batch(lps, tolerance=1e-7, max_iterations=60000, check_frequency=64) as b:
+        va, _ = b.spmv_variants()
+        assert va != capi.SPMV["staged"]
+        res, _, _ = b.run()
+    for q, r in zip(lps[:-1], res):
+        assert r.status == "OPTIMAL" and abs(r.objective - q.p_star) <= 1e-5 * (1 + abs(q.p_star))
+    assert res[-1].status == "OPTIMAL" and abs(res[-1].objective - 5 * (c[0] + c[1])) <= 1e-4
+
+
 def test_batch_one_lp_and_mixed_sizes(capi):
     """A batch of one LP and a batch whose sizes differ 100x (the work list
     and the per-LP chunk lengths), against each LP's closed form."""
