@@ -28,6 +28,7 @@ import json
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
+from pathlib import Path
 
 import numpy as np
 
@@ -250,6 +251,11 @@ def train(net, ts, epochs=8, batch=16, lr=2e-3, val_frac=0.05, log=print, seed=0
             hist.append(rec)
             log(json.dumps(rec))
     return dict(samples_per_s=seen / max(t_train, 1e-9), history=hist)
+
+
+def default_weights() -> Path:
+    """The shipped trained parameters (trained by `python -m pramaan_amd.rolling`)."""
+    return Path(__file__).resolve().parent / "data" / "warmstart_gnn.npz"
 
 
 def save(net: P.Gnn, path, meta: dict | None = None) -> None:

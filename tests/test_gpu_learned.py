@@ -53,3 +53,32 @@ def test_training_needs_a_training_graph(capi):
         b.learned_warm_start(net)
     with pytest.raises(capi.PramaanError):
         net.train_step(np.zeros(70, np.float32), np.zeros(30, np.float32), lr=1e-3)
+
+
+def test_trained_network_gains_over_cold_and_untrained(capi):
+    """What the trained network (pramaan_amd/data/warmstart_gnn.npz) gains on 42
+    windows spread over the rolling-horizon sequence, solved to 1e-6 in one
+    batch: fewer iterations than the cold start, its predictions kept by the
+    KKT judge, while an untrained network of the same shape gains nothing
+    (its predictions are worse than the cold start and are replaced).
+    Measured: 81,792 cold iterations, 62,592 learned (all 42 kept)."""
+    from pramaan_amd import rolling as R
+    rh = capi.RollingHorizon()
+    lps = [rh.window(w) for w in range(0, 500, 12)]
+    cpp, rpp = rh.cols_per_period, rh.rows_per_period
+    trained = R.load(R.default_weights())
+    untrained = capi.Gnn(seed=7)
+    out = {}
+    with capi.Batch(lps, tolerance=1e-6, max_iterations=100000, check_frequency=64) as b:
+        for name, net in (("cold", None), ("trained", trained), ("untrained", untrained)):
+            for i in range(len(lps)):
+                b.set_warm_start(i, None, None)
+            if net is not None:
+                b.learned_warm_start(net, cpp, rpp)
+            res, _, _ = b.run()
+            acc, _ = b.warm_start_verdicts()
+            assert all(r.status == "OPTIMAL" for r in res), name
+            out[name] = (sum(r.iterations for r in res), int((acc == 1).sum()))
+    cold, trained_run, untrained_run = out["cold"][0], out["trained"], out["untrained"]
+    assert trained_run[0] <= 0.85 * cold and trained_run[1] >= 40, out
+    assert untrained_run[1] <= 4 and untrained_run[0] >= 0.95 * cold, out
