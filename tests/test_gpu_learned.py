@@ -61,7 +61,7 @@ def test_trained_network_gains_over_cold_and_untrained(capi):
     batch: fewer iterations than the cold start, its predictions kept by the
     KKT judge, while an untrained network of the same shape gains nothing
     (its predictions are worse than the cold start and are replaced).
-    Measured: 81,792 cold iterations, 62,592 learned (all 42 kept)."""
+    Measured: 80,704 cold iterations, 62,016 learned (all 42 kept)."""
     from pramaan_amd import rolling as R
     rh = capi.RollingHorizon()
     lps = [rh.window(w) for w in range(0, 500, 12)]
