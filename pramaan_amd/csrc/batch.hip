@@ -762,13 +762,14 @@ __global__ __launch_bounds__(kBlock) void k_t_half(BatchArrays<VT> A, const Work
 
 // Steps j0 .. j1-1 of NA slices at once (one row per lane in each): the
 // 2 NA loads of a step are independent, so a wavefront keeps them in
-// flight together.  Each row is summed in CSR order with FMA, as the
+// flight together, and U steps are unrolled (4 for the longer rows of A,
+// 2 for A^T, measured).  Each row is summed in CSR order with FMA, as the
 // one-lane-per-row kernels sum it.
-template <int NA>
+template <int NA, int U>
 __device__ __forceinline__ void slice_steps(const float* __restrict__ val, const uint16_t* __restrict__ idx,
                                             const float* xv, const int (&base)[4], int j0, int j1,
                                             float (&s)[4]) {
-#pragma unroll 2
+#pragma unroll U
     for (int j = j0; j < j1; ++j) {
         float v[NA];
         int c[NA];
@@ -835,10 +836,10 @@ __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const Work
             xv = xs;
         }
         float sm[kPer] = {0.0f, 0.0f, 0.0f, 0.0f};
-        slice_steps<4>(val, idx, xv, base, 0, ln[3], sm);
-        slice_steps<3>(val, idx, xv, base, ln[3], ln[2], sm);
-        slice_steps<2>(val, idx, xv, base, ln[2], ln[1], sm);
-        slice_steps<1>(val, idx, xv, base, ln[1], ln[0], sm);
+        slice_steps<4, PRIMAL ? 2 : 4>(val, idx, xv, base, 0, ln[3], sm);
+        slice_steps<3, PRIMAL ? 2 : 4>(val, idx, xv, base, ln[3], ln[2], sm);
+        slice_steps<2, PRIMAL ? 2 : 4>(val, idx, xv, base, ln[2], ln[1], sm);
+        slice_steps<1, PRIMAL ? 2 : 4>(val, idx, xv, base, ln[1], ln[0], sm);
 #pragma unroll
         for (int u = 0; u < kPer; ++u)
             if (ps[u] >= 0) sums[ps[u]] = sm[u];
