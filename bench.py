@@ -192,6 +192,20 @@ def bench_rolling(args, P) -> dict:
         "batch_kernels": {k: {"us": v[0] * 1e3, "alg_bytes": v[1], "GBps": v[1] / v[0] / 1e6}
                           for k, v in kern.items()},
     }
+    if args.previous > 0:
+        # Previous-window warm start: inherently sequential (window w starts
+        # from window w-1's solution shifted by one period), so it runs on
+        # the single-LP engine; compared by iterations on the same windows.
+        npw = min(args.previous, nw)
+        st_p, _ = R.solve_previous(lps[:npw], cpp, rpp, args.tol)
+        out["previous_window"] = {
+            "windows": npw, "seconds_sequential": st_p.seconds,
+            "optimal": int(sum(s == "OPTIMAL" for s in st_p.statuses)),
+            "iterations_mean": float(st_p.iterations.mean()),
+            "cold_iterations_mean_same_windows": float(it_c[:npw].mean()),
+            "learned_iterations_mean_same_windows": float(it_l[:npw].mean())}
+        log(f"previous-window warm start on windows 0..{npw - 1}: iterations mean {st_p.iterations.mean():.0f} "
+            f"(cold {it_c[:npw].mean():.0f}, learned {it_l[:npw].mean():.0f}), {st_p.seconds:.2f}s sequential")
     if not args.no_lp1:
         out["spmv_lp1"] = spmv_lp1(P, args.kernel_reps)
     if not args.no_cpu:
@@ -301,6 +315,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=["rolling500", "lp1", "batch4096"], default="rolling500")
     ap.add_argument("--instances", type=int, default=4096, help="LPs of the batch4096 workload")
     ap.add_argument("--windows", type=int, default=500)
+    ap.add_argument("--previous", type=int, default=100,
+                    help="windows of the sequential previous-window warm-start comparison (0 skips it)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iters", type=int, default=200000)
     ap.add_argument("--kernel-reps", type=int, default=20)
