@@ -858,8 +858,6 @@ __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const Work
     }
 }
 
-// Phase 3: candidate -> FP64 KKT -> Control decision -> restart / refine /
-// finish.  Counts the LPs still active.
 // ---- round and check phases over the work items --------------------------
 //
 // The FP64 products of a refinement round's data and of the check's KKT
@@ -999,6 +997,8 @@ __global__ __launch_bounds__(kBlock) void k_b_items(BatchArrays<VT> A, const Bat
     }
 }
 
+// Phase 3, per LP: the KKT error from its blocks' partial sums -> Control
+// decision -> restart / refine / finish.  Counts the LPs still active.
 template <class VT>
 __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* S, int* active_count) {
     const int lp = A.order[blockIdx.x];
