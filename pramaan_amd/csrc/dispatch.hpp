@@ -95,7 +95,7 @@ void launch_rows_reduce(int variant, int nrows, const int* rp, const int* ci, co
         });
         PM_LAUNCHED();
     }
-    hipLaunchKernelGGL(k_final_reduce, dim3(1), dim3(kBlock), 0, s, partials, grid, Epi::NS, Epi::NM, out);
+    hipLaunchKernelGGL(k_final_reduce<>, dim3(1), dim3(kBlock), 0, s, partials, grid, Epi::NS, Epi::NM, out);
     PM_LAUNCHED();
     (void)NA;
 }
@@ -114,7 +114,7 @@ void launch_elem_reduce(int n, const F& f, double* partials, double* out, hipStr
         hipLaunchKernelGGL((k_elem_reduce<F>), dim3(grid), dim3(kBlock), 0, s, n, f, partials);
         PM_LAUNCHED();
     }
-    hipLaunchKernelGGL(k_final_reduce, dim3(1), dim3(kBlock), 0, s, partials, grid, F::NS, F::NM, out);
+    hipLaunchKernelGGL(k_final_reduce<>, dim3(1), dim3(kBlock), 0, s, partials, grid, F::NS, F::NM, out);
     PM_LAUNCHED();
 }
 

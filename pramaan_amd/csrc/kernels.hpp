@@ -163,9 +163,11 @@ __global__ __launch_bounds__(kBlock) void k_elem_reduce(int n, F f, double* __re
 
 // Sums (first ns) and maxima (next nm) of `nblocks` records of width
 // ns+nm, in a fixed order: one block, thread j owns blocks j, j+256, ...
-static __global__ __launch_bounds__(kBlock) void k_final_reduce(const double* __restrict__ partials,
-                                                         int nblocks, int ns, int nm,
-                                                         double* __restrict__ out) {
+// (A template only so that translation units that do not launch it do not
+// instantiate it.)
+template <int = 0>
+__global__ __launch_bounds__(kBlock) void k_final_reduce(const double* __restrict__ partials, int nblocks, int ns,
+                                                         int nm, double* __restrict__ out) {
     __shared__ double red[kBlock];
     const int w = ns + nm;
     for (int j = 0; j < w; ++j) {
