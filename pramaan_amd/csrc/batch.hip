@@ -4,8 +4,9 @@
 // back to back in shared device arrays, i.e. the batch is one block-diagonal
 // LP; a per-LP descriptor holds the offsets, and global row / column pointer
 // arrays address the concatenation.  A sliced ELL copy of A and A^T (rows
-// sorted by length within groups of 1024, 64-row slices, 16-bit indexes
-// relative to each group's column window) feeds the sliced iterate kernels.
+// sorted by length within groups of 1024 rows of A and 2048 of A^T, 64-row
+// slices, 16-bit indexes relative to each group's column window) feeds the
+// sliced iterate kernels.
 //
 // Creation: host threads transpose each LP and build its layouts; the
 // device scales the whole concatenation (Ruiz + Pock-Chambolle act on one
@@ -103,7 +104,7 @@ inline int lanes_variant(int l) {
 }
 
 // Sliced ELL form of one CSR (local indices), appended to the host arrays:
-// groups of kSliceGroup rows, rows sorted by length (longest first, ties in
+// groups of `group` rows, rows sorted by length (longest first, ties in
 // row order) inside each group, slices of kSliceRows sorted rows; each
 // row's entries stay in CSR order.
 struct SlicedHost {
@@ -118,10 +119,10 @@ struct SlicedHost {
 // first column for an empty row), so that they read an element the group
 // reads anyway.  The values are filled on the device from the scaled CSR
 // values: slot e takes value[src[e]] (global entry indexes from nz0).
-inline void build_sliced(int nrows, const int* rp, const int* ci, int64_t nz0, SlicedHost& out) {
+inline void build_sliced(int nrows, const int* rp, const int* ci, int64_t nz0, SlicedHost& out, int group) {
     std::vector<int> order;
-    for (int g0 = 0; g0 < nrows; g0 += kSliceGroup) {
-        const int gn = std::min(kSliceGroup, nrows - g0);
+    for (int g0 = 0; g0 < nrows; g0 += group) {
+        const int gn = std::min(group, nrows - g0);
         int lo = INT_MAX, hi = -1;
         for (int k = rp[g0]; k < rp[g0 + gn]; ++k) {
             lo = std::min(lo, ci[k]);
@@ -514,7 +515,7 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
     const int tid = static_cast<int>(threadIdx.x);
     // ga, gt: rows per item of the lanes kernels, 0 for stream segments,
     // negative for groups of the sliced layout.
-    constexpr int ra = kSliceGroup, rt = kSliceGroup;
+    constexpr int ra = kSliceGroup, rt = kSliceGroupT;
     if (tid == 0) carry_r = carry_c = carry_kr = carry_kc = 0;
     __syncthreads();
     for (int base = 0; base < count; base += 1024) {
@@ -783,11 +784,13 @@ __device__ __forceinline__ void slice_steps(const float* __restrict__ val, const
     }
 }
 
-// Sliced kernels: a work item is a group of up to kSliceGroup rows of one
-// LP in the sliced layout.  Wavefront w takes slices w, w + 4, w + 8 and
-// w + 12 of the group and walks them together: lane l reads entry j of its
-// row in slice q at ptr[q] + j * 64 + l, so each value and index load of
-// 64 lanes is one contiguous 256-byte run.  Slice lengths do not grow
+// Sliced kernels: a work item is a group of up to kSliceGroup rows of A
+// (kSliceGroupT of A^T, whose shorter rows gain from the larger group) of
+// one LP in the sliced layout, walked in passes of 16 slices.  In a pass,
+// wavefront w takes slices w, w + 4, w + 8 and w + 12 and walks them
+// together: lane l reads entry j of its row in slice q at ptr[q] + j * 64
+// + l, so each value and index load of 64 lanes is one contiguous 256-byte
+// run.  Slice lengths do not grow
 // along the group (its rows are sorted by length), so the walk runs all
 // four slices up to the shortest one's length, then three, two, one.
 // STAGED first copies the group's column window of the gathered vector (y
@@ -799,10 +802,10 @@ __device__ __forceinline__ void slice_steps(const float* __restrict__ val, const
 template <bool PRIMAL, bool STAGED, class VT>
 __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const WorkItem* __restrict__ work,
                                                    const int* __restrict__ wcnt, int i) {
-    constexpr int kWaves = kBlock / 64, kPer = kSliceGroup / kSliceRows / kWaves;
-    static_assert(kPer == 4, "four slices per wavefront");
+    constexpr int kWaves = kBlock / 64, kPer = 4, kPass = kWaves * kPer;
+    constexpr int kG = PRIMAL ? kSliceGroupT : kSliceGroup;
     extern __shared__ float xs[];
-    __shared__ float sums[kSliceGroup];
+    __shared__ float sums[kG];
     const int t = static_cast<int>(threadIdx.x);
     const int lane = t & 63, wv = t >> 6;
     const float* val = PRIMAL ? A.st_val : A.sa_val;
@@ -818,15 +821,6 @@ __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const Work
         const int row0 = it.a.x, cnt = it.a.y, chunk = it.a.w, first = it.b.w;
         if (i >= chunk) continue;  // whole item: uniform over the workgroup
         const int nsl = (cnt + kSliceRows - 1) / kSliceRows;
-        int base[kPer], ln[kPer], ps[kPer];
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int q = wv + kWaves * u;
-            const bool on = q < nsl;
-            base[u] = on ? ptr[first + q] + lane : 0;
-            ln[u] = on ? len[first + q] : 0;
-            ps[u] = on ? pos[(first + q) * kSliceRows + lane] : -1;
-        }
         const int2 wn = win[first];
         const float* xv = (PRIMAL ? A.y : A.xe) + it.b.y + wn.x;
         if constexpr (STAGED) {
@@ -835,14 +829,25 @@ __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const Work
             __syncthreads();
             xv = xs;
         }
-        float sm[kPer] = {0.0f, 0.0f, 0.0f, 0.0f};
-        slice_steps<4, PRIMAL ? 2 : 4>(val, idx, xv, base, 0, ln[3], sm);
-        slice_steps<3, PRIMAL ? 2 : 4>(val, idx, xv, base, ln[3], ln[2], sm);
-        slice_steps<2, PRIMAL ? 2 : 4>(val, idx, xv, base, ln[2], ln[1], sm);
-        slice_steps<1, PRIMAL ? 2 : 4>(val, idx, xv, base, ln[1], ln[0], sm);
+        for (int p0 = 0; p0 < nsl; p0 += kPass) {
+            int base[kPer], ln[kPer], ps[kPer];
 #pragma unroll
-        for (int u = 0; u < kPer; ++u)
-            if (ps[u] >= 0) sums[ps[u]] = sm[u];
+            for (int u = 0; u < kPer; ++u) {
+                const int q = p0 + wv + kWaves * u;
+                const bool on = q < nsl;
+                base[u] = on ? ptr[first + q] + lane : 0;
+                ln[u] = on ? len[first + q] : 0;
+                ps[u] = on ? pos[(first + q) * kSliceRows + lane] : -1;
+            }
+            float sm[kPer] = {0.0f, 0.0f, 0.0f, 0.0f};
+            slice_steps<4, PRIMAL ? 2 : 4>(val, idx, xv, base, 0, ln[3], sm);
+            slice_steps<3, PRIMAL ? 2 : 4>(val, idx, xv, base, ln[3], ln[2], sm);
+            slice_steps<2, PRIMAL ? 2 : 4>(val, idx, xv, base, ln[2], ln[1], sm);
+            slice_steps<1, PRIMAL ? 2 : 4>(val, idx, xv, base, ln[1], ln[0], sm);
+#pragma unroll
+            for (int u = 0; u < kPer; ++u)
+                if (ps[u] >= 0) sums[ps[u]] = sm[u];
+        }
         __syncthreads();
         const bool keep = i == chunk - 1;
         const float stp = __int_as_float(it.b.z), lam = Control::lambda(it.a.z + i);
@@ -1242,8 +1247,8 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         d.lanes_at = lanes_for(static_cast<double>(nnz) / n);
         stream_segments(m, L.row_ptr, seg_a_lp[k]);
         stream_segments(n, cp.data(), seg_at_lp[k]);
-        build_sliced(m, L.row_ptr, L.col_idx, d.nz_off, sl_a_lp[k]);
-        build_sliced(n, cp.data(), ri.data(), d.nz_off, sl_at_lp[k]);
+        build_sliced(m, L.row_ptr, L.col_idx, d.nz_off, sl_a_lp[k], kSliceGroup);
+        build_sliced(n, cp.data(), ri.data(), d.nz_off, sl_at_lp[k], kSliceGroupT);
     };
     {
         const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));

@@ -27,8 +27,8 @@ struct BatchDesc {
     // (rows of A) and seg_at (rows of A^T) from these offsets
     int32_t sa_off, sa_cnt, st_off, st_cnt;
     // slices of the sliced iterate kernels (64 rows each, rows sorted by
-    // length inside every group of kSliceGroup), from these offsets in the
-    // sliced arrays of A and A^T
+    // length inside every group of kSliceGroup / kSliceGroupT rows), from
+    // these offsets in the sliced arrays of A and A^T
     int32_t la_off, lt_off;
     // results
     int32_t status, iterations, restarts, refinements;
@@ -54,10 +54,11 @@ constexpr int kStreamNnz = 1024;
 // stream kernel, and the two kernels over the sliced layout, gathering
 // from global memory (sliced) or from a copy in LDS (staged).
 constexpr int kStreamKernel = 0, kSlicedKernel = -1, kStagedKernel = -2;
-// Sliced layout: groups of kSliceGroup consecutive rows, one group per
-// work item; inside a group the rows are sorted by length and cut into
-// slices of kSliceRows (one row per lane of a wavefront).
-constexpr int kSliceRows = 64, kSliceGroup = 1024;
+// Sliced layout: groups of kSliceGroup consecutive rows of A (kSliceGroupT
+// of A^T), one group per work item; inside a group the rows are sorted by
+// length and cut into slices of kSliceRows (one row per lane of a
+// wavefront).
+constexpr int kSliceRows = 64, kSliceGroup = 1024, kSliceGroupT = 2048;
 // Largest column window (floats) the staged kernel holds in LDS, and the
 // largest any sliced kernel takes (its indexes are 16-bit).
 constexpr int kStagedMaxVec = 15 * 1024, kSlicedMaxWin = 65536;

@@ -63,8 +63,9 @@ def test_batch_direct_launches_match_graph(capi):
 def test_batch_sliced_kernels_match_one_lane_per_row(capi):
     """The sliced kernels (global and LDS gathers) sum every row in CSR
     order with FMA, as the one-lane-per-row kernel does, so whole solves
-    give the same bits; the batch mixes groups of 1024 rows with partial
-    groups and slices, an empty row and a 1500-entry row."""
+    give the same bits; the batch mixes full groups (1024 rows of A, 2048
+    of A^T, walked in two passes of 16 slices) with partial groups and
+    slices, an empty row and a 1500-entry row."""
     lps = small_batch(capi, count=5, seed0=1700)
     lps.append(capi.generate(None, seed=1705, m=2300, n=5750, k_min=4, k_max=16))
     nd = 1500
