@@ -31,6 +31,7 @@ windows.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload lp1        (the single 500k x 1M LP instead)
        python bench.py --workload batch4096  (4,096 generated LPs in one batch)
+       python bench.py --workload ruiz       (Ruiz scaling of the 500k x 1M LP)
 """
 from __future__ import annotations
 
@@ -307,12 +308,53 @@ def bench_lp1(args, P) -> dict:
     return out
 
 
+def bench_ruiz(args, P) -> dict:
+    """pm_ruiz_scale (the reference's ruiz_scale, 10 passes) on the 500k x 1M
+    LP.  The reference's interface scales a host-side model in place, so a
+    call moves the LP to the device and back: `value` is that whole call;
+    the kernels' own durations come from a rocprofv3 kernel trace of this
+    command."""
+    from oracle import binding as O
+    lp = P.generate(1, seed=1)
+    for _ in range(args.warmup):
+        P.ruiz_scale(lp, 10)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        got = P.ruiz_scale(lp, 10)
+        times.append(time.perf_counter() - t0)
+    sec = float(np.mean(times))
+    # in: row pointers, indexes and values, c, four bound arrays; out:
+    # values, c, the bounds and both factor arrays
+    host_bytes = (4 * (lp.m + 1) + 12 * lp.nnz + 8 * lp.n + 16 * (lp.m + lp.n)) \
+        + (8 * lp.nnz + 8 * lp.n + 16 * (lp.m + lp.n) + 8 * (lp.m + lp.n))
+    out = {
+        "metric": "s per ruiz_scale call (10 passes, host model in and out)", "value": sec, "unit": "s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Ruiz equilibration, 10 passes, of the LP m={lp.m} n={lp.n} nnz={lp.nnz}; "
+                               "host arrays to the device and back inside the timed call",
+                   "m": lp.m, "n": lp.n, "nnz": lp.nnz, "passes": 10},
+        "host_device_bytes": host_bytes,
+        "roofline": None,
+        "roofline_note": "kernel durations per pass: profiles/ruiz_scale_lp1_kernel_trace.md",
+    }
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        want = O.ruiz_scale(lp, 10)
+        cpu = time.perf_counter() - t0
+        out["bitwise_equal_to_oracle"] = all(np.array_equal(got[k], want[k]) for k in want)
+        out["cpu_baseline"] = {"value": cpu, "unit": "s", "cores": 1, "kind": "port",
+                               "sample": "one full call of the oracle's restated ruiz_scale on the same LP"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["rolling500", "lp1", "batch4096"], default="rolling500")
+    ap.add_argument("--workload", choices=["rolling500", "lp1", "batch4096", "ruiz"], default="rolling500")
     ap.add_argument("--instances", type=int, default=4096, help="LPs of the batch4096 workload")
     ap.add_argument("--windows", type=int, default=500)
     ap.add_argument("--previous", type=int, default=100,
@@ -335,6 +377,8 @@ def main() -> int:
         out = bench_rolling(args, P)
     elif args.workload == "batch4096":
         out = bench_batch(args, P)
+    elif args.workload == "ruiz":
+        out = bench_ruiz(args, P)
     else:
         out = bench_lp1(args, P)
     print(json.dumps(out), flush=True)

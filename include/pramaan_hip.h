@@ -18,6 +18,9 @@
  *   pm_launch_spmv           <- gpu::launch_spmv   src/gpu/pdhg.cu:156-196
  *   pm_launch_spmv_t         <- gpu::launch_spmv_t src/gpu/pdhg.cu:198-241
  *   pm_launch_pdhg_iteration <- gpu::launch_pdhg_iteration src/gpu/pdhg.cu:77-146
+ *   pm_ruiz_scale            <- pramaan::ruiz_scale
+ *                               include/pramaan/presolve.hpp:45,
+ *                               src/presolve/scaling.cpp:29-150
  *   pm_last_error            <- the message CUDA_CHECK puts in the
  *                               std::runtime_error it throws
  *                               src/gpu/cuda_utils.cuh:8-15
@@ -182,6 +185,24 @@ int pm_launch_spmv_t(const int32_t* row_ptr, const int32_t* col_idx,
 int pm_launch_pdhg_iteration(float* x, float* y, const float* gradient,
                              const float* ax_bar, const float* b, int32_t n,
                              int32_t m, float tau, float sigma);
+
+/* ---- Ruiz equilibration ------------------------------------------------ */
+/* `passes` passes over the m x n LP (CSR row_ptr[m+1], col_idx/values[nnz]):
+ * each takes the infinity norm of every row and of every column (a column's
+ * including |c_j|) and scales rows and columns by 1/sqrt(norm) (1 for a
+ * zero norm).  In place: values, c, the finite row bounds (times the row
+ * factor) and the finite variable bounds (divided by the column factor;
+ * |bound| >= PM_INFINITY_VALUE is absent).  row_scale[m], col_scale[n]
+ * receive the cumulative factors: A_scaled = diag(row_scale) A
+ * diag(col_scale).  With m = 0, n = 0 or passes = 0 nothing changes and the
+ * factors are 1.  Host arrays in and out; the passes run on device 0, with
+ * the reference's FP64 expressions, so the results equal its host loops bit
+ * for bit. */
+int pm_ruiz_scale(int32_t m, int32_t n, const int32_t* row_ptr,
+                  const int32_t* col_idx, double* values, double* c,
+                  double* row_lower, double* row_upper, double* var_lower,
+                  double* var_upper, int32_t passes, double* row_scale,
+                  double* col_scale);
 
 /* ---- device-resident engine -------------------------------------------- */
 /* A context owns one device stream and the device copy of one LP (both CSR

@@ -6,7 +6,9 @@
 //
 //   ref_driver simplex <model.mps> [x.out] -> "status objective iterations"
 //                                    (and x, one value per line, to x.out)
-//   ref_driver ruiz <model.mps> <passes> -> "sum_row_scale sum_col_scale sum|A|"
+//   ref_driver ruiz <model.mps> <passes> [out.txt] -> "sum_row_scale sum_col_scale sum|A|"
+//       (and to out.txt, %.17g one per line: row_scale, col_scale, then per
+//       row its (column, value) entries, c, row bounds, variable bounds)
 //   ref_driver cert <model.mps> <x.txt> <status> <tolerance> <out.cert>
 //       the reference's generate_certificate + write_certificate
 //       (src/certificate/certificate.cpp:23-153) for the given x.
@@ -54,6 +56,22 @@ int main(int argc, char** argv) {
             for (double v : f.col_scale) sc += v;
             for (double v : model.A.values()) sa += std::fabs(v);
             std::printf("%.17g %.17g %.17g\n", sr, sc, sa);
+            if (argc > 4) {
+                std::FILE* out = std::fopen(argv[4], "w");
+                if (!out) return 1;
+                for (double v : f.row_scale) std::fprintf(out, "%.17g\n", v);
+                for (double v : f.col_scale) std::fprintf(out, "%.17g\n", v);
+                const auto& rp = model.A.rowPtr();
+                for (int r = 0; r < model.numRows(); ++r)
+                    for (int k = rp[r]; k < rp[r + 1]; ++k)
+                        std::fprintf(out, "%d %.17g\n", model.A.colIdx()[k], model.A.values()[k]);
+                for (double v : model.obj_coeffs) std::fprintf(out, "%.17g\n", v);
+                for (int r = 0; r < model.numRows(); ++r)
+                    std::fprintf(out, "%.17g %.17g\n", model.row_lower[r], model.row_upper[r]);
+                for (int j = 0; j < model.numVars(); ++j)
+                    std::fprintf(out, "%.17g %.17g\n", model.var_lower[j], model.var_upper[j]);
+                std::fclose(out);
+            }
             return 0;
         }
         if (std::strcmp(argv[1], "cert") == 0 && argc > 6) {

@@ -79,6 +79,7 @@ SIGNATURES = {
     "pm_pdhg_options_default": (None, [C.POINTER(pm_pdhg_options)]),
     "pm_pdhg_solve": (C.c_int, [C.POINTER(pm_lp), C.POINTER(pm_pdhg_options), C.POINTER(pm_pdhg_result)]),
     "pm_launch_spmv": (C.c_int, [_i, _i, _f, C.c_int32, _f, _f, C.c_int32, C.c_int32, C.c_int32]),
+    "pm_ruiz_scale": (C.c_int, [C.c_int32, C.c_int32, _i, _i, _d, _d, _d, _d, _d, _d, C.c_int32, _d, _d]),
     "pm_launch_spmv_t": (C.c_int, [_i, _i, _f, C.c_int32, _f, _f, C.c_int32, C.c_int32, C.c_int32]),
     "pm_launch_pdhg_iteration": (C.c_int, [_f, _f, _f, _f, _f, C.c_int32, C.c_int32, C.c_float, C.c_float]),
     "pm_ctx_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
@@ -617,6 +618,24 @@ def launch_spmv(row_ptr, col_idx, values, x, m, n, variant="auto") -> np.ndarray
     _check(lib().pm_launch_spmv(_p(rp, C.c_int32), _p(ci, C.c_int32), _p(v, C.c_float), len(v),
                                 _p(x, C.c_float), _p(y, C.c_float), m, n, SPMV.get(variant, variant)))
     return y
+
+
+def ruiz_scale(lp: LP, passes: int = 10) -> dict:
+    """pm_ruiz_scale (the reference's ruiz_scale, scaling.cpp:29-150) on
+    copies of the LP's arrays: the scaled values, c and bounds and the
+    cumulative row_scale / col_scale."""
+    rp = np.ascontiguousarray(lp.row_ptr, np.int32)
+    ci = np.ascontiguousarray(lp.col_idx, np.int32)
+    v = np.array(lp.values, np.float64)
+    c = np.array(lp.c, np.float64)
+    rl, ru = np.array(lp.row_lower, np.float64), np.array(lp.row_upper, np.float64)
+    xl, xu = np.array(lp.var_lower, np.float64), np.array(lp.var_upper, np.float64)
+    rs, cs = np.zeros(max(lp.m, 1)), np.zeros(max(lp.n, 1))
+    _check(lib().pm_ruiz_scale(lp.m, lp.n, _p(rp, C.c_int32), _p(ci, C.c_int32), _p(v, C.c_double),
+                               _p(c, C.c_double), _p(rl, C.c_double), _p(ru, C.c_double), _p(xl, C.c_double),
+                               _p(xu, C.c_double), passes, _p(rs, C.c_double), _p(cs, C.c_double)))
+    return dict(values=v, c=c, row_lower=rl, row_upper=ru, var_lower=xl, var_upper=xu,
+                row_scale=rs[:lp.m], col_scale=cs[:lp.n])
 
 
 def launch_spmv_t(row_ptr, col_idx, values, x, m, n, variant="auto") -> np.ndarray:

@@ -27,10 +27,10 @@ ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 ARCH = os.environ.get("PRAMAAN_ARCH", "gfx950")
 
-HIP_SOURCES = ["engine.hip", "launchers.hip", "batch.hip", "gnn.hip", "learned.hip"]
+HIP_SOURCES = ["engine.hip", "launchers.hip", "batch.hip", "gnn.hip", "learned.hip", "presolve.hip"]
 HOST_SOURCES = ["host/gen.cpp", "host/gen_rolling.cpp", "host/gnn_features.cpp"]
 CXX_SOURCES = ["cxx/model_ir.cpp", "cxx/pdhg_solver.cpp", "cxx/backend_hip.cpp", "cxx/test_abi.cpp",
-               "cxx/certificate.cpp", "cxx/precision_ladder.cpp"]
+               "cxx/certificate.cpp", "cxx/precision_ladder.cpp", "cxx/presolve.cpp"]
 CXX_LIB = PKG / "lib" / "libpramaan_cxx.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
