@@ -23,10 +23,12 @@ windows), timed with HIP events on the batch's stream; algorithmic bytes
 per launch = sum over windows of 8 nnz + 4 (n+1) + 4 m + 28 n (A^T) or
 8 nnz + 4 (m+1) + 4 n + 20 m (A).  spmv_lp1: the same for the single-LP
 engine's fused SpMV on the 500k x 1M LP (12M nonzeros).
-cpu_baseline: the oracle's restatement of the reference PDHG (FP32, one
-thread) timed on window 0 for a bounded number of iterations; value =
-seconds per iteration x the iterations the cold GPU solve used over all
-windows.
+cpu_baseline: the oracle's restatement of the reference PDHG (FP32) timed
+on window 0 for a bounded number of iterations, on one thread and on all
+the host threads at once (one window each, as independent windows would
+run); value = the parallel group's wall seconds per iteration x the
+iterations the cold GPU solve used over all windows (single_thread: the
+same from one thread).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload lp1        (the single 500k x 1M LP instead)
@@ -84,6 +86,35 @@ def oracle_seconds_per_iteration(lp, budget_s: float):
     t0 = time.perf_counter()
     O.pdhg_solve(lp, tol=0.0, max_iterations=iters, check_frequency=10**9)
     return max((time.perf_counter() - t0 - setup) / iters, 1e-9), iters
+
+
+def cpu_threads() -> int:
+    """Host threads for the parallel CPU baseline: OMP_NUM_THREADS when set
+    (the box's CPU share), at most the visible cores."""
+    n = os.cpu_count() or 1
+    try:
+        return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+    except ValueError:
+        return n
+
+
+def oracle_parallel_seconds_per_iteration(lp, budget_s: float, threads: int, per_it_1: float):
+    """The same oracle PDHG on `threads` host threads at once, one LP each
+    (independent windows are the CPU's natural parallel unit: each thread
+    solves its own): wall seconds per iteration of the whole group, setup
+    subtracted, over a bounded run."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import binding as O
+    t0 = time.perf_counter()
+    O.pdhg_solve(lp, tol=0.0, max_iterations=0, check_frequency=10**9)
+    setup = time.perf_counter() - t0
+    iters = max(20, min(200000, int(budget_s / per_it_1)))
+    with ThreadPoolExecutor(threads) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(lambda _: O.pdhg_solve(lp, tol=0.0, max_iterations=iters, check_frequency=10**9),
+                    range(threads)))
+        wall = time.perf_counter() - t0
+    return max((wall - setup) / (threads * iters), 1e-12), iters
 
 
 def spmv_lp1(P, reps: int) -> dict:
@@ -210,13 +241,19 @@ def bench_rolling(args, P) -> dict:
     if not args.no_lp1:
         out["spmv_lp1"] = spmv_lp1(P, args.kernel_reps)
     if not args.no_cpu:
-        per_it, iters = oracle_seconds_per_iteration(lps[0], args.cpu_budget)
+        per_it, iters = oracle_seconds_per_iteration(lps[0], args.cpu_budget / 2)
+        nt = cpu_threads()
+        per_it_p, iters_p = oracle_parallel_seconds_per_iteration(lps[0], args.cpu_budget / 2, nt, per_it)
+        total = float(it_c.sum())
         out["cpu_baseline"] = {
-            "value": per_it * float(it_c.sum()), "unit": "s", "cores": 1, "kind": "port",
-            "iterations_per_s": 1.0 / per_it,
-            "sample": (f"{iters} iterations of the oracle's restated reference PDHG (FP32, 1 thread) on window 0, "
-                       f"setup subtracted; value = s/iteration x the {int(it_c.sum())} iterations the cold GPU "
-                       f"solve used over all {nw} windows")}
+            "value": per_it_p * total, "unit": "s", "cores": nt, "kind": "port",
+            "iterations_per_s": 1.0 / per_it_p,
+            "sample": (f"{nt} host threads each running {iters_p} iterations of the oracle's restated reference "
+                       f"PDHG (FP32) on window 0 at once, setup subtracted; value = wall s per iteration of the "
+                       f"group x the {int(total)} iterations the cold GPU solve used over all {nw} windows "
+                       f"(windows spread over the threads)"),
+            "single_thread": {"value": per_it * total, "cores": 1, "iterations_per_s": 1.0 / per_it,
+                              "sample": f"{iters} iterations on one thread, the same scaling"}}
     return out
 
 
