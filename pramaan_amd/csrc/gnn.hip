@@ -159,7 +159,10 @@ __global__ __launch_bounds__(256) void k_sum_partials(int nb, int width, const f
 // Out[i][:] (+)= sum_k val[k] H[idx[k]][:], one wavefront per row.  The
 // grid is 8 x per blocks; block b runs on XCD b % 8, which takes the
 // (b % 8)-th eighth of the rows in order, so the rows of a window (and the
-// H rows they gather) stay in one XCD's L2.
+// H rows they gather) stay in one XCD's L2.  Entries are taken four at a
+// time (indexes and values, then the four gathers, then the four FMAs in
+// entry order), so four gathers are in flight per wavefront and the sum is
+// the same as one entry at a time.
 template <bool ACC>
 __global__ __launch_bounds__(256) void k_spmm(int nrows, int per, const int* __restrict__ rp,
                                               const int* __restrict__ idx, const float* __restrict__ val,
@@ -168,7 +171,19 @@ __global__ __launch_bounds__(256) void k_spmm(int nrows, int per, const int* __r
     const int row = blk * 4 + (threadIdx.x >> 6), f = threadIdx.x & 63;
     if (row >= nrows) return;
     float s = 0.0f;
-    for (int k = rp[row]; k < rp[row + 1]; ++k) s = __builtin_fmaf(val[k], Hin[static_cast<size_t>(idx[k]) * kH + f], s);
+    const int k1 = rp[row + 1];
+    int k = rp[row];
+    for (; k + 4 <= k1; k += 4) {
+        int i[4];
+        float v[4], h[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { i[u] = idx[k + u]; v[u] = val[k + u]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) h[u] = Hin[static_cast<size_t>(i[u]) * kH + f];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s = __builtin_fmaf(v[u], h[u], s);
+    }
+    for (; k < k1; ++k) s = __builtin_fmaf(val[k], Hin[static_cast<size_t>(idx[k]) * kH + f], s);
     float* o = Out + static_cast<size_t>(row) * kH + f;
     if (ACC) *o += s;
     else *o = s;

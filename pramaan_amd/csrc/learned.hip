@@ -79,66 +79,116 @@ __global__ __launch_bounds__(kLB) void k_l_scales(const BatchDesc* __restrict__ 
     }
 }
 
-// Per LP: scaled values, global indices and features.
-template <class VT>
-__global__ __launch_bounds__(kLB) void k_l_graph(const BatchDesc* __restrict__ desc, const int* __restrict__ rp,
-                                                 const int* __restrict__ ci, const VT* __restrict__ oa,
-                                                 const int* __restrict__ cp, const int* __restrict__ ri,
-                                                 const VT* __restrict__ oat, const double* __restrict__ c,
-                                                 const double* __restrict__ lr, const double* __restrict__ ur,
-                                                 const double* __restrict__ lx, const double* __restrict__ ux,
-                                                 const double* __restrict__ scal, int period_cols, int period_rows,
-                                                 int* __restrict__ gci, float* __restrict__ ar,
-                                                 int* __restrict__ gri, float* __restrict__ ac,
-                                                 float* __restrict__ xr, float* __restrict__ xc) {
-    const BatchDesc d = desc[blockIdx.x];
-    const double s = scal[4 * blockIdx.x], cs = scal[4 * blockIdx.x + 1];
-    const double maxc = scal[4 * blockIdx.x + 2], maxr = scal[4 * blockIdx.x + 3];
-    const int m = d.m, n = d.n;
-    const int pc = period_cols > 0 ? max(1, n / period_cols - 1) : 1;
-    const int pr = period_rows > 0 ? max(1, m / period_rows - 1) : 1;
-    for (int i = threadIdx.x; i < m; i += kLB) {
-        const int b = rp[d.rp_off + i], e = rp[d.rp_off + i + 1];
-        double mx = 0.0;
-        for (int k = b; k < e; ++k) mx = fmax(mx, fabs(static_cast<double>(oa[d.nz_off + k])));
-        const double den = mx > 0.0 ? mx : 1.0;
-        for (int k = b; k < e; ++k) {
-            ar[d.nz_off + k] = static_cast<float>(static_cast<double>(oa[d.nz_off + k]) / den);
-            gci[d.nz_off + k] = static_cast<int>(d.col_off) + ci[d.nz_off + k];
-        }
-        const int64_t r = d.row_off + i;
-        const double lo = lr[r], hi = ur[r];
-        float* f = xr + r * kF;
-        f[0] = static_cast<float>(fin(lo) ? lo / s : 0.0);
-        f[1] = static_cast<float>(fin(hi) ? hi / s : 0.0);
-        f[2] = lo > -1e29 ? 1.0f : 0.0f;
-        f[3] = hi < 1e29 ? 1.0f : 0.0f;
-        f[4] = lo == hi ? 1.0f : 0.0f;
-        f[5] = static_cast<float>(static_cast<double>(e - b) / maxr);
-        f[6] = period_rows > 0 ? static_cast<float>(static_cast<double>(i / period_rows) / pr) : 0.0f;
-        f[7] = 1.0f;
+// LP that owns global row (COLS: column) q: the last LP whose first row
+// (column) is at most q.  Offsets grow with the LP index; an LP without
+// rows shares its offset with the next one and owns none of them.
+template <bool COLS>
+__device__ __forceinline__ int lp_of(const BatchDesc* __restrict__ desc, int count, int64_t q) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((COLS ? desc[mid].col_off : desc[mid].row_off) <= q)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
-    for (int j = threadIdx.x; j < n; j += kLB) {
-        const int b = cp[d.cp_off + j], e = cp[d.cp_off + j + 1];
-        double mx = 0.0;
-        for (int k = b; k < e; ++k) mx = fmax(mx, fabs(static_cast<double>(oat[d.nz_off + k])));
-        const double den = mx > 0.0 ? mx : 1.0;
-        for (int k = b; k < e; ++k) {
-            ac[d.nz_off + k] = static_cast<float>(static_cast<double>(oat[d.nz_off + k]) / den);
-            gri[d.nz_off + k] = static_cast<int>(d.row_off) + ri[d.nz_off + k];
-        }
-        const int64_t q = d.col_off + j;
-        const double lo = lx[q], hi = ux[q];
-        float* f = xc + q * kF;
+    return lo;
+}
+
+// The graph and features of every row (COLS: column) of the batch, L lanes
+// per row: the lanes take the row's largest |a| (a maximum, exact in any
+// order), write its entries scaled by it with their global indexes, and the
+// first lane writes the row's features.  Rows of A give the row nodes'
+// edges (ar, gci) and features (xr), rows of A^T the column nodes'.
+template <class VT, int L, bool COLS>
+__global__ __launch_bounds__(256) void k_l_side(int total, int count, const BatchDesc* __restrict__ desc,
+                                                const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                const VT* __restrict__ val, const double* __restrict__ c,
+                                                const double* __restrict__ lo_b, const double* __restrict__ hi_b,
+                                                const double* __restrict__ scal, int period,
+                                                int* __restrict__ gidx, float* __restrict__ a,
+                                                float* __restrict__ feat) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t q = t / L;
+    const int lane = static_cast<int>(t % L);
+    const bool on = q < total;
+    int k = 0, b = 0, e = 0;
+    int64_t nz = 0;
+    double mx = 0.0;
+    if (on) {
+        k = lp_of<COLS>(desc, count, q);
+        const int64_t local = q - (COLS ? desc[k].col_off : desc[k].row_off);
+        const int64_t po = (COLS ? desc[k].cp_off : desc[k].rp_off) + local;
+        nz = desc[k].nz_off;
+        b = ptr[po];
+        e = ptr[po + 1];
+        for (int kk = b + lane; kk < e; kk += L) mx = fmax(mx, fabs(static_cast<double>(val[nz + kk])));
+    }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, L));
+    if (!on) return;
+    const BatchDesc& d = desc[k];
+    const double den = mx > 0.0 ? mx : 1.0;
+    const int base = static_cast<int>(COLS ? d.row_off : d.col_off);
+    for (int kk = b + lane; kk < e; kk += L) {
+        a[nz + kk] = static_cast<float>(static_cast<double>(val[nz + kk]) / den);
+        gidx[nz + kk] = base + idx[nz + kk];
+    }
+    if (lane != 0) return;
+    const int i = static_cast<int>(q - (COLS ? d.col_off : d.row_off));
+    const double s = scal[4 * k], lo = lo_b[q], hi = hi_b[q];
+    float* f = feat + q * kF;
+    if constexpr (COLS) {
+        const double cs = scal[4 * k + 1], maxc = scal[4 * k + 2];
+        const int pc = period > 0 ? max(1, d.n / period - 1) : 1;
         f[0] = static_cast<float>(c[q] / cs);
         f[1] = static_cast<float>(fin(lo) ? lo / s : 0.0);
         f[2] = static_cast<float>(fin(hi) ? hi / s : 0.0);
         f[3] = lo > -1e29 ? 1.0f : 0.0f;
         f[4] = hi < 1e29 ? 1.0f : 0.0f;
         f[5] = static_cast<float>(static_cast<double>(e - b) / maxc);
-        f[6] = period_cols > 0 ? static_cast<float>(static_cast<double>(j / period_cols) / pc) : 0.0f;
-        f[7] = 1.0f;
+        f[6] = period > 0 ? static_cast<float>(static_cast<double>(i / period) / pc) : 0.0f;
+    } else {
+        const double maxr = scal[4 * k + 3];
+        const int pr = period > 0 ? max(1, d.m / period - 1) : 1;
+        f[0] = static_cast<float>(fin(lo) ? lo / s : 0.0);
+        f[1] = static_cast<float>(fin(hi) ? hi / s : 0.0);
+        f[2] = lo > -1e29 ? 1.0f : 0.0f;
+        f[3] = hi < 1e29 ? 1.0f : 0.0f;
+        f[4] = lo == hi ? 1.0f : 0.0f;
+        f[5] = static_cast<float>(static_cast<double>(e - b) / maxr);
+        f[6] = period > 0 ? static_cast<float>(static_cast<double>(i / period) / pr) : 0.0f;
     }
+    f[7] = 1.0f;
+}
+
+// Lanes per row for a mean row length: about two entries per lane.
+int side_lanes(double mean) {
+    int l = 1;
+    while (l < 32 && 2.0 * l < mean) l *= 2;
+    return l;
+}
+
+template <class VT, bool COLS>
+void launch_side(hipStream_t s, int lanes, int total, int count, const BatchDesc* desc, const int* ptr,
+                 const int* idx, const VT* val, const double* c, const double* lo, const double* hi,
+                 const double* scal, int period, int* gidx, float* a, float* feat) {
+    if (total <= 0) return;
+    auto go = [&](auto lc) {
+        constexpr int L = decltype(lc)::value;
+        const dim3 grid(static_cast<unsigned>((static_cast<int64_t>(total) * L + 255) / 256));
+        hipLaunchKernelGGL((k_l_side<VT, L, COLS>), grid, dim3(256), 0, s, total, count, desc, ptr, idx, val, c,
+                           lo, hi, scal, period, gidx, a, feat);
+    };
+    switch (lanes) {
+        case 1: go(std::integral_constant<int, 1>{}); break;
+        case 2: go(std::integral_constant<int, 2>{}); break;
+        case 4: go(std::integral_constant<int, 4>{}); break;
+        case 8: go(std::integral_constant<int, 8>{}); break;
+        case 16: go(std::integral_constant<int, 16>{}); break;
+        default: go(std::integral_constant<int, 32>{}); break;
+    }
+    PM_LAUNCHED();
 }
 
 // Predictions -> warm starts: x = clamp(xhat * s, l, u), y = yhat * cs
@@ -186,14 +236,16 @@ void learned_warm_start(pm_batch* b, pm_gnn* g, int period_cols, int period_rows
     PM_LAUNCHED();
     auto graph = [&](auto oa, auto oat) {
         using VT = std::remove_cv_t<std::remove_pointer_t<decltype(oa)>>;
-        hipLaunchKernelGGL(k_l_graph<VT>, dim3(count), dim3(kLB), 0, s, b->d_desc.get(), b->rp.get(), b->ci.get(),
-                           oa, b->cp.get(), b->ri.get(), oat, b->c_model_dev.get(), b->lr.get(), b->ur.get(),
-                           b->lx.get(), b->ux.get(), scal.get(), period_cols, period_rows, g->ci.get(), g->ar.get(),
-                           g->ri.get(), g->ac.get(), g->xr.get(), g->xc.get());
+        const BatchDesc* dd = b->d_desc.get();
+        launch_side<VT, false>(s, side_lanes(M > 0 ? static_cast<double>(Z) / M : 0.0), M, count, dd, b->rp.get(),
+                               b->ci.get(), oa, nullptr, b->lr.get(), b->ur.get(), scal.get(), period_rows,
+                               g->ci.get(), g->ar.get(), g->xr.get());
+        launch_side<VT, true>(s, side_lanes(N > 0 ? static_cast<double>(Z) / N : 0.0), N, count, dd, b->cp.get(),
+                              b->ri.get(), oat, b->c_model_dev.get(), b->lx.get(), b->ux.get(), scal.get(),
+                              period_cols, g->ri.get(), g->ac.get(), g->xc.get());
     };
     if (b->exact) graph(static_cast<const float*>(b->oa32.get()), static_cast<const float*>(b->oat32.get()));
     else graph(static_cast<const double*>(b->oa64.get()), static_cast<const double*>(b->oat64.get()));
-    PM_LAUNCHED();
     // The network runs on its own stream: order it after the features and
     // the warm-start writes after it.
     hipEvent_t ready = nullptr, done = nullptr;
