@@ -159,10 +159,10 @@ __global__ __launch_bounds__(256) void k_sum_partials(int nb, int width, const f
 // Out[i][:] (+)= sum_k val[k] H[idx[k]][:], one wavefront per row.  The
 // grid is 8 x per blocks; block b runs on XCD b % 8, which takes the
 // (b % 8)-th eighth of the rows in order, so the rows of a window (and the
-// H rows they gather) stay in one XCD's L2.  Entries are taken four at a
-// time (indexes and values, then the four gathers, then the four FMAs in
-// entry order), so four gathers are in flight per wavefront and the sum is
-// the same as one entry at a time.
+// H rows they gather) stay in one XCD's L2.  Entries are taken eight, then
+// four at a time (indexes and values, then the gathers, then the FMAs in
+// entry order), so up to eight gathers are in flight per wavefront and the
+// sum is the same as one entry at a time.
 template <bool ACC>
 __global__ __launch_bounds__(256) void k_spmm(int nrows, int per, const int* __restrict__ rp,
                                               const int* __restrict__ idx, const float* __restrict__ val,
@@ -173,6 +173,16 @@ __global__ __launch_bounds__(256) void k_spmm(int nrows, int per, const int* __r
     float s = 0.0f;
     const int k1 = rp[row + 1];
     int k = rp[row];
+    for (; k + 8 <= k1; k += 8) {
+        int i[8];
+        float v[8], h[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { i[u] = idx[k + u]; v[u] = val[k + u]; }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) h[u] = Hin[static_cast<size_t>(i[u]) * kH + f];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s = __builtin_fmaf(v[u], h[u], s);
+    }
     for (; k + 4 <= k1; k += 4) {
         int i[4];
         float v[4], h[4];
