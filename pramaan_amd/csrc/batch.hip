@@ -1352,8 +1352,9 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         }
         b->dr.resize(R);
         b->dc.resize(Cn);
-        ruiz_pc_device(s, opts.ruiz_iterations, static_cast<int>(ro), static_cast<int>(co), b->grp.get(), gci.get(),
-                       b->gcp.get(), gri.get(), vs.get(), vst.get(), b->dr.get(), b->dc.get(), rn.get(), cn.get());
+        ruiz_pc_device(s, opts.ruiz_iterations, static_cast<int>(ro), static_cast<int>(co), Z, b->grp.get(),
+                       gci.get(), b->gcp.get(), gri.get(), vs.get(), vst.get(), b->dr.get(), b->dc.get(), rn.get(),
+                       cn.get());
         b->va.resize(Z);
         b->vat.resize(Z);
         hipLaunchKernelGGL(k_to_f32, gz, blk, 0, s, static_cast<int64_t>(Z), vs.get(), b->va.get());

@@ -295,8 +295,8 @@ void device_ruiz_pc(pm_ctx* c, int passes, DevBuf<double>& vs, DevBuf<double>& v
     DevBuf<double> rn(std::max(m, 1)), cn(std::max(n, 1));
     dr.resize(std::max(m, 1));
     dc.resize(n);
-    ruiz_pc_device(c->stream, passes, m, n, c->rp.get(), c->ci.get(), c->cp.get(), c->ri.get(), vs.get(), vst.get(),
-                   dr.get(), dc.get(), rn.get(), cn.get());
+    ruiz_pc_device(c->stream, passes, m, n, c->nnz, c->rp.get(), c->ci.get(), c->cp.get(), c->ri.get(), vs.get(),
+                   vst.get(), dr.get(), dc.get(), rn.get(), cn.get());
     PM_LAUNCHED();
 }
 

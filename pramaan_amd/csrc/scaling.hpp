@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -35,16 +36,45 @@ __global__ __launch_bounds__(kBlock) void k_row_norm(int nrows, const int* __res
     out[i] = a > 0.0 ? 1.0 / sqrt(a) : 1.0;
 }
 
-// v[k] *= rs[row] * cs[col] over one CSR order; acc[row] *= rs[row].
+// v[k] *= rs[row] * cs[col] over one CSR order, L lanes per row striding
+// over its entries (each entry is scaled on its own, so the lanes change
+// nothing but the access pattern); acc[row] *= rs[row].
+template <int L>
 __global__ __launch_bounds__(kBlock) void k_row_scale(int nrows, const int* __restrict__ rp,
                                                       const int* __restrict__ col, double* __restrict__ v,
                                                       const double* __restrict__ rs, const double* __restrict__ cs,
                                                       double* __restrict__ acc) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const int i = static_cast<int>(t / L), lane = static_cast<int>(t % L);
     if (i >= nrows) return;
     const double r = rs[i];
-    for (int k = rp[i]; k < rp[i + 1]; ++k) v[k] *= r * cs[col[k]];
-    if (acc) acc[i] *= r;
+    for (int k = rp[i] + lane; k < rp[i + 1]; k += L) v[k] *= r * cs[col[k]];
+    if (acc && lane == 0) acc[i] *= r;
+}
+
+// Lanes per row of k_row_scale for a mean row length: about two entries per
+// lane.
+inline int scale_lanes(double mean) {
+    int l = 1;
+    while (l < 32 && 2.0 * l < mean) l *= 2;
+    return l;
+}
+
+inline void row_scale(hipStream_t s, int lanes, int nrows, const int* rp, const int* col, double* v, const double* rs,
+                      const double* cs, double* acc) {
+    auto go = [&](auto lc) {
+        constexpr int L = decltype(lc)::value;
+        const dim3 g(static_cast<unsigned>((static_cast<int64_t>(nrows) * L + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL(k_row_scale<L>, g, dim3(kBlock), 0, s, nrows, rp, col, v, rs, cs, acc);
+    };
+    switch (lanes) {
+        case 1: go(std::integral_constant<int, 1>{}); break;
+        case 2: go(std::integral_constant<int, 2>{}); break;
+        case 4: go(std::integral_constant<int, 4>{}); break;
+        case 8: go(std::integral_constant<int, 8>{}); break;
+        case 16: go(std::integral_constant<int, 16>{}); break;
+        default: go(std::integral_constant<int, 32>{}); break;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -59,12 +89,15 @@ __global__ __launch_bounds__(kBlock) void k_fill_f64(int64_t n, double* __restri
 
 inline dim3 grid_for(int64_t n) { return dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)); }
 
-// Scales the m x n matrix whose values are in vs (CSR rp / ci) and vst (CSC
-// cp / ri), both overwritten with the scaled values; dr[m], dc[n] receive
-// the factors; rn[m], cn[n] are workspace.
-inline void ruiz_pc_device(hipStream_t s, int passes, int m, int n, const int* rp, const int* ci, const int* cp,
-                           const int* ri, double* vs, double* vst, double* dr, double* dc, double* rn, double* cn) {
+// Scales the m x n matrix with nnz nonzeros whose values are in vs (CSR
+// rp / ci) and vst (CSC cp / ri), both overwritten with the scaled values;
+// dr[m], dc[n] receive the factors; rn[m], cn[n] are workspace.
+inline void ruiz_pc_device(hipStream_t s, int passes, int m, int n, int64_t nnz, const int* rp, const int* ci,
+                           const int* cp, const int* ri, double* vs, double* vst, double* dr, double* dc, double* rn,
+                           double* cn) {
     const dim3 b(kBlock);
+    const int la = scale_lanes(m > 0 ? static_cast<double>(nnz) / m : 0.0);
+    const int lt = scale_lanes(n > 0 ? static_cast<double>(nnz) / n : 0.0);
     if (m > 0) hipLaunchKernelGGL(k_fill_f64, grid_for(m), b, 0, s, static_cast<int64_t>(m), dr, 1.0);
     if (n > 0) hipLaunchKernelGGL(k_fill_f64, grid_for(n), b, 0, s, static_cast<int64_t>(n), dc, 1.0);
     for (int pass = 0; pass <= passes; ++pass) {
@@ -72,8 +105,8 @@ inline void ruiz_pc_device(hipStream_t s, int passes, int m, int n, const int* r
         if (m > 0) hipLaunchKernelGGL(k_row_norm, grid_for(m), b, 0, s, m, rp, vs, sum, rn);
         if (n > 0) hipLaunchKernelGGL(k_row_norm, grid_for(n), b, 0, s, n, cp, vst, sum, cn);
         if (m > 0 && n > 0) {
-            hipLaunchKernelGGL(k_row_scale, grid_for(m), b, 0, s, m, rp, ci, vs, rn, cn, dr);
-            hipLaunchKernelGGL(k_row_scale, grid_for(n), b, 0, s, n, cp, ri, vst, cn, rn, dc);
+            row_scale(s, la, m, rp, ci, vs, rn, cn, dr);
+            row_scale(s, lt, n, cp, ri, vst, cn, rn, dc);
         }
     }
 }
