@@ -88,6 +88,32 @@ def oracle_seconds_per_iteration(lp, budget_s: float):
     return max((time.perf_counter() - t0 - setup) / iters, 1e-9), iters
 
 
+def reference_residuals(lp, x, y, tol):
+    """The reference's convergence measures of (x, y) (ComputeFp64Residuals,
+    src/gpu/pdhg_solver.cu:107-161): pv = largest row or bound violation of
+    x in FP64; dv = largest reduced-cost violation c + A^T y (c and A rounded
+    to FP32 as the reference holds them, minimisation form) allowed by where
+    x sits within tol of its bounds, divided by max(1, |c|inf)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    rows = np.repeat(np.arange(lp.m), np.diff(lp.row_ptr))
+    act = np.zeros(lp.m)
+    np.add.at(act, rows, lp.values * x[lp.col_idx])
+    pv = max(0.0, float(np.max(lp.row_lower - act, initial=0.0)), float(np.max(act - lp.row_upper, initial=0.0)),
+             float(np.max(lp.var_lower - x, initial=0.0)), float(np.max(x - lp.var_upper, initial=0.0)))
+    c = ((-1.0 if lp.sense == 1 else 1.0) * lp.c).astype(np.float32).astype(np.float64)
+    aty = np.zeros(lp.n)
+    np.add.at(aty, lp.col_idx, lp.values.astype(np.float32).astype(np.float64) * y[rows])
+    rc = c + aty
+    at_lb = x <= lp.var_lower + tol
+    at_ub = x >= lp.var_upper - tol
+    viol = np.where(~at_lb & ~at_ub, np.abs(rc),
+                    np.where(at_lb & ~at_ub, np.maximum(0.0, -rc),
+                             np.where(at_ub & ~at_lb, np.maximum(0.0, rc), 0.0)))
+    dv = float(np.max(viol, initial=0.0)) / max(1.0, float(np.max(np.abs(c), initial=0.0)))
+    return pv, dv
+
+
 def cpu_threads() -> int:
     """Host threads for the parallel CPU baseline: OMP_NUM_THREADS when set
     (the box's CPU share), at most the visible cores."""
@@ -185,6 +211,12 @@ def bench_rolling(args, P) -> dict:
     fx = json.loads((ROOT / "tests" / "golden" / "fixtures.json").read_text())["rolling"]
     obj_err = max(abs(learned_res[int(w)].objective - v) / (1 + abs(v)) for w, v in fx.items() if int(w) < nw)
 
+    # the reference's pv / dv of every learned solution (one more learned
+    # solve, untimed, that also returns the vectors)
+    b.learned_warm_start(net, cpp, rpp)
+    vec_res, _, _ = b.run(want_vectors=True)
+    pvdv = np.array([reference_residuals(lp, r.x, r.y, args.tol) for lp, r in zip(lps, vec_res)])
+
     kern = {}
     for name, kid in (("batch_spmv_a_dual", P.PM_KERNEL_SPMV_A), ("batch_spmv_at_primal", P.PM_KERNEL_SPMV_AT)):
         ms, by = b.time_kernel(kid, args.kernel_reps)
@@ -213,6 +245,9 @@ def bench_rolling(args, P) -> dict:
                        "learned_mean": float(it_l.mean()), "learned_max": int(it_l.max())},
         "warm_starts_kept": int((acc == 1).sum()), "warm_starts_replaced_by_cold": int((acc == 0).sum()),
         "objective_rel_error_vs_highs": obj_err,
+        "reference_residuals_learned": {"pv_max": float(pvdv[:, 0].max()), "dv_max": float(pvdv[:, 1].max()),
+                                        "definition": "pdhg_solver.cu:107-161 (max over windows, tol = the "
+                                                      "tolerance)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "kernel_us": ms * 1e3, "alg_bytes_per_launch": by,
@@ -334,6 +369,8 @@ def bench_lp1(args, P) -> dict:
         "config": {"workload": f"single LP m={lp.m} n={lp.n} nnz={lp.nnz}, cold start to rel KKT {args.tol:g}",
                    "m": lp.m, "n": lp.n, "nnz": lp.nnz, "tolerance": args.tol},
         "status": res.status, "iterations": res.iterations, "rel_kkt": res.rel_kkt, "load_seconds": load_s,
+        "reference_residuals": {"pv": res.primal_residual, "dv": res.dual_residual,
+                                "definition": "pdhg_solver.cu:107-161"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "kernel_us": ms * 1e3,
                      "alg_bytes_per_launch": by},
