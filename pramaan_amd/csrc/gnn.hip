@@ -156,58 +156,90 @@ __global__ __launch_bounds__(256) void k_sum_partials(int nb, int width, const f
     out[e] = s;
 }
 
-// Out[i][:] (+)= sum_k val[k] H[idx[k]][:], one wavefront per row.  The
-// grid is 8 x per blocks; block b runs on XCD b % 8, which takes the
-// (b % 8)-th eighth of the rows in order, so the rows of a window (and the
-// H rows they gather) stay in one XCD's L2.  Entries are taken eight, then
-// four at a time (indexes and values, then the gathers, then the FMAs in
-// entry order), so up to eight gathers are in flight per wavefront and the
-// sum is the same as one entry at a time.
+// Out[i][:] (+)= sum_k val[k] H[idx[k]][:], eight lanes per row (eight
+// features each, two 16-byte loads), so a wavefront walks eight rows at
+// once.  The grid is 8 x per blocks of 32 rows; block b runs on XCD b % 8,
+// which takes the (b % 8)-th eighth of the rows in order, so the rows of a
+// window (and the H rows they gather) stay in one XCD's L2.  Entries are
+// taken eight at a time (indexes and values, then the gathers, then the
+// FMAs in entry order), so each feature's sum is the same as one entry at a
+// time and up to 64 gathers are in flight per wavefront.
 template <bool ACC>
 __global__ __launch_bounds__(256) void k_spmm(int nrows, int per, const int* __restrict__ rp,
                                               const int* __restrict__ idx, const float* __restrict__ val,
                                               const float* __restrict__ Hin, float* __restrict__ Out) {
     const int blk = (static_cast<int>(blockIdx.x) & 7) * per + (static_cast<int>(blockIdx.x) >> 3);
-    const int row = blk * 4 + (threadIdx.x >> 6), f = threadIdx.x & 63;
+    const int row = blk * 32 + (threadIdx.x >> 3), f = (threadIdx.x & 7) * 8;
     if (row >= nrows) return;
-    float s = 0.0f;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int k1 = rp[row + 1];
     int k = rp[row];
     for (; k + 8 <= k1; k += 8) {
         int i[8];
-        float v[8], h[8];
+        float v[8];
+        float4 a[8], b[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) { i[u] = idx[k + u]; v[u] = val[k + u]; }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) h[u] = Hin[static_cast<size_t>(i[u]) * kH + f];
+        for (int u = 0; u < 8; ++u) {
+            const float4* h = reinterpret_cast<const float4*>(Hin + static_cast<size_t>(i[u]) * kH + f);
+            a[u] = h[0];
+            b[u] = h[1];
+        }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s = __builtin_fmaf(v[u], h[u], s);
+        for (int u = 0; u < 8; ++u) {
+            s[0] = __builtin_fmaf(v[u], a[u].x, s[0]); s[1] = __builtin_fmaf(v[u], a[u].y, s[1]);
+            s[2] = __builtin_fmaf(v[u], a[u].z, s[2]); s[3] = __builtin_fmaf(v[u], a[u].w, s[3]);
+            s[4] = __builtin_fmaf(v[u], b[u].x, s[4]); s[5] = __builtin_fmaf(v[u], b[u].y, s[5]);
+            s[6] = __builtin_fmaf(v[u], b[u].z, s[6]); s[7] = __builtin_fmaf(v[u], b[u].w, s[7]);
+        }
     }
-    for (; k + 4 <= k1; k += 4) {
-        int i[4];
-        float v[4], h[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) { i[u] = idx[k + u]; v[u] = val[k + u]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) h[u] = Hin[static_cast<size_t>(i[u]) * kH + f];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) s = __builtin_fmaf(v[u], h[u], s);
+    for (; k < k1; ++k) {
+        const float4* h = reinterpret_cast<const float4*>(Hin + static_cast<size_t>(idx[k]) * kH + f);
+        const float4 a = h[0], b = h[1];
+        const float vk = val[k];
+        s[0] = __builtin_fmaf(vk, a.x, s[0]); s[1] = __builtin_fmaf(vk, a.y, s[1]);
+        s[2] = __builtin_fmaf(vk, a.z, s[2]); s[3] = __builtin_fmaf(vk, a.w, s[3]);
+        s[4] = __builtin_fmaf(vk, b.x, s[4]); s[5] = __builtin_fmaf(vk, b.y, s[5]);
+        s[6] = __builtin_fmaf(vk, b.z, s[6]); s[7] = __builtin_fmaf(vk, b.w, s[7]);
     }
-    for (; k < k1; ++k) s = __builtin_fmaf(val[k], Hin[static_cast<size_t>(idx[k]) * kH + f], s);
-    float* o = Out + static_cast<size_t>(row) * kH + f;
-    if (ACC) *o += s;
-    else *o = s;
+    float4* o = reinterpret_cast<float4*>(Out + static_cast<size_t>(row) * kH + f);
+    if (ACC) {
+        float4 t0 = o[0], t1 = o[1];
+        t0.x += s[0]; t0.y += s[1]; t0.z += s[2]; t0.w += s[3];
+        t1.x += s[4]; t1.y += s[5]; t1.z += s[6]; t1.w += s[7];
+        o[0] = t0;
+        o[1] = t1;
+    } else {
+        o[0] = make_float4(s[0], s[1], s[2], s[3]);
+        o[1] = make_float4(s[4], s[5], s[6], s[7]);
+    }
 }
 
-// Input layer: Out = relu(X W + b), X [N][8], W [8][64] FP32.
+// Input layer: Out = relu(X W + b), X [N][8], W [8][64] FP32.  Sixteen
+// lanes per row, four outputs each (one 16-byte store), so a wavefront
+// writes four rows.
 __global__ __launch_bounds__(256) void k_input(int N, const float* __restrict__ X, const float* __restrict__ W,
                                                const float* __restrict__ b, float* __restrict__ Out) {
-    const int t = blockIdx.x * 256 + threadIdx.x, r = t >> 6, c = t & 63;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t r = t >> 4;
+    const int c = static_cast<int>(t & 15) * 4;
     if (r >= N) return;
-    float s = b[c];
+    const float4 x0 = *reinterpret_cast<const float4*>(X + r * kF);
+    const float4 x1 = *reinterpret_cast<const float4*>(X + r * kF + 4);
+    const float xs[kF] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+    const float4 b4 = *reinterpret_cast<const float4*>(b + c);
+    float s[4] = {b4.x, b4.y, b4.z, b4.w};
 #pragma unroll
-    for (int k = 0; k < kF; ++k) s = __builtin_fmaf(X[static_cast<size_t>(r) * kF + k], W[k * kH + c], s);
-    Out[static_cast<size_t>(r) * kH + c] = fmaxf(s, 0.0f);
+    for (int k = 0; k < kF; ++k) {
+        const float4 w = *reinterpret_cast<const float4*>(W + k * kH + c);
+        s[0] = __builtin_fmaf(xs[k], w.x, s[0]);
+        s[1] = __builtin_fmaf(xs[k], w.y, s[1]);
+        s[2] = __builtin_fmaf(xs[k], w.z, s[2]);
+        s[3] = __builtin_fmaf(xs[k], w.w, s[3]);
+    }
+    *reinterpret_cast<float4*>(Out + r * kH + c) =
+        make_float4(fmaxf(s[0], 0.0f), fmaxf(s[1], 0.0f), fmaxf(s[2], 0.0f), fmaxf(s[3], 0.0f));
 }
 
 // Head: out[r] = H[r] . w + b.
@@ -395,7 +427,7 @@ void dense(pm_gnn* g, int N, const float* A1, const short* B1, const float* A2, 
 void spmm(pm_gnn* g, int nrows, const int* rp, const int* idx, const float* val, const float* Hin, float* out,
           bool acc) {
     if (nrows <= 0) return;
-    const int per = cdiv(cdiv(nrows, 4), 8);
+    const int per = cdiv(cdiv(nrows, 32), 8);
     const dim3 grid(8 * per);
     if (acc) hipLaunchKernelGGL(k_spmm<true>, grid, dim3(256), 0, g->s, nrows, per, rp, idx, val, Hin, out);
     else hipLaunchKernelGGL(k_spmm<false>, grid, dim3(256), 0, g->s, nrows, per, rp, idx, val, Hin, out);
@@ -464,10 +496,10 @@ void forward(pm_gnn* g) {
     const int m = g->m, n = g->n;
     const float* P = g->p.get();
     if (n > 0)
-        hipLaunchKernelGGL(k_input, dim3(cdiv(static_cast<long long>(n) * kH, 256)), dim3(256), 0, g->s, n,
+        hipLaunchKernelGGL(k_input, dim3(cdiv(static_cast<long long>(n) * 16, 256)), dim3(256), 0, g->s, n,
                            g->xc.get(), P + kOff.We_c, P + kOff.be_c, g->Hc[0].get());
     if (m > 0)
-        hipLaunchKernelGGL(k_input, dim3(cdiv(static_cast<long long>(m) * kH, 256)), dim3(256), 0, g->s, m,
+        hipLaunchKernelGGL(k_input, dim3(cdiv(static_cast<long long>(m) * 16, 256)), dim3(256), 0, g->s, m,
                            g->xr.get(), P + kOff.We_r, P + kOff.be_r, g->Hr[0].get());
     PM_LAUNCHED();
     for (int l = 1; l <= 2; ++l) {
