@@ -57,14 +57,17 @@ void pm_gen_free(pm_gen_lp* g);
 
 /* ---- rolling-horizon blending LPs --------------------------------------
  * A T-period blending plan (24 components, 15 products, 8 quality
- * attributes, 10 processing units; 400 columns and 194 rows per period,
- * periods linked by inventory-balance rows).  Window w is the LP of periods
- * [w, w + window) with the exogenous opening stock of period w; column
- * tau*400 + j and row tau*194 + i of a window are column j / row i of its
- * period tau.  The default horizon is 523 periods, i.e. 500 windows of 24.
- * demand_noise / blend_noise (with perturb_seed) scale every demand and
- * blend coefficient by U(1 - noise, 1 + noise).  Windows carry no closed-
- * form optimum (x_star / y_star views are empty, p_star is NaN). */
+ * attributes, 10 processing units).  A period has 400 columns and 215
+ * rows: a 200-row block (120 maximum- and 21 minimum-quality rows, 24
+ * supply, 10 capacity, 10 market rows, then 15 demand rows after the
+ * balance rows) and 15 inventory-balance rows that link it to the period
+ * before.  Window w is the LP of periods [w, w + window) with the
+ * exogenous opening stock of period w; column tau*400 + j and row
+ * tau*215 + i of a window are column j / row i of its period tau.  The
+ * default horizon is 523 periods, i.e. 500 windows of 24.  demand_noise
+ * and blend_noise (with perturb_seed) scale every demand and blend
+ * coefficient by U(1 - noise, 1 + noise).  Windows carry no closed-form
+ * optimum (x_star / y_star views are empty, p_star is NaN). */
 typedef struct pm_rh_params {
     int32_t periods_total;
     int32_t window;

@@ -1,12 +1,15 @@
 // gen_rolling.cpp -- rolling-horizon multi-period blending LPs (see the
 // pm_rh_* declarations in pramaan_gen.h).
 //
-// One period (400 columns, 194 rows):
+// One period (400 columns; a 200-row block plus 15 inventory-balance rows
+// that link it to the period before, 215 rows in all):
 //   columns  f[c][p]  c < 24 components, p < 15 products   (360, c-major)
 //            s[p]     sales                                  (15)
 //            I[p]     end-of-period product inventory        (15)
 //            o[u]     overtime of processing unit u < 10     (10)
 //   rows     quality  sum_c (a[c][q] - qmax[p][q]) f[c][p] <= 0   (15 x 8 = 120)
+//            minimum  sum_c (a[c][q] - qmin[p][q]) f[c][p] >= 0   (7 x 3 = 21:
+//                     products p < 7, attributes q < 3)
 //            supply   sum_p f[c][p] <= avail[c][t]                 (24)
 //            capacity sum_{p in unit u} sum_c f[c][p] - o[u] <= cap[u]  (10)
 //            market   sum_{p in unit u} s[p] <= market[u]                 (10)
@@ -18,9 +21,10 @@
 //   bounds   f, s >= 0; 0 <= I <= store[p]; 0 <= o <= otmax[u].
 // Blend coefficients a ~ U(0.5, 1.5); demand d = base_p (1 + 0.4 sin(2 pi t
 // / 168 + phase_p) + N(0, 0.1)), clipped at 0; qmax[p][q] >= the average
-// blend + 0.01 so the uniform blend is always quality-feasible, and the
-// zero plan (I = opening stock) satisfies every row: every window is
-// feasible, and bounded because supply, demand and the bounds cap it.
+// blend + 0.01, and qmin[p][q] = the average blend times U(0.98, 1.02), so
+// about half the minimum rows bind where a product is made.  The zero plan
+// (I = opening stock) satisfies every row: every window is feasible, and
+// bounded because supply, demand and the bounds cap it.
 // Perturbations (demand_noise, blend_noise) scale d and a per entry by
 // U(1 - noise, 1 + noise) from their own random stream.
 #include <algorithm>
@@ -34,8 +38,10 @@
 namespace {
 
 constexpr int C = 24, P = 15, Q = 8, U = 10;
+constexpr int PMIN = 7, QMIN = 3;                         // minimum-quality rows
 constexpr int kCols = C * P + P + P + U;                  // 400
-constexpr int kRows = P * Q + C + U + U + P + P;          // 194
+constexpr int kBlock = P * Q + PMIN * QMIN + C + U + U + P;  // 200
+constexpr int kRows = kBlock + P;                         // 215 with the balance rows
 
 struct Rng {  // xoshiro256** seeded by SplitMix64, as in gen.cpp
     uint64_t s[4];
@@ -77,6 +83,7 @@ struct pm_rh {
     // per period
     std::vector<double> a;        // [T][C][Q] blend coefficients
     std::vector<double> qmax;     // [P][Q]
+    std::vector<double> qmin;     // [PMIN][QMIN]
     std::vector<double> avail;    // [T][C]
     std::vector<double> demand;   // [T][P]
     std::vector<double> opening;  // [T][P] exogenous opening stock of a window starting at t
@@ -145,6 +152,17 @@ int pm_rh_create(const pm_rh_params* prm, pm_rh** out) {
                 for (int c = 0; c < C; ++c) avg += a0[c * Q + q];
                 avg /= C;
                 h->qmax[p * Q + q] = std::max(r.uniform(1.0, 1.3), avg * 1.03 * (1.0 + prm->blend_noise) + 0.01);
+            }
+        // Minimum specifications come from a stream of their own, so every
+        // other draw of the plan is the same with or without them.
+        Rng rq(prm->seed ^ 0x9A11C0DEull);
+        h->qmin.resize(PMIN * QMIN);
+        for (int p = 0; p < PMIN; ++p)
+            for (int q = 0; q < QMIN; ++q) {
+                double avg = 0.0;
+                for (int c = 0; c < C; ++c) avg += a0[c * Q + q];
+                avg /= C;
+                h->qmin[p * QMIN + q] = avg * rq.uniform(0.98, 1.02);
             }
         for (int t = 0; t < T; ++t) {
             const double season = 2.0 * 3.141592653589793 * t / 168.0;
@@ -215,6 +233,12 @@ int pm_rh_window(const pm_rh* h, int32_t start, pm_gen_lp** out) {
                 for (int cc = 0; cc < C; ++cc)
                     put(cb + col_f(cc, p), h->a[(static_cast<size_t>(t) * C + cc) * Q + q] - h->qmax[p * Q + q]);
                 row_end(-PM_INFINITY_VALUE, 0.0, i++);
+            }
+        for (int p = 0; p < PMIN; ++p)  // minimum quality
+            for (int q = 0; q < QMIN; ++q) {
+                for (int cc = 0; cc < C; ++cc)
+                    put(cb + col_f(cc, p), h->a[(static_cast<size_t>(t) * C + cc) * Q + q] - h->qmin[p * QMIN + q]);
+                row_end(0.0, PM_INFINITY_VALUE, i++);
             }
         for (int cc = 0; cc < C; ++cc) {  // supply
             for (int p = 0; p < P; ++p) put(cb + col_f(cc, p), 1.0);
