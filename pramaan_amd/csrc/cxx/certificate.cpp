@@ -112,7 +112,9 @@ std::string fmt(double v) {
 
 Certificate generate_certificate(const ModelIR& model, const std::vector<double>& x, const std::string& status,
                                  int ledger_entries) {
-    if (static_cast<int>(x.size()) != model.numVars()) throw std::invalid_argument("certificate: x size != num_vars");
+    if (static_cast<int>(x.size()) != model.numVars())
+        throw std::invalid_argument("certificate: x has " + std::to_string(x.size()) + " entries for " +
+                                    std::to_string(model.numVars()) + " columns");
     Certificate c;
     c.status = status;
     c.num_rows = model.numRows();
@@ -131,7 +133,9 @@ Certificate generate_certificate(const ModelIR& model, const std::vector<double>
 Certificate generate_certificate_kkt(const ModelIR& model, const std::vector<double>& x,
                                      const std::vector<double>& y, const std::string& status, int ledger_entries,
                                      double tolerance) {
-    if (static_cast<int>(y.size()) != model.numRows()) throw std::invalid_argument("certificate: y size != num_rows");
+    if (static_cast<int>(y.size()) != model.numRows())
+        throw std::invalid_argument("certificate: y has " + std::to_string(y.size()) + " entries for " +
+                                    std::to_string(model.numRows()) + " rows");
     Certificate c = generate_certificate(model, x, status, ledger_entries);
     c.format_id = "PRAMAAN_CERTIFICATE_V2";
     c.tolerance = tolerance;
@@ -147,7 +151,7 @@ Certificate generate_certificate_kkt(const ModelIR& model, const std::vector<dou
 // writer (src/certificate/certificate.cpp:113-153); V2 adds Y ... END_Y.
 void write_certificate(const Certificate& c, const std::string& path) {
     std::ofstream out(path);
-    if (!out.is_open()) throw std::runtime_error("Cannot open certificate file for writing: " + path);
+    if (!out.is_open()) throw std::runtime_error("certificate: " + path + " could not be created");
     out << std::setprecision(std::numeric_limits<double>::max_digits10);
     auto opt = [&](const char* key, const std::optional<double>& v) {
         out << key << " ";
@@ -177,7 +181,7 @@ void write_certificate(const Certificate& c, const std::string& path) {
         out << "END_Y\n";
     }
     out << "END_CERTIFICATE\n";
-    if (!out.good()) throw std::runtime_error("Error writing certificate file: " + path);
+    if (!out.good()) throw std::runtime_error("certificate: writing " + path + " failed");
 }
 
 // The reference reader's grammar and rejections (certificate.cpp:159-287):
@@ -185,32 +189,33 @@ void write_certificate(const Certificate& c, const std::string& path) {
 // match COLS (ROWS), nothing but blank lines after END_CERTIFICATE.
 Certificate read_certificate(const std::string& path) {
     std::ifstream in(path);
-    if (!in.is_open()) throw std::runtime_error("Cannot open certificate file: " + path);
+    const auto bad = [&path](const std::string& why) { throw std::runtime_error("certificate " + path + ": " + why); };
+    if (!in.is_open()) bad("cannot be read");
     std::string line;
     auto next = [&]() -> std::string {
-        if (!std::getline(in, line)) throw std::runtime_error("Unexpected end of certificate file");
+        if (!std::getline(in, line)) bad("ends before END_CERTIFICATE");
         if (!line.empty() && line.back() == '\r') line.pop_back();
         return line;
     };
-    auto value = [](const std::string& l, const std::string& key) {
-        if (l.compare(0, key.size(), key) != 0) throw std::runtime_error("Expected '" + key + "', got: " + l);
+    auto value = [&](const std::string& l, const std::string& key) {
+        if (l.compare(0, key.size(), key) != 0) bad("line \"" + l + "\" where the " + key + "field belongs");
         return l.substr(key.size());
     };
-    auto num = [](const std::string& s, const std::string& key) {
-        if (s.empty()) throw std::runtime_error("Empty value for " + key);
+    auto num = [&](const std::string& s, const std::string& key) {
+        if (s.empty()) bad(key + "has no value");
         size_t pos = 0;
         const double d = std::stod(s, &pos);
         while (pos < s.size() && std::isspace(static_cast<unsigned char>(s[pos]))) ++pos;
-        if (pos != s.size()) throw std::runtime_error("Trailing garbage in " + key + ": " + s);
-        if (!std::isfinite(d)) throw std::runtime_error("Certificate contains NaN/Inf for " + key + ": " + s);
+        if (pos != s.size()) bad(key + "value \"" + s + "\" is not a number alone");
+        if (!std::isfinite(d)) bad(key + "value \"" + s + "\" is not finite");
         return d;
     };
-    auto integer = [](const std::string& s, const std::string& key) {
-        if (s.empty()) throw std::runtime_error("Empty value for " + key);
+    auto integer = [&](const std::string& s, const std::string& key) {
+        if (s.empty()) bad(key + "has no value");
         size_t pos = 0;
         const int i = std::stoi(s, &pos);
         while (pos < s.size() && std::isspace(static_cast<unsigned char>(s[pos]))) ++pos;
-        if (pos != s.size()) throw std::runtime_error("Trailing garbage in " + key + ": " + s);
+        if (pos != s.size()) bad(key + "value \"" + s + "\" is not an integer alone");
         return i;
     };
     auto optional = [&](const std::string& s, const std::string& key) -> std::optional<double> {
@@ -218,18 +223,18 @@ Certificate read_certificate(const std::string& path) {
         return num(s, key);
     };
     Certificate c;
-    if (!std::getline(in, line)) throw std::runtime_error("Certificate file is empty");
+    if (!std::getline(in, line)) bad("is empty");
     if (!line.empty() && line.back() == '\r') line.pop_back();
     c.format_id = line;
     if (c.format_id != "PRAMAAN_CERTIFICATE_V1" && c.format_id != "PRAMAAN_CERTIFICATE_V2")
-        throw std::runtime_error("Unknown certificate format: " + c.format_id);
+        bad("format \"" + c.format_id + "\" is neither V1 nor V2");
     c.status = value(next(), "STATUS ");
     c.num_rows = integer(value(next(), "ROWS "), "ROWS ");
     c.num_cols = integer(value(next(), "COLS "), "COLS ");
-    if (c.num_rows < 0 || c.num_cols < 0) throw std::runtime_error("Negative dimension");
+    if (c.num_rows < 0 || c.num_cols < 0) bad("ROWS and COLS must not be negative");
     c.objective_value = num(value(next(), "OBJECTIVE "), "OBJECTIVE ");
     c.tolerance = num(value(next(), "TOLERANCE "), "TOLERANCE ");
-    if (c.tolerance <= 0.0) throw std::runtime_error("Zero or negative TOLERANCE");
+    if (c.tolerance <= 0.0) bad("TOLERANCE must be positive");
     c.primal_residual = num(value(next(), "PRIMAL_RESIDUAL "), "PRIMAL_RESIDUAL ");
     c.dual_residual = optional(value(next(), "DUAL_RESIDUAL "), "DUAL_RESIDUAL ");
     c.complementarity_residual = optional(value(next(), "COMPLEMENTARITY_RESIDUAL "), "COMPLEMENTARITY_RESIDUAL ");
@@ -238,27 +243,27 @@ Certificate read_certificate(const std::string& path) {
     c.objective_bound_gap = optional(value(next(), "OBJECTIVE_BOUND_GAP "), "OBJECTIVE_BOUND_GAP ");
     c.model_fingerprint = value(next(), "MODEL_FINGERPRINT ");
     c.ledger_entries = integer(value(next(), "LEDGER_ENTRIES "), "LEDGER_ENTRIES ");
-    if (c.ledger_entries < 0) throw std::runtime_error("Negative LEDGER_ENTRIES");
-    if (next() != "X") throw std::runtime_error("Expected 'X', got: " + line);
+    if (c.ledger_entries < 0) bad("LEDGER_ENTRIES must not be negative");
+    if (next() != "X") bad("line \"" + line + "\" where the X block starts");
     for (;;) {
         const std::string l = next();
         if (l == "END_X") break;
         c.x.push_back(num(l, "X element"));
     }
-    if (static_cast<int>(c.x.size()) != c.num_cols) throw std::runtime_error("X vector size mismatch with COLS");
+    if (static_cast<int>(c.x.size()) != c.num_cols) bad("X has " + std::to_string(c.x.size()) + " values, COLS says " + std::to_string(c.num_cols));
     if (c.format_id == "PRAMAAN_CERTIFICATE_V2") {
-        if (next() != "Y") throw std::runtime_error("Expected 'Y', got: " + line);
+        if (next() != "Y") bad("line \"" + line + "\" where the Y block starts");
         for (;;) {
             const std::string l = next();
             if (l == "END_Y") break;
             c.y.push_back(num(l, "Y element"));
         }
-        if (static_cast<int>(c.y.size()) != c.num_rows) throw std::runtime_error("Y vector size mismatch with ROWS");
+        if (static_cast<int>(c.y.size()) != c.num_rows) bad("Y has " + std::to_string(c.y.size()) + " values, ROWS says " + std::to_string(c.num_rows));
     }
-    if (next() != "END_CERTIFICATE") throw std::runtime_error("Expected 'END_CERTIFICATE', got: " + line);
+    if (next() != "END_CERTIFICATE") bad("line \"" + line + "\" where END_CERTIFICATE belongs");
     std::string extra;
     if (std::getline(in, extra) && extra.find_first_not_of(" \r\n\t") != std::string::npos)
-        throw std::runtime_error("Unexpected extra data after END_CERTIFICATE");
+        bad("text follows END_CERTIFICATE");
     return c;
 }
 

@@ -20,19 +20,30 @@ ModelIR::ModelIR(ObjSense s, double off, std::vector<double> c, CSRMatrix A_in, 
 }
 
 void ModelIR::validate() const {
-    const auto n = static_cast<std::size_t>(numVars()), m = static_cast<std::size_t>(numRows());
-    if (obj_coeffs.size() != n) throw std::invalid_argument("ModelIR: obj_coeffs size != num_vars");
-    if (var_lower.size() != n || var_upper.size() != n)
-        throw std::invalid_argument("ModelIR: var_lower/var_upper size != num_vars");
-    if (var_types.size() != n) throw std::invalid_argument("ModelIR: var_types size != num_vars");
-    if (var_names.size() != n) throw std::invalid_argument("ModelIR: var_names size != num_vars");
-    if (row_lower.size() != m || row_upper.size() != m)
-        throw std::invalid_argument("ModelIR: row_lower/row_upper size != num_rows");
-    if (row_names.size() != m) throw std::invalid_argument("ModelIR: row_names size != num_rows");
-    for (std::size_t j = 0; j < n; ++j)
-        if (var_lower[j] > var_upper[j]) throw std::invalid_argument("ModelIR: var_lower > var_upper for some variable");
-    for (std::size_t r = 0; r < m; ++r)
-        if (row_lower[r] > row_upper[r]) throw std::invalid_argument("ModelIR: row_lower > row_upper for some row");
+    // Every per-variable and per-row array must match the matrix, and no
+    // lower bound may exceed its upper bound.
+    const std::size_t n = static_cast<std::size_t>(numVars()), m = static_cast<std::size_t>(numRows());
+    const auto length = [](const char* what, std::size_t got, std::size_t want, const char* per) {
+        if (got != want)
+            throw std::invalid_argument(std::string("ModelIR: ") + what + " holds " + std::to_string(got) +
+                                        " entries for " + std::to_string(want) + " " + per);
+    };
+    length("the objective", obj_coeffs.size(), n, "columns");
+    length("the variable lower bounds", var_lower.size(), n, "columns");
+    length("the variable upper bounds", var_upper.size(), n, "columns");
+    length("the variable types", var_types.size(), n, "columns");
+    length("the variable names", var_names.size(), n, "columns");
+    length("the row lower bounds", row_lower.size(), m, "rows");
+    length("the row upper bounds", row_upper.size(), m, "rows");
+    length("the row names", row_names.size(), m, "rows");
+    const auto crossed = [](const std::vector<double>& lo, const std::vector<double>& hi, const char* what) {
+        for (std::size_t k = 0; k < lo.size(); ++k)
+            if (lo[k] > hi[k])
+                throw std::invalid_argument(std::string("ModelIR: ") + what + " " + std::to_string(k) +
+                                            " has its lower bound above its upper bound");
+    };
+    crossed(var_lower, var_upper, "variable");
+    crossed(row_lower, row_upper, "row");
 }
 
 CSRMatrix CSRMatrix::transpose() const {
