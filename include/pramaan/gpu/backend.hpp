@@ -22,6 +22,6 @@ public:
     virtual SolveResult solve_lp_relaxation(const ModelIR& model, const RevisedSimplex::Options& options) = 0;
 };
 
-std::unique_ptr<Backend> make_hip_backend();
-
 }  // namespace pramaan
+
+#include "pramaan/gpu/hip_backend.hpp"  // make_hip_backend()

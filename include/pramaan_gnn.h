@@ -12,7 +12,7 @@
  * oracle/gnn_oracle.py, which the tests check the kernels against.
  *
  * Parameters are one flat FP32 vector in this order:
- *   We_c[8][64] be_c[64] We_r[8][64] be_r[64]
+ *   We_c[16][64] be_c[64] We_r[16][64] be_r[64]
  *   for l = 1, 2: Wrr[64][64] Wrz[64][64] br[64] Wcc[64][64] Wcz[64][64] bc[64]
  *   wx[64] bx[1] wy[64] by[1]
  * (row-major; W[k][c] multiplies input feature k into output feature c).
@@ -27,7 +27,8 @@ extern "C" {
 #endif
 
 #define PM_GNN_HIDDEN 64
-#define PM_GNN_FEATURES 8
+#define PM_GNN_FEATURES 16
+#define PM_GNN_ROLE_BITS 9
 
 struct pm_lp;
 
@@ -36,15 +37,17 @@ struct pm_lp;
  *                              column scaled to unit max-norm;
  *   ar[nnz]                    A's values (CSR order of lp) with each row
  *                              scaled to unit max-norm;
- *   xc[n][8] column features   c/cs, l/s, u/s (0 when infinite), l finite,
- *                              u finite, nnz/max nnz, period position, 1;
- *   xr[m][8] row features      l/s, u/s, l finite, u finite, equality,
- *                              nnz/max nnz, period position, 1;
+ *   xc[n][16] column features  c/cs, l/s, u/s (0 when infinite), l finite,
+ *                              u finite, nnz/max nnz, period position, then
+ *                              the 9 role bits;
+ *   xr[m][16] row features     l/s, u/s, l finite, u finite, equality,
+ *                              nnz/max nnz, period position, role bits;
  *   scales[2] = {s, cs}        s = max(1, largest finite |bound|),
  *                              cs = max(1e-12, max |c|).
  * The period position is (index / period) / (count / period - 1) when a
- * period length is given (> 0), else 0.  Targets of the network are x / s
- * and y / cs. */
+ * period length is given (> 0), else 0.  The role bits say which node of
+ * its period a node is: bit b (b < 9) of (index mod period) as -1 or +1,
+ * all 0 without a period.  Targets of the network are x / s and y / cs. */
 int pm_gnn_lp_features(const struct pm_lp* lp, int32_t period_cols,
                        int32_t period_rows, int32_t* cp, int32_t* ri,
                        float* ar, float* ac, float* xr, float* xc,

@@ -68,7 +68,12 @@ extern "C" {
 #define PM_ALG_REFERENCE 0
 #define PM_ALG_PDLP      1
 
-/* ---- termination test --------------------------------------------------- */
+/* ---- termination test ---------------------------------------------------
+ * PM_ALG_REFERENCE accepts PM_TERM_REFERENCE only.  PM_ALG_PDLP under
+ * PM_TERM_REFERENCE stops when the relative KKT error and the reference's
+ * pv and dv all meet the tolerance (a check whose relative error meets it
+ * while pv or dv do not tightens the relative target tenfold).  Other
+ * pairs fail with PM_ERR_INVALID_ARG. */
 #define PM_TERM_REFERENCE    0 /* pv <= tol && dv <= tol (pdhg_solver.cu:356) */
 #define PM_TERM_RELATIVE_KKT 1 /* max(rel primal, rel dual, rel gap) <= tol   */
 
@@ -126,7 +131,9 @@ typedef struct pm_pdhg_options {
     int32_t device;            /* HIP device ordinal (0)                     */
     int32_t verbose;           /* 0: silent; 1: one line per check to stderr */
     double  primal_weight;     /* PDLP: initial primal weight, <= 0 = auto   */
-    const double* x0;          /* optional warm start [n], original space    */
+    const double* x0;          /* optional warm start [n], original space
+                                  (PM_ALG_PDLP only: the reference
+                                  algorithm rejects a warm start)          */
     const double* y0;          /* optional warm start [m], reference sign    */
 } pm_pdhg_options;
 

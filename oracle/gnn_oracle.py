@@ -22,7 +22,7 @@ from __future__ import annotations
 import numpy as np
 
 H = 64
-DIN = 8
+DIN = 16  # node features: 7 from the LP and 9 role bits (include/pramaan_gnn.h)
 LAYER_NAMES = ["Wrr", "Wrz", "br", "Wcc", "Wcz", "bc"]
 
 

@@ -468,6 +468,10 @@ class Gnn:
 
     def set_params(self, p) -> None:
         p = np.ascontiguousarray(p, np.float32)
+        want = lib().pm_gnn_num_params()
+        if p.ndim != 1 or p.size != want:
+            raise ValueError(f"parameter vector of {p.size} values; this network has {want} "
+                             f"(weights saved for a different feature layout?)")
         _check(lib().pm_gnn_set_params(self._h, _p(p, C.c_float)))
 
     def set_graph(self, g: dict) -> None:

@@ -81,7 +81,7 @@ def build_cxx() -> Path:
     """The C++ interface layer (include/pramaan/...) over the C-ABI."""
     srcs = [str(CSRC / s) for s in CXX_SOURCES]
     tmp = CXX_LIB.with_suffix(".so.tmp")
-    _run([CXX, *CXX_FLAGS, "-shared", *srcs, "-o", str(tmp), f"-L{LIB.parent}", "-lpramaan_hip",
+    _run([CXX, *CXX_FLAGS, "-DPRAMAAN_ENABLE_HIP", "-shared", *srcs, "-o", str(tmp), f"-L{LIB.parent}", "-lpramaan_hip",
           "-Wl,-rpath,$ORIGIN"], "libpramaan_cxx")
     os.replace(tmp, CXX_LIB)
     return CXX_LIB
@@ -98,7 +98,7 @@ def build_reference_oracle() -> bool:
     mk = ORACLE_DIR / "Makefile"
     if not Path("/root/reference").is_dir() or not mk.exists() or shutil.which("make") is None:
         return False
-    _run(["make", "-s", "-C", str(ORACLE_DIR), "-j8", "ref", "reftests"], "oracle/_ref")
+    _run(["make", "-B", "-s", "-C", str(ORACLE_DIR), "-j8", "ref", "reftests"], "oracle/_ref")
     return True
 
 

@@ -13,16 +13,14 @@
 #include <string>
 #include <vector>
 
+#include "abi_bridge.hpp"
 #include "pramaan/gpu/pdhg_solver.hpp"
 #include "pramaan_hip.h"
 
 namespace pramaan {
-namespace gpu {
+namespace hip_abi {
 
-PdhgSolver::PdhgSolver(const PdhgOptions& opts) : options_(opts) {}
-PdhgSolver::~PdhgSolver() {}
-
-SolveResult PdhgSolver::solve(const ModelIR& model) {
+SolveResult solve(const ModelIR& model, const pm_pdhg_options& o, std::vector<double>* y_out, double* kkt_out) {
     const int m = static_cast<int>(model.numRows());
     const int n = static_cast<int>(model.numVars());
     pm_lp lp{};
@@ -41,21 +39,6 @@ SolveResult PdhgSolver::solve(const ModelIR& model) {
     lp.var_upper = model.var_upper.data();
     const std::vector<CSRMatrix::Index> empty_rp(1, 0);
     if (model.A.rowPtr().empty()) lp.row_ptr = empty_rp.data();
-
-    pm_pdhg_options o;
-    pm_pdhg_options_default(&o);
-    o.tolerance = options_.tolerance;
-    o.max_iterations = options_.max_iterations;
-    o.check_frequency = options_.check_frequency;
-#ifdef PRAMAAN_HIP_EXTENDED_OPTIONS
-    if (options_.pdlp) o.algorithm = PM_ALG_PDLP;
-    if (options_.relative_kkt) o.termination = PM_TERM_RELATIVE_KKT;
-    o.spmv_variant = options_.spmv_variant;
-    o.max_refinements = options_.max_refinements;
-    o.device = options_.device;
-    if (static_cast<int>(options_.x0.size()) == n && n > 0) o.x0 = options_.x0.data();
-    if (static_cast<int>(options_.y0.size()) == m && m > 0) o.y0 = options_.y0.data();
-#endif
 
     SolveResult result;
     result.x.assign(static_cast<size_t>(n), 0.0);
@@ -84,11 +67,38 @@ SolveResult PdhgSolver::solve(const ModelIR& model) {
     result.objective_value = r.objective;
     result.primal_residual = r.primal_residual;
     result.dual_residual = r.dual_residual;
-#ifdef PRAMAAN_HIP_EXTENDED_OPTIONS
-    last_y_ = std::move(y);
-    last_kkt_ = r.rel_kkt;
-#endif
+    if (y_out) *y_out = std::move(y);
+    if (kkt_out) *kkt_out = r.rel_kkt;
     return result;
+}
+
+}  // namespace hip_abi
+
+namespace gpu {
+
+PdhgSolver::PdhgSolver(const PdhgOptions& opts) : options_(opts) {}
+PdhgSolver::~PdhgSolver() {}
+
+SolveResult PdhgSolver::solve(const ModelIR& model) {
+    pm_pdhg_options o;
+    pm_pdhg_options_default(&o);
+    o.tolerance = options_.tolerance;
+    o.max_iterations = options_.max_iterations;
+    o.check_frequency = options_.check_frequency;
+#ifdef PRAMAAN_HIP_EXTENDED_OPTIONS
+    const int m = static_cast<int>(model.numRows());
+    const int n = static_cast<int>(model.numVars());
+    if (options_.pdlp) o.algorithm = PM_ALG_PDLP;
+    if (options_.relative_kkt) o.termination = PM_TERM_RELATIVE_KKT;
+    o.spmv_variant = options_.spmv_variant;
+    o.max_refinements = options_.max_refinements;
+    o.device = options_.device;
+    if (static_cast<int>(options_.x0.size()) == n && n > 0) o.x0 = options_.x0.data();
+    if (static_cast<int>(options_.y0.size()) == m && m > 0) o.y0 = options_.y0.data();
+    return hip_abi::solve(model, o, &last_y_, &last_kkt_);
+#else
+    return hip_abi::solve(model, o, nullptr, nullptr);
+#endif
 }
 
 }  // namespace gpu

@@ -4,6 +4,11 @@
 // returned only when the status is optimal and both reference-style
 // residuals are within the threshold; otherwise the first failing condition
 // names the escalation reason.
+//
+// PRAMAAN_ENABLE_HIP (set by the build of the GPU library) compiles the GPU
+// rungs in.  Without it the controller needs no GPU code at all: it
+// escalates at once with kGpuDisabled to the CPU polish, as the reference
+// does when built without CUDA (src/gpu/precision_ladder.cpp:156-173).
 #include "pramaan/gpu/precision_ladder.hpp"
 
 #include <algorithm>
@@ -34,6 +39,7 @@ using Clock = std::chrono::steady_clock;
 
 double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
 
+#ifdef PRAMAAN_ENABLE_HIP
 // The gate: kNone when r passes, else the first reason it fails.
 PrecisionLadderResult::EscalationReason gate(const SolveResult& r, double T) {
     using R = PrecisionLadderResult::EscalationReason;
@@ -91,6 +97,7 @@ void fp64_residuals(const ModelIR& model, const std::vector<double>& x, const st
     dv /= c_norm;
 }
 #endif
+#endif  // PRAMAAN_ENABLE_HIP
 
 }  // namespace
 
@@ -99,6 +106,8 @@ PrecisionLadderResult PrecisionLadder::solve(const ModelIR& model) const {
     using Reason = PrecisionLadderResult::EscalationReason;
     PrecisionLadderResult result;
     const auto t_start = Clock::now();
+    SolveResult best;
+#ifdef PRAMAAN_ENABLE_HIP
     const double T = options_.residual_threshold;
 
     // Rung 1: GPU FP32 PDHG (reference algorithm).
@@ -107,7 +116,6 @@ PrecisionLadderResult PrecisionLadder::solve(const ModelIR& model) const {
     o1.max_iterations = options_.gpu_max_iterations;
     o1.check_frequency = options_.gpu_check_frequency;
     auto t = Clock::now();
-    SolveResult best;
     {
         gpu::PdhgSolver s1(o1);
         best = s1.solve(model);
@@ -166,12 +174,18 @@ PrecisionLadderResult PrecisionLadder::solve(const ModelIR& model) const {
         if (r2.status == SolveStatus::kOptimal || best.status != SolveStatus::kOptimal) best = std::move(r2);
     }
 #endif
+#else
+    // Built without the GPU engine: no GPU rung runs.
+    result.escalated = true;
+    result.escalation_reason = Reason::kGpuDisabled;
+    best.status = SolveStatus::kIterationLimit;
+#endif
 
     // Rung 3: CPU FP64 polish seeded with the GPU x.
     RevisedSimplex::Options cpu_opts = options_.cpu_options;
     if (static_cast<int>(best.x.size()) == model.numVars()) cpu_opts.primal_start_hint = best.x;
     result.stage = Stage::kCpuFp64Polish;
-    t = Clock::now();
+    const auto t_cpu = Clock::now();
     if (options_.cpu_polish) {
         result.final_result = options_.cpu_polish(model, cpu_opts);
     } else {
@@ -180,11 +194,12 @@ PrecisionLadderResult PrecisionLadder::solve(const ModelIR& model) const {
         result.final_result = cpu.solve(model);
 #else
         result.polish_unavailable = true;
-        result.stage = result.refined_rung_run ? Stage::kGpuFp64Refined : Stage::kGpuFp32;
+        if (result.escalation_reason != Reason::kGpuDisabled)
+            result.stage = result.refined_rung_run ? Stage::kGpuFp64Refined : Stage::kGpuFp32;
         result.final_result = std::move(best);
 #endif
     }
-    result.cpu_elapsed_ms = ms_since(t);
+    result.cpu_elapsed_ms = ms_since(t_cpu);
     result.total_elapsed_ms = ms_since(t_start);
     return result;
 }

@@ -15,8 +15,10 @@
 //    (power iteration), KKT-driven adaptive restarts (0.2 / 0.8 / 0.36), a
 //    primal-weight update at each restart, and an FP64 iterative-refinement
 //    outer loop: each round solves, in FP32, the correction LP of the FP64
-//    residuals of the current point, amplified by 1/KKT.  Termination and
-//    every decision use the FP64 relative KKT error of the original LP.
+//    residuals of the current point, amplified by 1/KKT.  Every decision
+//    uses the FP64 relative KKT error of the original LP; it is also the
+//    termination test under PM_TERM_RELATIVE_KKT, while PM_TERM_REFERENCE
+//    further requires the reference's pv, dv <= tolerance.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -300,17 +302,30 @@ void device_ruiz_pc(pm_ctx* c, int passes, DevBuf<double>& vs, DevBuf<double>& v
     PM_LAUNCHED();
 }
 
-void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
-    check_lp(lp);
-    pm_pdhg_options opts;
-    pm_pdhg_options_default(&opts);
-    if (o) opts = *o;
+// Host-side checks of the options (no device work).
+void check_options(const pm_pdhg_options& opts) {
     if (opts.check_frequency <= 0) fail(PM_ERR_INVALID_ARG, "check_frequency must be positive");
     if (opts.max_iterations < 0) fail(PM_ERR_INVALID_ARG, "max_iterations must be non-negative");
     if (opts.spmv_variant < 0 || opts.spmv_variant >= PM_SPMV_NUM_VARIANTS)
         fail(PM_ERR_INVALID_ARG, "unknown spmv_variant");
     if (opts.algorithm != PM_ALG_REFERENCE && opts.algorithm != PM_ALG_PDLP)
         fail(PM_ERR_INVALID_ARG, "unknown algorithm");
+    if (opts.termination != PM_TERM_REFERENCE && opts.termination != PM_TERM_RELATIVE_KKT)
+        fail(PM_ERR_INVALID_ARG, "unknown termination");
+    // The reference algorithm is the reference's solver unchanged: its own
+    // test and its own starting point (pdhg_solver.cu:288-297,356).
+    if (opts.algorithm == PM_ALG_REFERENCE && opts.termination != PM_TERM_REFERENCE)
+        fail(PM_ERR_INVALID_ARG, "PM_ALG_REFERENCE stops on the reference test only; relative KKT needs PM_ALG_PDLP");
+    if (opts.algorithm == PM_ALG_REFERENCE && (opts.x0 || opts.y0))
+        fail(PM_ERR_INVALID_ARG, "PM_ALG_REFERENCE starts from x = clamp(0, l, u), y = 0; warm starts need PM_ALG_PDLP");
+}
+
+void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
+    check_lp(lp);
+    pm_pdhg_options opts;
+    pm_pdhg_options_default(&opts);
+    if (o) opts = *o;
+    check_options(opts);
     c->opts = opts;
     c->loaded = false;
     const int m = lp->m, n = lp->n, nnz = lp->nnz;
@@ -779,6 +794,11 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
     Control ctl;
     ctl.init(o.tolerance, o.max_iterations, o.max_refinements, c->omega0, !c->y0_h.empty() && m > 0);
     res->status = PM_SOLVE_ITERATION_LIMIT;
+    // Under PM_TERM_REFERENCE a point is optimal only when the reference's
+    // own measures agree (pv, dv <= tolerance, pdhg_solver.cu:356): each
+    // check that meets the relative target but not them tightens the
+    // target tenfold and the solve goes on.
+    const bool ref_test = o.termination == PM_TERM_REFERENCE;
     const auto t0 = std::chrono::steady_clock::now();
     bool have_candidate = false;
     bool round_start = true;
@@ -805,7 +825,16 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
         if (o.verbose)
             fprintf(stderr, "[pramaan] it %d round %d kkt %.3e (p %.2e d %.2e g %.2e) omega %.3g restarts %d\n",
                     ctl.total_it, ctl.rounds, kk.rel, kk.rel_p, kk.rel_d, kk.rel_g, ctl.omega, ctl.restarts);
-        if (act == Control::OPTIMAL) { res->status = PM_SOLVE_OPTIMAL; break; }
+        if (act == Control::OPTIMAL) {
+            double pv = 0.0, dv = 0.0;
+            if (ref_test) ref_style_residuals(c, c->xtot.get(), c->ytot.get(), o.tolerance, pv, dv);
+            if (!ref_test || (pv <= o.tolerance && dv <= o.tolerance)) {
+                res->status = PM_SOLVE_OPTIMAL;
+                break;
+            }
+            ctl.tol = std::max(ctl.tol * 0.1, 1e-14);
+            continue;
+        }
         if (act == Control::FAIL) { res->status = PM_SOLVE_NUMERICAL_FAILURE; break; }
         if (act == Control::LIMIT) break;
         if (act == Control::REFINE) {
@@ -972,9 +1001,20 @@ int pm_ctx_solve(pm_ctx* c, pm_pdhg_result* res) {
 }
 
 int pm_pdhg_solve(const pm_lp* lp, const pm_pdhg_options* opts, pm_pdhg_result* res) {
+    // The LP and the options are checked on the host before a device is
+    // touched, so bad arguments fail the same way with or without a GPU.
+    int rc = pm::guarded([&] {
+        pm::check_lp(lp);
+        pm_pdhg_options o;
+        pm_pdhg_options_default(&o);
+        if (opts) o = *opts;
+        pm::check_options(o);
+        if (!res) pm::fail(PM_ERR_INVALID_ARG, "result is NULL");
+    });
+    if (rc != PM_OK) return rc;
     pm_ctx* c = nullptr;
     const int dev = opts ? opts->device : 0;
-    int rc = pm_ctx_create(dev, &c);
+    rc = pm_ctx_create(dev, &c);
     if (rc != PM_OK) return rc;
     rc = pm_ctx_load(c, lp, opts);
     if (rc == PM_OK) rc = pm_ctx_solve(c, res);

@@ -24,7 +24,8 @@
 namespace pm {
 namespace {
 
-constexpr int kH = 64, kF = 8;
+constexpr int kH = 64, kF = PM_GNN_FEATURES;
+static_assert(kF % 4 == 0, "features are loaded four at a time");
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(256) void k_wgrad(int N, int rows_per, const float*
 }
 
 // Partial column sums over row chunks: P[b][c] = sum_r D[r][c] (width 64),
-// and optionally of X^T D for the 8-feature input layer: P2[b][k][c].
+// and optionally of X^T D for the kF-feature input layer: P2[b][k][c].
 __global__ __launch_bounds__(256) void k_colsum(int N, int rows_per, const float* __restrict__ D,
                                                 const float* __restrict__ X, float* __restrict__ P,
                                                 float* __restrict__ P2) {
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(256) void k_spmm(int nrows, int per, const int* __r
     }
 }
 
-// Input layer: Out = relu(X W + b), X [N][8], W [8][64] FP32.  Sixteen
+// Input layer: Out = relu(X W + b), X [N][kF], W [kF][64] FP32.  Sixteen
 // lanes per row, four outputs each (one 16-byte store), so a wavefront
 // writes four rows.
 __global__ __launch_bounds__(256) void k_input(int N, const float* __restrict__ X, const float* __restrict__ W,
@@ -225,9 +226,15 @@ __global__ __launch_bounds__(256) void k_input(int N, const float* __restrict__ 
     const int64_t r = t >> 4;
     const int c = static_cast<int>(t & 15) * 4;
     if (r >= N) return;
-    const float4 x0 = *reinterpret_cast<const float4*>(X + r * kF);
-    const float4 x1 = *reinterpret_cast<const float4*>(X + r * kF + 4);
-    const float xs[kF] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+    float xs[kF];
+#pragma unroll
+    for (int q = 0; q < kF / 4; ++q) {
+        const float4 x4 = *reinterpret_cast<const float4*>(X + r * kF + 4 * q);
+        xs[4 * q] = x4.x;
+        xs[4 * q + 1] = x4.y;
+        xs[4 * q + 2] = x4.z;
+        xs[4 * q + 3] = x4.w;
+    }
     const float4 b4 = *reinterpret_cast<const float4*>(b + c);
     float s[4] = {b4.x, b4.y, b4.z, b4.w};
 #pragma unroll

@@ -20,13 +20,16 @@
 //   objective (min)  sum cost[c] f - price[p] s + hold[p] I + otcost[u] o
 //   bounds   f, s >= 0; 0 <= I <= store[p]; 0 <= o <= otmax[u].
 // Blend coefficients a ~ U(0.5, 1.5); demand d = base_p (1 + 0.4 sin(2 pi t
-// / 168 + phase_p) + N(0, 0.1)), clipped at 0; qmax[p][q] >= the average
-// blend + 0.01, and qmin[p][q] = the average blend times U(0.98, 1.02), so
+// / 168 + phase_p) + N(0, 0.1)), clipped at 0; qmax[p][q] >= 1.03 x the
+// average blend + 0.01, and qmin[p][q] = the average blend times
+// U(0.98, 1.02), both from the unperturbed coefficients, so
 // about half the minimum rows bind where a product is made.  The zero plan
 // (I = opening stock) satisfies every row: every window is feasible, and
 // bounded because supply, demand and the bounds cap it.
 // Perturbations (demand_noise, blend_noise) scale d and a per entry by
-// U(1 - noise, 1 + noise) from their own random stream.
+// U(1 - noise, 1 + noise) from their own random stream (perturb_seed); the
+// rest of the plan depends on seed alone, so a perturbed plan is the
+// unperturbed one with its demands and blend coefficients moved.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -151,7 +154,7 @@ int pm_rh_create(const pm_rh_params* prm, pm_rh** out) {
                 double avg = 0.0;
                 for (int c = 0; c < C; ++c) avg += a0[c * Q + q];
                 avg /= C;
-                h->qmax[p * Q + q] = std::max(r.uniform(1.0, 1.3), avg * 1.03 * (1.0 + prm->blend_noise) + 0.01);
+                h->qmax[p * Q + q] = std::max(r.uniform(1.0, 1.3), avg * 1.03 + 0.01);
             }
         // Minimum specifications come from a stream of their own, so every
         // other draw of the plan is the same with or without them.

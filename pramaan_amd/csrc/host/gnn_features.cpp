@@ -1,6 +1,6 @@
 // gnn_features.cpp -- graph and features of one LP for the warm-start
 // network (pramaan_gnn.h): the row / column max-norm scaled matrix in CSR
-// and CSC order and the 8 column and 8 row features.  Host code; a few
+// and CSC order and the 16 column and 16 row features.  Host code; a few
 // tenths of a millisecond per rolling-horizon window, so training batches
 // can be rebuilt from the LP every step instead of being stored.
 #include <algorithm>
@@ -12,6 +12,10 @@
 
 namespace {
 inline bool fin(double v) { return std::fabs(v) < 1e29; }
+// Bits of a node's index within its period as -1 / +1; all 0 for role < 0.
+inline void role_bits(float* f, int role) {
+    for (int b = 0; b < PM_GNN_ROLE_BITS; ++b) f[b] = role < 0 ? 0.0f : (((role >> b) & 1) ? 1.0f : -1.0f);
+}
 }  // namespace
 
 extern "C" int pm_gnn_lp_features(const pm_lp* lp, int32_t period_cols, int32_t period_rows, int32_t* cp,
@@ -77,7 +81,7 @@ extern "C" int pm_gnn_lp_features(const pm_lp* lp, int32_t period_cols, int32_t 
         f[4] = hi < 1e29 ? 1.0f : 0.0f;
         f[5] = static_cast<float>(static_cast<double>(collen[j]) / maxc);
         f[6] = period_cols > 0 ? static_cast<float>(static_cast<double>(j / period_cols) / pc) : 0.0f;
-        f[7] = 1.0f;
+        role_bits(f + 7, period_cols > 0 ? j % period_cols : -1);
     }
     for (int i = 0; i < m; ++i) {
         const double lo = lp->row_lower[i], hi = lp->row_upper[i];
@@ -89,7 +93,7 @@ extern "C" int pm_gnn_lp_features(const pm_lp* lp, int32_t period_cols, int32_t 
         f[4] = lo == hi ? 1.0f : 0.0f;
         f[5] = static_cast<float>(static_cast<double>(rp[i + 1] - rp[i]) / maxr);
         f[6] = period_rows > 0 ? static_cast<float>(static_cast<double>(i / period_rows) / pr) : 0.0f;
-        f[7] = 1.0f;
+        role_bits(f + 7, period_rows > 0 ? i % period_rows : -1);
     }
     return PM_OK;
 }

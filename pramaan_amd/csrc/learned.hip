@@ -159,7 +159,10 @@ __global__ __launch_bounds__(256) void k_l_side(int total, int count, const Batc
         f[5] = static_cast<float>(static_cast<double>(e - b) / maxr);
         f[6] = period > 0 ? static_cast<float>(static_cast<double>(i / period) / pr) : 0.0f;
     }
-    f[7] = 1.0f;
+    const int role = period > 0 ? i % period : -1;  // bits of the node's index within its period
+#pragma unroll
+    for (int bit = 0; bit < PM_GNN_ROLE_BITS; ++bit)
+        f[7 + bit] = role < 0 ? 0.0f : (((role >> bit) & 1) ? 1.0f : -1.0f);
 }
 
 // Lanes per row for a mean row length: about two entries per lane.

@@ -36,6 +36,7 @@ from . import _capi as P
 from . import warmstart as W
 
 TOL = 1e-6
+HELD_OUT = 1000  # perturbation seeds from here on never enter a training set
 
 
 def shift(v: np.ndarray, per: int) -> np.ndarray:
@@ -156,12 +157,15 @@ class TrainingSet:
         return len(self.index)
 
 
-def training_set(seeds, windows_per_plan=500, tol=1e-5, demand_noise=0.2, blend_noise=0.1, log=print):
-    """Solves the windows of perturbed plans (seed 1000 + s, perturbation
-    seed s) cold in batches and keeps the optimal ones as targets."""
+def training_set(seeds, windows_per_plan=500, tol=1e-5, demand_noise=0.2, blend_noise=0.1, base_seed=1, log=print):
+    """Solves the windows of perturbed plans cold in batches and keeps the
+    optimal ones as targets.  Plan s is the base plan (seed base_seed, the
+    measured sequence's) with every demand scaled by U(1 -+ demand_noise)
+    and every blend coefficient by U(1 -+ blend_noise) under perturbation
+    seed s; seeds >= HELD_OUT are kept for evaluation."""
     ts = TrainingSet()
     for sd in seeds:
-        rh = P.RollingHorizon(seed=1000 + sd, demand_noise=demand_noise, blend_noise=blend_noise, perturb_seed=sd)
+        rh = P.RollingHorizon(seed=base_seed, demand_noise=demand_noise, blend_noise=blend_noise, perturb_seed=sd)
         pi = len(ts.plans)
         ts.plans.append(rh)
         nw = min(windows_per_plan, rh.windows)
@@ -300,20 +304,25 @@ def main(argv=None) -> int:
 
 def evaluate(net, nwin=500, tol=TOL, previous=0, log=print) -> dict:
     """Cold vs learned (and optionally sequential previous-window) on the
-    unperturbed 500-window sequence."""
-    rh = P.RollingHorizon()
-    lps = [rh.window(w) for w in range(min(nwin, rh.windows))]
-    graphs = [W.graph_of(lp, rh.cols_per_period, rh.rows_per_period) for lp in lps]
+    unperturbed 500-window sequence, and cold vs learned on a perturbed
+    sequence whose perturbation seed no training set uses."""
     out = {}
-    with P.Batch(lps, tolerance=tol, max_iterations=200000, check_frequency=64) as b:
-        cold, _ = solve_cold(lps, tol, batch=b)
-        learned, _ = solve_learned(lps, graphs, net, tol, batch=b)
-    out["cold"] = cold.summary()
-    out["learned"] = learned.summary()
-    out["speedup_learned_vs_cold"] = cold.seconds / learned.seconds
-    if previous:
-        prev, _ = solve_previous(lps[:previous], rh.cols_per_period, rh.rows_per_period, tol)
-        out["previous"] = prev.summary()
+    for name, kw in (("nominal", {}), ("held_out", dict(demand_noise=0.2, blend_noise=0.1, perturb_seed=HELD_OUT))):
+        rh = P.RollingHorizon(**kw)
+        lps = [rh.window(w) for w in range(min(nwin, rh.windows))]
+        with P.Batch(lps, tolerance=tol, max_iterations=200000, check_frequency=64) as b:
+            cold, _ = solve_cold(lps, tol, batch=b)
+            for i in range(len(lps)):
+                b.set_warm_start(i, None, None)
+            infer = b.learned_warm_start(net, rh.cols_per_period, rh.rows_per_period)
+            res, secs, inst = b.run()
+            acc, _ = b.warm_start_verdicts()
+            learned = _stats(res, secs + infer, inst, infer, int((acc == 0).sum()))
+        out[name] = dict(cold=cold.summary(), learned=learned.summary(),
+                         speedup_learned_vs_cold=cold.seconds / learned.seconds)
+        if previous and name == "nominal":
+            prev, _ = solve_previous(lps[:previous], rh.cols_per_period, rh.rows_per_period, tol)
+            out[name]["previous"] = prev.summary()
     log(json.dumps(out))
     return out
 

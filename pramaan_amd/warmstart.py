@@ -2,13 +2,17 @@
 the mapping between network outputs and PDHG starting points.
 
 The network itself (pm_gnn_*, HIP) sees one LP as a bipartite graph:
-  column features (8): c_j / max|c|, l_j / s, u_j / s (0 when infinite),
+  column features (16): c_j / max|c|, l_j / s, u_j / s (0 when infinite),
                        1[l finite], 1[u finite], nnz_j / max nnz,
-                       position of the column's period in the window, 1
-  row features (8):    l_i / s, u_i / s (0 when infinite), 1[l finite],
+                       position of the column's period in the window,
+                       role bits
+  row features (16):   l_i / s, u_i / s (0 when infinite), 1[l finite],
                        1[u finite], 1[equality], nnz_i / max nnz,
-                       period position, 1
-with s = max(1, largest finite |bound|).  Targets are x_j / s and
+                       period position, role bits
+with s = max(1, largest finite |bound|).  The role bits are the 9 bits of
+the node's index within its period as -1 / +1 (0 without a period): every
+window of a plan shares its period layout, so they say which component,
+product or row kind a node is.  Targets are x_j / s and
 y_i / max|c| (y in the engine's sign).  Several LPs are batched as one
 block-diagonal graph.
 """
@@ -18,7 +22,7 @@ import numpy as np
 
 from ._capi import INF, LP
 
-FEAT = 8
+FEAT = 16
 
 
 def graph_of(lp: LP, period_cols: int | None = None, period_rows: int | None = None) -> dict:

@@ -30,11 +30,18 @@ def features_numpy(lp, period_cols, period_rows):
     fin = lambda a: np.where(np.abs(a) < 1e29, a, 0.0)
     cpos = (np.arange(n) // period_cols) / max(1, n // period_cols - 1) if period_cols else np.zeros(n)
     rpos = (np.arange(m) // period_rows) / max(1, m // period_rows - 1) if period_rows else np.zeros(m)
+    def bits(count, period):
+        if not period:
+            return np.zeros((count, 9))
+        role = np.arange(count) % period
+        return np.stack([np.where((role >> b) & 1, 1.0, -1.0) for b in range(9)], axis=1)
     xc = np.stack([lp.c / cs, fin(lp.var_lower) / s, fin(lp.var_upper) / s, (lp.var_lower > -1e29) * 1.0,
-                   (lp.var_upper < 1e29) * 1.0, collen / max(1, collen.max(initial=0)), cpos, np.ones(n)], axis=1)
+                   (lp.var_upper < 1e29) * 1.0, collen / max(1, collen.max(initial=0)), cpos], axis=1)
     xr = np.stack([fin(lp.row_lower) / s, fin(lp.row_upper) / s, (lp.row_lower > -1e29) * 1.0,
                    (lp.row_upper < 1e29) * 1.0, (lp.row_lower == lp.row_upper) * 1.0,
-                   rowlen / max(1, rowlen.max(initial=0)), rpos, np.ones(m)], axis=1)
+                   rowlen / max(1, rowlen.max(initial=0)), rpos], axis=1)
+    xc = np.concatenate([xc, bits(n, period_cols)], axis=1)
+    xr = np.concatenate([xr, bits(m, period_rows)], axis=1)
     return dict(ar=ar, cp=cp, ri=ri, ac=ac, xr=xr, xc=xc, s=s, cs=cs)
 
 
