@@ -12,9 +12,13 @@
 //      (reflected Halpern, relative-KKT termination at 1e-2 x the
 //      threshold), warm-started from rung 1's x, judged by the same gate on
 //      the reference residuals recomputed in FP64 with the threshold as the
-//      activity tolerance.  It turns a GPU
-//      result into the answer where the reference would fall back to the
-//      CPU.
+//      activity tolerance; a failed gate re-solves from the rung's own
+//      (x, y) at a 100x tighter target, at most three solves in all.  It
+//      turns a GPU result into the answer where the reference would fall
+//      back to the CPU.
+//   Built without PRAMAAN_ENABLE_HIP, no GPU rung exists: the ladder
+//   escalates at once with kGpuDisabled, as the reference built without
+//   CUDA does.
 //   3. CPU FP64 polish with the best GPU x as primal_start_hint:
 //      Options::cpu_polish when set; otherwise RevisedSimplex when the
 //      library is built inside the PRAMAAN tree (PRAMAAN_LADDER_CPU_SIMPLEX);

@@ -1045,7 +1045,7 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
         if (tid == 0) rel = kkt_finish(keep, red, d);
         if (tid == 0) {
             ctl.advance(chunk);
-            act_s = ctl.on_check(rel);
+            act_s = ctl.on_check(rel, d.rel_p, d.rel_d, d.rel_g);
             st.have_cand = 1;
         }
         __syncthreads();

@@ -176,6 +176,7 @@ struct pm_ctx {
     // Host FP64 model in minimisation form (c negated for a maximisation).
     std::vector<double> c_model, cmin, lr, ur, lx, ux;
     std::vector<double> x0_h, y0_h;
+    int warm_accepted = -1;  // last solve: 1 warm start used, 0 replaced by the cold start, -1 none
 
     // Device structure: A (rp, ci) and A^T (cp, ri).
     DevBuf<int> rp, ci, cp, ri;
@@ -791,8 +792,32 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
     if (!c->y0_h.empty() && m > 0) c->ybase.upload(c->y0_h, s);
     else launch_elem(std::max(m, 1), FnFill<double>{c->ybase.get(), 0.0}, s);
 
+    // A warm start is judged by its FP64 relative KKT error against the
+    // cold start's (x = 0 clipped to the bounds, y = 0): the worse of the
+    // two is not used.
+    const bool warm_given = !c->x0_h.empty() || (!c->y0_h.empty() && m > 0);
+    bool warm = warm_given;
+    if (warm_given) {
+        launch_elem(n, FnCopy<double>{c->xtot.get(), c->xbase.get()}, s);
+        if (m > 0) launch_elem(m, FnCopy<double>{c->ytot.get(), c->ybase.get()}, s);
+        const double rw = eval_kkt(c, c->xtot.get(), c->ytot.get()).rel;
+        std::vector<double> xc(n);
+        for (int j = 0; j < n; ++j) xc[j] = c->lx[j] > 0.0 ? c->lx[j] : (c->ux[j] < 0.0 ? c->ux[j] : 0.0);
+        c->xtot.upload(xc, s);
+        if (m > 0) launch_elem(m, FnFill<double>{c->ytot.get(), 0.0}, s);
+        const double r0 = eval_kkt(c, c->xtot.get(), c->ytot.get()).rel;
+        warm = rw <= r0;
+        c->warm_accepted = warm ? 1 : 0;
+        if (!warm) {
+            launch_elem(n, FnFill<double>{c->xbase.get(), 0.0}, s);
+            launch_elem(std::max(m, 1), FnFill<double>{c->ybase.get(), 0.0}, s);
+        }
+    } else {
+        c->warm_accepted = -1;
+    }
+
     Control ctl;
-    ctl.init(o.tolerance, o.max_iterations, o.max_refinements, c->omega0, !c->y0_h.empty() && m > 0);
+    ctl.init(o.tolerance, o.max_iterations, o.max_refinements, c->omega0, warm && !c->y0_h.empty() && m > 0);
     res->status = PM_SOLVE_ITERATION_LIMIT;
     // Under PM_TERM_REFERENCE a point is optimal only when the reference's
     // own measures agree (pv, dv <= tolerance, pdhg_solver.cu:356): each
@@ -821,7 +846,7 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
                         s);
         have_candidate = true;
         const Kkt kk = eval_kkt(c, c->xtot.get(), c->ytot.get());
-        const int act = ctl.on_check(kk.rel);
+        const int act = ctl.on_check(kk.rel, kk.rel_p, kk.rel_d, kk.rel_g);
         if (o.verbose)
             fprintf(stderr, "[pramaan] it %d round %d kkt %.3e (p %.2e d %.2e g %.2e) omega %.3g restarts %d\n",
                     ctl.total_it, ctl.rounds, kk.rel, kk.rel_p, kk.rel_d, kk.rel_g, ctl.omega, ctl.restarts);
