@@ -211,6 +211,27 @@ int pm_ruiz_scale(int32_t m, int32_t n, const int32_t* row_ptr,
                   double* var_upper, int32_t passes, double* row_scale,
                   double* col_scale);
 
+/* pm_ruiz_scale without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * array is caller-owned DEVICE memory on the current device, with the
+ * meaning and results of pm_ruiz_scale (bit for bit): row_ptr[m+1],
+ * col_idx/values[nnz] (values scaled in place), c, bounds in place,
+ * row_scale[m], col_scale[n] written.  The per-pass factors and column
+ * norms live in a fixed device array of the library (32 MiB per device,
+ * held until the process ends), so m + 2n may not exceed 4,194,304: larger
+ * LPs fail with PM_ERR_INVALID_ARG before any launch.  A row pointer or
+ * column index outside the sizes given fails with PM_ERR_INVALID_ARG (the
+ * arrays are then partly scaled).  Runs on `stream` (a hipStream_t; NULL =
+ * the null stream) and returns when the work is done.  Calls that overlap
+ * on one device share the array and race: serialise them. */
+int pm_ruiz_scale_noalloc(int32_t m, int32_t n, int32_t nnz,
+                          const int32_t* row_ptr, const int32_t* col_idx,
+                          double* values, double* c, double* row_lower,
+                          double* row_upper, double* var_lower,
+                          double* var_upper, int32_t passes,
+                          double* row_scale, double* col_scale, void* stream);
+
 /* ---- device-resident engine -------------------------------------------- */
 /* A context owns one device stream and the device copy of one LP (both CSR
  * orientations, scaled FP32 working data and FP64 originals).  A context is

@@ -80,6 +80,8 @@ SIGNATURES = {
     "pm_pdhg_solve": (C.c_int, [C.POINTER(pm_lp), C.POINTER(pm_pdhg_options), C.POINTER(pm_pdhg_result)]),
     "pm_launch_spmv": (C.c_int, [_i, _i, _f, C.c_int32, _f, _f, C.c_int32, C.c_int32, C.c_int32]),
     "pm_ruiz_scale": (C.c_int, [C.c_int32, C.c_int32, _i, _i, _d, _d, _d, _d, _d, _d, C.c_int32, _d, _d]),
+    "pm_ruiz_scale_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_int32] + [C.c_void_p] * 8 + [C.c_int32]
+                              + [C.c_void_p] * 3),
     "pm_launch_spmv_t": (C.c_int, [_i, _i, _f, C.c_int32, _f, _f, C.c_int32, C.c_int32, C.c_int32]),
     "pm_launch_pdhg_iteration": (C.c_int, [_f, _f, _f, _f, _f, C.c_int32, C.c_int32, C.c_float, C.c_float]),
     "pm_ctx_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
@@ -640,6 +642,16 @@ def ruiz_scale(lp: LP, passes: int = 10) -> dict:
                                _p(xu, C.c_double), passes, _p(rs, C.c_double), _p(cs, C.c_double)))
     return dict(values=v, c=c, row_lower=rl, row_upper=ru, var_lower=xl, var_upper=xu,
                 row_scale=rs[:lp.m], col_scale=cs[:lp.n])
+
+
+def ruiz_scale_noalloc(m: int, n: int, nnz: int, ptrs: dict, passes: int = 10, stream: int = 0) -> None:
+    """pm_ruiz_scale_noalloc on caller-owned device arrays, given as device
+    addresses (ints) under the keys row_ptr, col_idx, values, c, row_lower,
+    row_upper, var_lower, var_upper, row_scale, col_scale; scaled in place."""
+    keys = ("row_ptr", "col_idx", "values", "c", "row_lower", "row_upper", "var_lower", "var_upper")
+    _check(lib().pm_ruiz_scale_noalloc(m, n, nnz, *[C.c_void_p(ptrs[k]) for k in keys], passes,
+                                       C.c_void_p(ptrs["row_scale"]), C.c_void_p(ptrs["col_scale"]),
+                                       C.c_void_p(stream)))
 
 
 def launch_spmv_t(row_ptr, col_idx, values, x, m, n, variant="auto") -> np.ndarray:
