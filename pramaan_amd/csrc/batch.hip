@@ -417,10 +417,15 @@ __global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S
     const uint64_t t0 = wall_clock64();
     double lam = 0.0;
     if (d.eta <= 0.0) {  // ||A_hat||_2 by power iteration, once per batch
+        // At least 48 steps, then until ||A^T A v||^2 changes by <= 1e-5
+        // relative, at most 1,024: the estimate approaches sigma_max from
+        // below, and a fixed count stopped 0.8 % short on a window whose
+        // top singular values cluster (0.8571, 0.8567, 0.8566).
         for (int j = tid; j < n; j += kBB) v.xt[j] = rsqrtf(static_cast<float>(n));
         __syncthreads();
         if (m > 0 && d.nnz > 0) {
-            for (int it = 0; it < 48; ++it) {
+            double prev = -1.0;
+            for (int it = 0; it < 1024; ++it) {
                 brows_any<float>(v.la, m, v.rp, v.ci, v.va, v.xt, EpiStore<float>{v.yt});
                 __syncthreads();
                 brows_any<float>(v.lat, n, v.cp, v.ri, v.vat, v.yt, EpiStore<float>{v.xe});
@@ -432,6 +437,9 @@ __global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S
                 const double inv = lam > 0.0 ? 1.0 / sqrt(lam) : 0.0;
                 for (int j = tid; j < n; j += kBB) v.xt[j] = static_cast<float>(v.xe[j] * inv);
                 __syncthreads();
+                // every thread holds the same lam (block_reduce_all), so all leave together
+                if (it + 1 >= 48 && (!(lam > 0.0) || fabs(lam - prev) <= 1e-5 * lam)) break;
+                prev = lam;
             }
         }
     }
