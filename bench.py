@@ -199,13 +199,14 @@ def bench_rolling(args, P) -> dict:
     for w in range(args.warmup):
         cold_step()
         learned_step()
-    cold_t, learned_t, infer_t = [], [], []
+    cold_t, learned_t, infer_t, learned_dev = [], [], [], []
     cold_res = learned_res = None
     for s in range(args.steps):
         wall, dev, cold_res = cold_step()
         cold_t.append(wall)
         wall2, dev2, learned_res, inf = learned_step()
         learned_t.append(wall2)
+        learned_dev.append(dev2)
         infer_t.append(inf)
         log(f"step {s}: cold {wall:.3f}s (device {dev:.3f}s)  learned {wall2:.3f}s (device {dev2:.3f}s, "
             f"inference {inf * 1e3:.1f} ms)")
@@ -222,6 +223,15 @@ def bench_rolling(args, P) -> dict:
     b.learned_warm_start(net, cpp, rpp)
     vec_res, _, _ = b.run(want_vectors=True)
     pvdv = np.array([reference_residuals(lp, r.x, r.y, args.tol) for lp, r in zip(lps, vec_res)])
+
+    # What the network's primal-weight prediction does: the same learned
+    # solve with it applied (untimed, beside the timed steps, which start
+    # every window from the default weight).
+    net.set_omega_head(True)
+    _, dev_on, res_on, _ = learned_step()
+    net.set_omega_head(False)
+    it_on = np.array([r.iterations for r in res_on])
+    log(f"learned start with the predicted primal weight: device {dev_on:.3f}s, iterations mean {it_on.mean():.0f}")
 
     kern = {}
     for name, kid in (("batch_spmv_a_dual", P.PM_KERNEL_SPMV_A), ("batch_spmv_at_primal", P.PM_KERNEL_SPMV_AT)):
@@ -250,6 +260,11 @@ def bench_rolling(args, P) -> dict:
         "iterations": {"cold_mean": float(it_c.mean()), "cold_max": int(it_c.max()),
                        "learned_mean": float(it_l.mean()), "learned_max": int(it_l.max())},
         "warm_starts_kept": int((acc == 1).sum()), "warm_starts_replaced_by_cold": int((acc == 0).sum()),
+        "omega_head": {"learned_device_seconds_without": float(np.mean(learned_dev)),
+                       "learned_device_seconds_with": dev_on,
+                       "iterations_mean_without": float(it_l.mean()), "iterations_mean_with": float(it_on.mean()),
+                       "note": "the learned start from the default primal weight (the timed steps) and from the "
+                               "network's predicted weight"},
         "objective_rel_error_vs_highs": obj_err,
         "reference_residuals_learned": {"pv_max": float(pvdv[:, 0].max()), "dv_max": float(pvdv[:, 1].max()),
                                         "definition": "pdhg_solver.cu:107-161 (max over windows, tol = the "

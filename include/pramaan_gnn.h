@@ -1,7 +1,7 @@
 /*
  * pramaan_gnn.h -- the learned warm start: a bipartite row/column
  * message-passing network (2 layers, hidden 64) that predicts a primal and
- * dual starting point for an LP.  Dense 64x64 layers run on MFMA in
+ * dual starting point and the initial primal weight for an LP.  Dense 64x64 layers run on MFMA in
  * bfloat16 with FP32 accumulation; message passing (sparse aggregation over
  * the constraint matrix), heads, loss and Adam run in FP32.  Forward,
  * backward and Adam are hand-written HIP kernels.
@@ -14,8 +14,11 @@
  * Parameters are one flat FP32 vector in this order:
  *   We_c[16][64] be_c[64] We_r[16][64] be_r[64]
  *   for l = 1, 2: Wrr[64][64] Wrz[64][64] br[64] Wcc[64][64] Wcz[64][64] bc[64]
- *   wx[64] bx[1] wy[64] by[1]
+ *   wx[64] bx[1] wy[64] by[1] wom[64] bom[1]
  * (row-major; W[k][c] multiplies input feature k into output feature c).
+ * The primal-weight head predicts, for each LP k of a (block-diagonal)
+ * graph, log(final primal weight / default primal weight) as
+ * mean_{columns j of k} Hc2[j] . wom + bom.
  */
 #ifndef PRAMAAN_GNN_H
 #define PRAMAAN_GNN_H
@@ -73,8 +76,26 @@ int pm_gnn_set_graph(pm_gnn* g, int32_t m, int32_t n, int32_t nnz,
                      const float* xr, const float* xc);
 /* Predictions for the current graph: xhat[n], yhat[m]. */
 int pm_gnn_forward(pm_gnn* g, float* xhat, float* yhat);
-/* Loss (mean squared error over columns plus over rows) and its gradient
- * w.r.t. the flat parameters, without updating them. */
+/* The LPs of the current graph for the primal-weight head: LP k owns the
+ * columns [col_off[k], col_off[k+1]) (count + 1 offsets from 0 to n).
+ * pm_gnn_set_graph resets this to one LP holding every column. */
+int pm_gnn_set_segments(pm_gnn* g, int32_t count, const int32_t* col_off);
+/* Training targets of the primal-weight head, one per LP (log of the final
+ * over the default primal weight); with them the loss gains 0.01 x the
+ * mean squared error of the head.  NULL removes them.  Cleared by
+ * pm_gnn_set_graph and pm_gnn_set_segments. */
+int pm_gnn_set_omega_targets(pm_gnn* g, const float* targets);
+/* Forward pass and the head's predictions, one per LP. */
+int pm_gnn_forward_omega(pm_gnn* g, float* omega_hat);
+/* Whether pm_batch_learned_warm_start applies the head (default 0: on the
+ * rolling-horizon windows the predicted weight made the learned solve
+ * slower, 3 % on the measured sequence and 10 % on a held-out one; the
+ * head's target, the weight a cold solve ends with, is not the best
+ * weight to start a warm solve with). */
+int pm_gnn_set_omega_head(pm_gnn* g, int32_t enabled);
+/* Loss (mean squared error over columns plus over rows, plus the head's
+ * when targets are set) and its gradient w.r.t. the flat parameters,
+ * without updating them. */
 int pm_gnn_gradient(pm_gnn* g, const float* tx, const float* ty, float* loss,
                     float* grad);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the current graph. */
@@ -87,7 +108,10 @@ int pm_gnn_train_step(pm_gnn* g, const float* tx, const float* ty, float lr,
  * (period lengths as there), one forward pass runs over the block-diagonal
  * graph of all LPs, and x0 = clamp(xhat * s, l, u), y0 = yhat * cs become
  * the warm starts of every LP for the following pm_batch_run, which judges
- * each against the cold start.  The network's current graph is replaced
+ * each against the cold start; an LP whose warm start is kept starts from
+ * the default primal weight, or, with pm_gnn_set_omega_head(g, 1), the
+ * default times exp(the head's prediction), the factor kept within
+ * [1/100, 100] (pm_batch_set_primal_weight still fixes the weight).  The network's current graph is replaced
  * (call pm_gnn_set_graph again before training).  *seconds (may be NULL)
  * receives the device time of the whole step. */
 struct pm_batch;

@@ -11,7 +11,12 @@ Network (bipartite row/column message passing, hidden H = 64):
     Zr = Ar Hc(l-1);  Hr(l) = relu(Hr(l-1) Wrr_l + Zr Wrz_l + br_l)
     Zc = Ac Hr(l);    Hc(l) = relu(Hc(l-1) Wcc_l + Zc Wcz_l + bc_l)
   xhat = Hc2 wx + bx;  yhat = Hr2 wy + by
+  primal-weight head, per LP g of a block-diagonal graph whose columns
+  [seg[g], seg[g+1]) belong to it:  what_g = mean_{j in g} Hc2[j] . wom + bom
   loss = mean_j (xhat_j - tx_j)^2 + mean_i (yhat_i - ty_i)^2
+         (+ LW * mean_g (what_g - tw_g)^2 when targets tw are given, LW = 0.01:
+          Adam scales each parameter's step, so the head learns at full speed
+          while its term moves the shared layers little)
 Ar is A with rows scaled by 1/max|row| (m x n); Ac is A^T with rows scaled
 by 1/max|column| (n x m).  With bf16=True every 64x64 product rounds its two
 inputs to bfloat16 (round to nearest even) and accumulates in FP64, as the
@@ -22,6 +27,7 @@ from __future__ import annotations
 import numpy as np
 
 H = 64
+LW = 0.01  # weight of the primal-weight head's term in the loss
 DIN = 16  # node features: 7 from the LP and 9 role bits (include/pramaan_gnn.h)
 LAYER_NAMES = ["Wrr", "Wrz", "br", "Wcc", "Wcz", "bc"]
 
@@ -39,7 +45,7 @@ def param_shapes():
     for l in (1, 2):
         shapes += [(f"Wrr{l}", (H, H)), (f"Wrz{l}", (H, H)), (f"br{l}", (H,)),
                    (f"Wcc{l}", (H, H)), (f"Wcz{l}", (H, H)), (f"bc{l}", (H,))]
-    shapes += [("wx", (H,)), ("bx", (1,)), ("wy", (H,)), ("by", (1,))]
+    shapes += [("wx", (H,)), ("bx", (1,)), ("wy", (H,)), ("by", (1,)), ("wom", (H,)), ("bom", (1,))]
     return shapes
 
 
@@ -101,7 +107,7 @@ def _mm(a, b, use_bf16):
     return (bf16(a) @ bf16(b)) if use_bf16 else (a @ b)
 
 
-def forward(p, g: Graph, Xc, Xr, use_bf16=True):
+def forward(p, g: Graph, Xc, Xr, use_bf16=True, seg=None):
     relu = lambda t: np.maximum(t, 0.0)
     c = {}
     c["Hc0"] = relu(Xc @ p["We_c"] + p["be_c"])
@@ -115,11 +121,15 @@ def forward(p, g: Graph, Xc, Xr, use_bf16=True):
         c[f"Zr{l}"], c[f"Hr{l}"], c[f"Zc{l}"], c[f"Hc{l}"] = Zr, Hr, Zc, Hc
     c["xhat"] = Hc @ p["wx"] + p["bx"][0]
     c["yhat"] = Hr @ p["wy"] + p["by"][0]
+    seg = np.array([0, g.n]) if seg is None else np.asarray(seg)
+    c["pool"] = np.stack([Hc[seg[k]:seg[k + 1]].mean(axis=0) if seg[k + 1] > seg[k] else np.zeros(Hc.shape[1])
+                          for k in range(len(seg) - 1)])
+    c["what"] = c["pool"] @ p["wom"] + p["bom"][0]
     return c
 
 
-def loss_and_grad(p, g: Graph, Xc, Xr, tx, ty, use_bf16=True):
-    c = forward(p, g, Xc, Xr, use_bf16)
+def loss_and_grad(p, g: Graph, Xc, Xr, tx, ty, use_bf16=True, seg=None, tw=None):
+    c = forward(p, g, Xc, Xr, use_bf16, seg)
     n, m = g.n, g.m
     ex, ey = c["xhat"] - tx, c["yhat"] - ty
     loss = float(np.mean(ex ** 2) + np.mean(ey ** 2))
@@ -131,6 +141,17 @@ def loss_and_grad(p, g: Graph, Xc, Xr, tx, ty, use_bf16=True):
     gr["by"] = np.array([dy.sum()])
     dHc = np.outer(dx, p["wx"])
     dHr = np.outer(dy, p["wy"])
+    if tw is not None:
+        seg = np.array([0, n]) if seg is None else np.asarray(seg)
+        ew = c["what"] - np.asarray(tw, np.float64)
+        loss += LW * float(np.mean(ew ** 2))
+        dw = 2.0 * LW * ew / len(ew)
+        gr["wom"] = c["pool"].T @ dw
+        gr["bom"] = np.array([dw.sum()])
+        for k in range(len(seg) - 1):
+            cnt = seg[k + 1] - seg[k]
+            if cnt > 0:
+                dHc[seg[k]:seg[k + 1]] += dw[k] * p["wom"] / cnt
     for l in (2, 1):
         Hc_prev = c[f"Hc{l - 1}"]
         Hr_prev = c[f"Hr{l - 1}"]

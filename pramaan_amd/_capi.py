@@ -111,6 +111,10 @@ SIGNATURES = {
     "pm_gen_view": (C.c_int, [C.c_void_p, C.POINTER(pm_lp), C.POINTER(_d), C.POINTER(_d), _d]),
     "pm_gen_free": (None, [C.c_void_p]),
     "pm_gnn_num_params": (C.c_int32, []),
+    "pm_gnn_set_segments": (C.c_int, [C.c_void_p, C.c_int32, _i]),
+    "pm_gnn_set_omega_targets": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pm_gnn_forward_omega": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pm_gnn_set_omega_head": (C.c_int, [C.c_void_p, C.c_int32]),
     "pm_gnn_create": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pm_gnn_destroy": (C.c_int, [C.c_void_p]),
     "pm_gnn_get_params": (C.c_int, [C.c_void_p, _f]),
@@ -481,11 +485,36 @@ class Gnn:
         a = {k: np.ascontiguousarray(g[k], np.int32 if k in ("rp", "ci", "cp", "ri") else np.float32)
              for k in ("rp", "ci", "ar", "cp", "ri", "ac", "xr", "xc")}
         self._graph = a
+        self._nseg = 1
         self.m, self.n = int(g["m"]), int(g["n"])
         _check(lib().pm_gnn_set_graph(self._h, self.m, self.n, len(a["ci"]), _p(a["rp"], C.c_int32),
                                       _p(a["ci"], C.c_int32), _p(a["ar"], C.c_float), _p(a["cp"], C.c_int32),
                                       _p(a["ri"], C.c_int32), _p(a["ac"], C.c_float), _p(a["xr"], C.c_float),
                                       _p(a["xc"], C.c_float)))
+
+    def set_segments(self, col_off) -> None:
+        """LP k of the current graph owns the columns [col_off[k], col_off[k+1])."""
+        off = np.ascontiguousarray(col_off, np.int32)
+        self._nseg = len(off) - 1
+        _check(lib().pm_gnn_set_segments(self._h, self._nseg, _p(off, C.c_int32)))
+
+    def set_omega_targets(self, t) -> None:
+        """Targets of the primal-weight head, one per LP (None removes them)."""
+        if t is None:
+            _check(lib().pm_gnn_set_omega_targets(self._h, None))
+            return
+        t = np.ascontiguousarray(t, np.float32)
+        if len(t) != getattr(self, "_nseg", 1):
+            raise ValueError("one primal-weight target per LP of the graph")
+        _check(lib().pm_gnn_set_omega_targets(self._h, _p(t, C.c_float)))
+
+    def forward_omega(self) -> np.ndarray:
+        out = np.zeros(getattr(self, "_nseg", 1), np.float32)
+        _check(lib().pm_gnn_forward_omega(self._h, _p(out, C.c_float)))
+        return out
+
+    def set_omega_head(self, enabled: bool) -> None:
+        _check(lib().pm_gnn_set_omega_head(self._h, 1 if enabled else 0))
 
     def forward(self):
         x = np.zeros(self.n, np.float32)

@@ -490,6 +490,7 @@ __global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S
         BatchState& st = S[lp];
         double om = (c_norm2 > 1e-20 && b_norm2 > 1e-20) ? sqrt(c_norm2) / sqrt(b_norm2) : 1.0;
         if (d.omega0 > 0.0) om = d.omega0;
+        else if (d.omega_scale > 0.0 && warm) om *= d.omega_scale;
         if (d.eta <= 0.0) d.eta = lam > 0.0 ? 0.99 / sqrt(sqrt(lam)) : 1.0;
         d.warm_accepted = (d.warm_x || d.warm_y) ? (warm ? 1 : 0) : -1;
         d.start_kkt = (d.warm_x || d.warm_y) ? (warm ? rel_s[0] : rel_s[1]) : -1.0;
@@ -1826,6 +1827,7 @@ int pm_batch_set_warm_start(pm_batch* b, int32_t i, const double* x0, const doub
         pm::BatchDesc& d = b->desc[i];
         d.warm_x = x0 != nullptr;
         d.warm_y = y0 != nullptr && d.m > 0;
+        d.omega_scale = 0.0;  // a learned primal weight belongs to the learned start
         if (x0) PM_HIP(hipMemcpyAsync(b->wx.get() + d.col_off, x0, sizeof(double) * d.n, hipMemcpyHostToDevice, b->stream));
         if (y0 && d.m > 0)
             PM_HIP(hipMemcpyAsync(b->wy.get() + d.row_off, y0, sizeof(double) * d.m, hipMemcpyHostToDevice, b->stream));

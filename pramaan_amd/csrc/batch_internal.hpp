@@ -18,7 +18,9 @@ struct BatchDesc {
     int32_t lanes_a, lanes_at;
     int32_t warm_x, warm_y;
     int32_t warm_accepted;  // -1 no warm start, 1 kept, 0 replaced by the cold start
-    double omega0;          // initial primal weight; <= 0: ||c_hat|| / ||b_hat||
+    double omega0;          // initial primal weight; <= 0: ||c_hat|| / ||b_hat|| x omega_scale
+    double omega_scale;     // learned factor on the default weight (<= 0: none); set by the
+                            // learned warm start, cleared with the warm start
     double eta;             // 0.99 / ||A_hat||_2, 0 until the first run measured it
     double start_kkt;       // FP64 relative KKT of the starting point that was used
     int64_t rp_off, cp_off, nz_off, row_off, col_off;

@@ -25,6 +25,7 @@ namespace pm {
 namespace {
 
 constexpr int kH = 64, kF = PM_GNN_FEATURES;
+constexpr float kOmegaLossWeight = 0.01f;  // weight of the primal-weight head's term in the loss
 static_assert(kF % 4 == 0, "features are loaded four at a time");
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -249,6 +250,81 @@ __global__ __launch_bounds__(256) void k_input(int N, const float* __restrict__ 
         make_float4(fmaxf(s[0], 0.0f), fmaxf(s[1], 0.0f), fmaxf(s[2], 0.0f), fmaxf(s[3], 0.0f));
 }
 
+// Primal-weight head, one workgroup per segment (LP) k: pool[k][c] = mean
+// of H[j][c] over the segment's columns (four column groups summed in LDS
+// in a fixed order), what[k] = pool[k] . w + b.
+__global__ __launch_bounds__(256) void k_pool_head(const int* __restrict__ seg, const float* __restrict__ H,
+                                                   const float* __restrict__ w, const float* __restrict__ b,
+                                                   float* __restrict__ pool, float* __restrict__ what) {
+    const int k = blockIdx.x, c = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int j0 = seg[k], j1 = seg[k + 1];
+    float s = 0.0f;
+    for (int j = j0 + q; j < j1; j += 4) s += H[static_cast<size_t>(j) * kH + c];
+    __shared__ float red[4][kH];
+    __shared__ float dot[kH];
+    red[q][c] = s;
+    __syncthreads();
+    if (q == 0) {
+        const int cnt = j1 - j0;
+        const float mean = cnt > 0 ? (red[0][c] + red[1][c] + red[2][c] + red[3][c]) / static_cast<float>(cnt) : 0.0f;
+        pool[static_cast<size_t>(k) * kH + c] = mean;
+        dot[c] = mean * w[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int e = 0; e < kH; ++e) t += dot[e];
+        what[k] = t + b[0];
+    }
+}
+
+// Loss term and parameter gradient of the primal-weight head (one
+// workgroup), with weight LW = kOmegaLossWeight: dw[k] = 2 LW (what[k] -
+// tw[k]) / count; loss += LW x mean of the squared errors; grad_w[c] =
+// sum_k dw[k] pool[k][c]; grad_b = sum_k dw[k].  Adam scales each
+// parameter's step, so the head learns at full speed while its small
+// weight keeps it from pulling the shared layers away from x and y.
+__global__ __launch_bounds__(kH) void k_omega_grad(int count, const float* __restrict__ what,
+                                                   const float* __restrict__ tw, const float* __restrict__ pool,
+                                                   float* __restrict__ dw, float* __restrict__ grad_w,
+                                                   float* __restrict__ grad_b, float* __restrict__ loss) {
+    const int c = threadIdx.x;
+    for (int k = c; k < count; k += kH)
+        dw[k] = 2.0f * kOmegaLossWeight * (what[k] - tw[k]) / static_cast<float>(count);
+    __syncthreads();
+    float g = 0.0f;
+    for (int k = 0; k < count; ++k) g += dw[k] * pool[static_cast<size_t>(k) * kH + c];
+    grad_w[c] = g;
+    if (c == 0) {
+        float gb = 0.0f, l = 0.0f;
+        for (int k = 0; k < count; ++k) {
+            gb += dw[k];
+            const float e = what[k] - tw[k];
+            l += e * e;
+        }
+        grad_b[0] = gb;
+        loss[0] += kOmegaLossWeight * l / static_cast<float>(count);
+    }
+}
+
+// dH[j][c] += dw[k] w[c] / (columns of k) for column j of segment k.
+__global__ __launch_bounds__(256) void k_omega_bwd(int n, int count, const int* __restrict__ seg,
+                                                   const float* __restrict__ dw, const float* __restrict__ w,
+                                                   float* __restrict__ dH) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t j = t >> 6;
+    const int c = static_cast<int>(t & 63);
+    if (j >= n) return;
+    int lo = 0, hi = count;  // the segment k with seg[k] <= j < seg[k+1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (seg[mid] <= j) lo = mid;
+        else hi = mid;
+    }
+    const int cnt = seg[lo + 1] - seg[lo];
+    dH[j * kH + c] += dw[lo] * w[c] / static_cast<float>(cnt);
+}
+
 // Head: out[r] = H[r] . w + b.
 __global__ __launch_bounds__(256) void k_head(int N, const float* __restrict__ Hin, const float* __restrict__ w,
                                               const float* __restrict__ b, float* __restrict__ out) {
@@ -356,7 +432,7 @@ __global__ __launch_bounds__(256) void k_prep_w(const float* __restrict__ W, sho
 struct Offsets {
     int We_c, be_c, We_r, be_r;
     int Wrr[3], Wrz[3], br[3], Wcc[3], Wcz[3], bc[3];
-    int wx, bx, wy, by, total;
+    int wx, bx, wy, by, wom, bom, total;
     Offsets() {
         int o = 0;
         auto take = [&](int k) { const int r = o; o += k; return r; };
@@ -365,7 +441,7 @@ struct Offsets {
             Wrr[l] = take(kH * kH); Wrz[l] = take(kH * kH); br[l] = take(kH);
             Wcc[l] = take(kH * kH); Wcz[l] = take(kH * kH); bc[l] = take(kH);
         }
-        wx = take(kH); bx = take(1); wy = take(kH); by = take(1);
+        wx = take(kH); bx = take(1); wy = take(kH); by = take(1); wom = take(kH); bom = take(1);
         total = o;
     }
 };
@@ -523,6 +599,8 @@ void forward(pm_gnn* g) {
     if (m > 0)
         hipLaunchKernelGGL(k_head, dim3(cdiv(m, 256)), dim3(256), 0, g->s, m, g->Hr[2].get(), P + kOff.wy, P + kOff.by,
                            g->yhat.get());
+    hipLaunchKernelGGL(k_pool_head, dim3(g->nseg), dim3(256), 0, g->s, g->seg.get(), g->Hc[2].get(), P + kOff.wom,
+                       P + kOff.bom, g->pool.get(), g->what.get());
     PM_LAUNCHED();
 }
 
@@ -553,6 +631,16 @@ float backward(pm_gnn* g) {
         hipLaunchKernelGGL(k_head_bwd, dim3(cdiv(static_cast<long long>(n) * kH, 256)), dim3(256), 0, g->s, n,
                            g->dx.get(), P + kOff.wx, dHc);
         PM_LAUNCHED();
+    }
+    if (g->has_tw) {
+        hipLaunchKernelGGL(k_omega_grad, dim3(1), dim3(kH), 0, g->s, g->nseg, g->what.get(), g->tw.get(),
+                           g->pool.get(), g->dw.get(), g->grad.get() + kOff.wom, g->grad.get() + kOff.bom, loss_dev);
+        if (n > 0)
+            hipLaunchKernelGGL(k_omega_bwd, dim3(cdiv(static_cast<long long>(n) * kH, 256)), dim3(256), 0, g->s, n,
+                               g->nseg, g->seg.get(), g->dw.get(), P + kOff.wom, dHc);
+        PM_LAUNCHED();
+    } else {
+        fill(g, g->grad.get() + kOff.wom, kH + 1, 0.0f);
     }
     if (m > 0) {
         hipLaunchKernelGGL(k_head_bwd, dim3(cdiv(static_cast<long long>(m) * kH, 256)), dim3(256), 0, g->s, m,
@@ -610,6 +698,9 @@ void gnn_reserve(pm_gnn* g, int m, int n, int nnz, bool training) {
     }
     g->xhat.resize(nc);
     g->yhat.resize(mr);
+    const int whole[2] = {0, n};  // one segment until pm_gnn_set_segments says otherwise
+    gnn_set_segments(g, 1, whole);
+    g->has_tw = false;
     g->grad_ready = training;
     if (!training) return;
     g->dHc.resize(nc * kH);
@@ -625,6 +716,18 @@ void gnn_reserve(pm_gnn* g, int m, int n, int nnz, bool training) {
 }
 
 void gnn_forward(pm_gnn* g) { forward(g); }
+
+void gnn_set_segments(pm_gnn* g, int count, const int* col_off) {
+    if (count < 1 || !col_off) fail(PM_ERR_INVALID_ARG, "segments: need at least one");
+    if (col_off[0] != 0 || col_off[count] != g->n) fail(PM_ERR_INVALID_ARG, "segments must run from 0 to n");
+    for (int k = 0; k < count; ++k)
+        if (col_off[k + 1] < col_off[k]) fail(PM_ERR_INVALID_ARG, "segment offsets decrease");
+    g->nseg = count;
+    g->seg.upload(col_off, static_cast<size_t>(count) + 1, g->s);
+    g->pool.resize(static_cast<size_t>(count) * kH);
+    for (auto* b : {&g->what, &g->tw, &g->dw}) b->resize(count);
+    g->has_tw = false;
+}
 
 }  // namespace pm
 
@@ -763,6 +866,42 @@ int pm_gnn_set_graph(pm_gnn* g, int32_t m, int32_t n, int32_t nnz, const int32_t
         g->xc.upload(xc, static_cast<size_t>(n) * kF, s);
         pm::gnn_reserve(g, m, n, nnz, true);
         PM_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int pm_gnn_set_segments(pm_gnn* g, int32_t count, const int32_t* col_off) {
+    return pm::guarded([&] {
+        if (!g) pm::fail(PM_ERR_INVALID_ARG, "NULL network");
+        PM_HIP(hipSetDevice(g->device));
+        pm::gnn_set_segments(g, count, col_off);
+        PM_HIP(hipStreamSynchronize(g->s));
+    });
+}
+
+int pm_gnn_set_omega_targets(pm_gnn* g, const float* targets) {
+    return pm::guarded([&] {
+        if (!g) pm::fail(PM_ERR_INVALID_ARG, "NULL network");
+        PM_HIP(hipSetDevice(g->device));
+        g->has_tw = targets != nullptr;
+        if (targets) g->tw.upload(targets, g->nseg, g->s);
+        PM_HIP(hipStreamSynchronize(g->s));
+    });
+}
+
+int pm_gnn_forward_omega(pm_gnn* g, float* omega_hat) {
+    return pm::guarded([&] {
+        if (!g || !omega_hat) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
+        PM_HIP(hipSetDevice(g->device));
+        pm::forward(g);
+        g->what.download(omega_hat, g->nseg, g->s);
+        PM_HIP(hipStreamSynchronize(g->s));
+    });
+}
+
+int pm_gnn_set_omega_head(pm_gnn* g, int32_t enabled) {
+    return pm::guarded([&] {
+        if (!g) pm::fail(PM_ERR_INVALID_ARG, "NULL network");
+        g->omega_head = enabled != 0;
     });
 }
 

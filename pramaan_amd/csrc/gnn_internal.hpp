@@ -19,6 +19,15 @@ struct pm_gnn {
     pm::DevBuf<float> ar, ac, ar_t, ac_t, xr, xc;
     // activations: Hc0..2, Hr0..2, Zr1..2, Zc1..2, and gradients
     pm::DevBuf<float> Hc[3], Hr[3], Zr[3], Zc[3], dHc, dHr, dHc2, dZ, dT, xhat, yhat, dx, dy, tx, ty, part;
+    // primal-weight head: the graph's LPs own the columns [seg[k], seg[k+1]);
+    // pool[k][64] = mean of their final column embeddings, what[k] the
+    // prediction, tw[k] its training target (used when has_tw), dw[k] the
+    // loss derivative.
+    int nseg = 1;
+    bool has_tw = false;
+    bool omega_head = false;  // pm_batch_learned_warm_start applies what (pm_gnn_set_omega_head)
+    pm::DevBuf<int> seg;
+    pm::DevBuf<float> pool, what, tw, dw;
     std::vector<float> host_tmp;
 };
 
@@ -26,6 +35,10 @@ namespace pm {
 // Device buffers for a graph of m rows, n columns and nnz edges (the
 // gradient buffers too when training); the graph arrays are not touched.
 void gnn_reserve(pm_gnn* g, int m, int n, int nnz, bool training);
-// Forward pass over the graph in g->rp ... g->xc into g->xhat, g->yhat.
+// Forward pass over the graph in g->rp ... g->xc into g->xhat, g->yhat and
+// the primal-weight predictions g->what (one per segment).
 void gnn_forward(pm_gnn* g);
+// Segments of the primal-weight head: count LPs whose columns start at
+// col_off[k] (host array of count + 1, from 0 to n).
+void gnn_set_segments(pm_gnn* g, int count, const int* col_off);
 }  // namespace pm

@@ -228,6 +228,10 @@ void learned_warm_start(pm_batch* b, pm_gnn* g, int period_cols, int period_rows
     g->xr.resize(static_cast<size_t>(std::max(M, 1)) * kF);
     g->xc.resize(static_cast<size_t>(std::max(N, 1)) * kF);
     gnn_reserve(g, M, N, Z, false);
+    std::vector<int> seg(static_cast<size_t>(count) + 1);  // each LP's columns, for the primal-weight head
+    for (int k = 0; k < count; ++k) seg[k] = static_cast<int>(b->desc[k].col_off);
+    seg[count] = N;
+    gnn_set_segments(g, count, seg.data());
     b->d_desc.upload(b->desc, s);
     PM_HIP(hipEventRecord(b->ev0, s));
     PM_HIP(hipMemcpyAsync(g->rp.get(), b->grp.get(), sizeof(int) * (static_cast<size_t>(M) + 1),
@@ -275,9 +279,17 @@ void learned_warm_start(pm_batch* b, pm_gnn* g, int period_cols, int period_rows
     float ms = 0.0f;
     PM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     if (seconds) *seconds = ms * 1e-3;
+    // The primal-weight head predicts log(final weight / default weight);
+    // the factor is kept within the engine's span around the default.
+    std::vector<float> what(count);
+    g->what.download(what.data(), count, g->s);
+    PM_HIP(hipStreamSynchronize(g->s));
+    const double lim = std::log(Control::kOmegaSpan);
     for (int k = 0; k < count; ++k) {
         b->desc[k].warm_x = 1;
         b->desc[k].warm_y = b->desc[k].m > 0 ? 1 : 0;
+        const double w = std::isfinite(what[k]) ? std::max(-lim, std::min(lim, static_cast<double>(what[k]))) : 0.0;
+        b->desc[k].omega_scale = g->omega_head ? std::exp(w) : 0.0;
     }
 }
 

@@ -151,7 +151,8 @@ class TrainingSet:
     index: list = field(default_factory=list)        # (plan, window)
     tx: list = field(default_factory=list)           # x / s, float32
     ty: list = field(default_factory=list)           # y / cs, float32
-    omega: list = field(default_factory=list)
+    omega: list = field(default_factory=list)        # final primal weight of the cold solve
+    tw: list = field(default_factory=list)           # log(final / default primal weight)
 
     def __len__(self):
         return len(self.index)
@@ -170,7 +171,9 @@ def training_set(seeds, windows_per_plan=500, tol=1e-5, demand_noise=0.2, blend_
         ts.plans.append(rh)
         nw = min(windows_per_plan, rh.windows)
         lps = [rh.window(w) for w in range(nw)]
-        st, res = solve_cold(lps, tol, want_vectors=True)
+        with P.Batch(lps, tolerance=tol, max_iterations=200000, check_frequency=64) as b:
+            st, res = solve_cold(lps, tol, want_vectors=True, batch=b)
+            scal = [b.scaling(w) for w in range(nw)]
         ok = 0
         for w, (lp, r) in enumerate(zip(lps, res)):
             if r.status != "OPTIMAL":
@@ -180,9 +183,22 @@ def training_set(seeds, windows_per_plan=500, tol=1e-5, demand_noise=0.2, blend_
             ts.tx.append((r.x / s).astype(np.float32))
             ts.ty.append((r.y / cs).astype(np.float32))
             ts.omega.append(r.primal_weight)
+            ts.tw.append(float(np.log(r.primal_weight / default_primal_weight(lp, *scal[w]))))
             ok += 1
         log(f"plan {sd}: {ok}/{nw} windows solved in {st.seconds:.2f}s (iterations mean {st.iterations.mean():.0f})")
     return ts
+
+
+def default_primal_weight(lp, row_scale, col_scale):
+    """The batch engine's initial primal weight of an LP: ||D_c c|| / ||D_r b||
+    on the scaled problem (c in minimisation form, b_i the larger finite
+    |bound| of row i)."""
+    c = (-1.0 if lp.sense == 1 else 1.0) * lp.c
+    lo = np.where(lp.row_lower > -1e30, np.abs(lp.row_lower), 0.0)
+    hi = np.where(lp.row_upper < 1e30, np.abs(lp.row_upper), 0.0)
+    cn = np.linalg.norm(col_scale * c)
+    bn = np.linalg.norm(row_scale * np.maximum(lo, hi))
+    return cn / bn if cn > 1e-10 and bn > 1e-10 else 1.0
 
 
 def _scales(lp):
@@ -201,8 +217,19 @@ def _sample_graph(ts: TrainingSet, k: int):
 
 
 def _minibatch(ts, idx, pool):
+    """Block-diagonal graph of the windows idx, their x and y targets, each
+    window's column offset and primal-weight target."""
     gs = list(pool.map(lambda k: _sample_graph(ts, k), idx))
-    return W.batch_graphs(gs), np.concatenate([ts.tx[k] for k in idx]), np.concatenate([ts.ty[k] for k in idx])
+    bg = W.batch_graphs(gs)
+    bg["seg"] = np.concatenate([[0], np.cumsum([g["n"] for g in gs])]).astype(np.int32)
+    bg["tw"] = np.array([ts.tw[k] for k in idx], np.float32)
+    return bg, np.concatenate([ts.tx[k] for k in idx]), np.concatenate([ts.ty[k] for k in idx])
+
+
+def _load(net, bg):
+    net.set_graph(bg)
+    net.set_segments(bg["seg"])
+    net.set_omega_targets(bg["tw"])
 
 
 def evaluate_loss(net, ts, idx, batch, pool):
@@ -210,10 +237,11 @@ def evaluate_loss(net, ts, idx, batch, pool):
     for s in range(0, len(idx), batch):
         sel = idx[s:s + batch]
         bg, tx, ty = _minibatch(ts, sel, pool)
-        net.set_graph(bg)
+        _load(net, bg)
         loss, _ = net.gradient(tx, ty)
         tot += loss * len(sel)
-        base += (float(np.mean(tx.astype(np.float64) ** 2)) + float(np.mean(ty.astype(np.float64) ** 2))) * len(sel)
+        base += (float(np.mean(tx.astype(np.float64) ** 2)) + float(np.mean(ty.astype(np.float64) ** 2))
+                 + 0.01 * float(np.mean(bg["tw"].astype(np.float64) ** 2))) * len(sel)
         cnt += len(sel)
     return tot / max(cnt, 1), base / max(cnt, 1)
 
@@ -244,7 +272,7 @@ def train(net, ts, epochs=8, batch=16, lr=2e-3, val_frac=0.05, log=print, seed=0
                 fut = ahead.submit(_minibatch, ts, nxt, pool) if len(nxt) else None
                 step_lr = lr * 0.5 * (1.0 + np.cos(np.pi * step / max(1, steps_total)))
                 t0 = time.perf_counter()
-                net.set_graph(bg)
+                _load(net, bg)
                 losses.append(net.train_step(tx, ty, lr=float(step_lr)))
                 t_train += time.perf_counter() - t0
                 seen += len(perm[s:s + batch])
@@ -318,7 +346,12 @@ def evaluate(net, nwin=500, tol=TOL, previous=0, log=print) -> dict:
             res, secs, inst = b.run()
             acc, _ = b.warm_start_verdicts()
             learned = _stats(res, secs + infer, inst, infer, int((acc == 0).sum()))
-        out[name] = dict(cold=cold.summary(), learned=learned.summary(),
+            net.set_omega_head(True)  # the same start from the predicted primal weight
+            infer = b.learned_warm_start(net, rh.cols_per_period, rh.rows_per_period)
+            res, secs, inst = b.run()
+            net.set_omega_head(False)
+            with_head = _stats(res, secs + infer, inst, infer)
+        out[name] = dict(cold=cold.summary(), learned=learned.summary(), learned_with_omega_head=with_head.summary(),
                          speedup_learned_vs_cold=cold.seconds / learned.seconds)
         if previous and name == "nominal":
             prev, _ = solve_previous(lps[:previous], rh.cols_per_period, rh.rows_per_period, tol)

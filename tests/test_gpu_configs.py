@@ -74,11 +74,12 @@ def test_config4_training_set_and_epoch(capi):
     idx = [0, 5001, 12345, 19999]
     bg, tx, ty = R._minibatch(ts, idx, map_pool())
     net = capi.Gnn(seed=11)
-    net.set_graph(bg)
+    R._load(net, bg)  # graph, the windows' column ranges and primal-weight targets
     loss, grad = net.gradient(tx, ty)
     p = go.unflatten(net.get_params().astype(np.float64))
     g = go.Graph(bg["m"], bg["n"], bg["rp"], bg["ci"], f32(bg["ar"]), bg["cp"], bg["ri"], f32(bg["ac"]))
-    oloss, ogr, _ = go.loss_and_grad(p, g, f32(bg["xc"]), f32(bg["xr"]), f32(tx), f32(ty), use_bf16=True)
+    oloss, ogr, _ = go.loss_and_grad(p, g, f32(bg["xc"]), f32(bg["xr"]), f32(tx), f32(ty), use_bf16=True,
+                                     seg=bg["seg"], tw=bg["tw"])
     assert loss == pytest.approx(oloss, rel=5e-3)
     gg = go.unflatten(grad.astype(np.float64))
     for name, _ in go.param_shapes():

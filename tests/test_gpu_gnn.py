@@ -65,6 +65,43 @@ def test_gradient_matches_oracle(capi):
         assert np.linalg.norm(a - b) <= 2e-2 * den, (name, np.linalg.norm(a - b) / den)
 
 
+def test_primal_weight_head_matches_oracle(capi):
+    """Three LPs in one block-diagonal graph: the head's predictions and the
+    gradient of the loss with its term, against the oracle."""
+    from oracle import gnn_oracle as go
+    from pramaan_amd import warmstart as W
+    lps = [capi.generate(None, seed=s, m=30 + 10 * s, n=70 + 25 * s, k_min=3, k_max=6, eq_frac=0.7)
+           for s in range(3)]
+    gs = [W.graph_of(lp, None, None) for lp in lps]
+    g = W.batch_graphs(gs)
+    seg = np.concatenate([[0], np.cumsum([x["n"] for x in gs])])
+    rng = np.random.default_rng(12)
+    tx, ty = rng.uniform(0, 1, g["n"]), rng.standard_normal(g["m"])
+    tw = np.array([0.4, -1.1, 2.0])
+    net = capi.Gnn(seed=13)
+    p0 = net.get_params()
+    names = [name for name, shp in go.param_shapes() for _ in range(int(np.prod(shp)))]
+    p0[[k for k, nm in enumerate(names) if nm == "wom"]] = rng.standard_normal(64) * 0.2  # a non-zero head
+    net.set_params(p0)
+    net.set_graph(g)
+    net.set_segments(seg)
+    what = net.forward_omega()
+    p = go.unflatten(net.get_params().astype(np.float64))
+    og = oracle_graph(go, g)
+    c = go.forward(p, og, f32(g["xc"]), f32(g["xr"]), use_bf16=True, seg=seg)
+    assert np.max(np.abs(what - c["what"])) <= 5e-3 * (np.abs(c["what"]).max() + 1e-6)
+    net.set_omega_targets(tw)
+    loss, grad = net.gradient(tx, ty)
+    oloss, ogr, _ = go.loss_and_grad(p, og, f32(g["xc"]), f32(g["xr"]), f32(tx), f32(ty), use_bf16=True, seg=seg,
+                                     tw=tw)
+    assert loss == pytest.approx(oloss, rel=5e-3)
+    gg = go.unflatten(grad.astype(np.float64))
+    for name, _ in go.param_shapes():
+        den = np.linalg.norm(ogr[name])
+        if den > 1e-9:
+            assert np.linalg.norm(gg[name] - ogr[name]) <= 2e-2 * den, name
+
+
 def test_adam_step_matches_oracle(capi):
     from oracle import gnn_oracle as go
     _, g, tx, ty = small_problem(capi, seed=7)
