@@ -370,7 +370,7 @@ def bench_batch(args, P) -> dict:
         "objective_rel_error_vs_closed_form_max": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "kernel_us": ms * 1e3,
-                     "alg_bytes_per_launch": by},
+                     "alg_bytes_per_launch": by, "spmv_variants": ctx_variants},
     }
 
 
@@ -379,7 +379,7 @@ def bench_lp1(args, P) -> dict:
     ctx = P.Context(0)
     t0 = time.perf_counter()
     ctx.load(lp, algorithm="pdlp", termination="relative_kkt", tolerance=args.tol, max_iterations=args.max_iters,
-             check_frequency=64)
+             check_frequency=64, spmv_variant=args.spmv)
     load_s = time.perf_counter() - t0
     log(f"loaded m={lp.m} n={lp.n} nnz={lp.nnz} in {load_s:.2f}s (transpose on the host; scaling, "
         f"autotuning and power iteration on the device)")
@@ -398,6 +398,7 @@ def bench_lp1(args, P) -> dict:
     dom = max(kern, key=lambda k: kern[k][0])
     ms, by = kern[dom]
     achieved = by / (ms * 1e-3) / 1e9
+    ctx_variants = ctx.spmv_variants()
     ctx.close()
     out = {
         "metric": METRIC, "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -410,7 +411,7 @@ def bench_lp1(args, P) -> dict:
                                 "definition": "pdhg_solver.cu:107-161"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "kernel_us": ms * 1e3,
-                     "alg_bytes_per_launch": by},
+                     "alg_bytes_per_launch": by, "spmv_variants": ctx_variants},
     }
     if not args.no_cpu:
         per_it, iters = oracle_seconds_per_iteration(lp, args.cpu_budget)
@@ -538,6 +539,7 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iters", type=int, default=200000)
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--spmv", default="auto", help="A x-bar / A^T y kernel of the lp1 workload (a name of _capi.SPMV)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lp1", action="store_true", help="skip the 500k x 1M SpMV measurement")

@@ -94,7 +94,16 @@ extern "C" {
                                   per row, every load coalesced              */
 #define PM_SPMV_STAGED      11 /* sliced ELL with the gathered vector of each
                                   LP staged in LDS (batch engine)            */
-#define PM_SPMV_NUM_VARIANTS 12
+#define PM_SPMV_COLBLOCK    12 /* single-LP PDLP engine: A x-bar in column
+                                  blocks (x-bar cut in 2, in slabs of at most
+                                  1 MB or of at most 512 KB, whichever times
+                                  fastest), one launch per block gathering
+                                  from an L2-resident slab, row sums
+                                  accumulated across blocks; A^T y and the
+                                  checks autotuned as AUTO.  AUTO times it
+                                  beside the row kernels on LPs with
+                                  n >= 2^19 and nnz >= 2^22.              */
+#define PM_SPMV_NUM_VARIANTS 13
 
 /* ---- the LP, in the layout of ModelIR (include/pramaan/ir.hpp:62-119) ---
  * optimize sense: c^T x + offset  s.t.  row_lower <= A x <= row_upper,

@@ -196,6 +196,16 @@ struct pm_ctx {
     int var_a = PM_SPMV_VECTOR8, var_at = PM_SPMV_VECTOR8;
     double eta = 1.0, omega0 = 1.0, sigma_max = 1.0;
 
+    // Column-blocked copy of the scaled A for the iteration's A x-bar
+    // (PDLP): block b holds the entries of every row whose column lies in
+    // [b * cols / nblk, (b + 1) * cols / nblk), as a CSR whose row pointers
+    // rpb[b * (m + 1) + i] index the block-major cib / vb.  Each block's
+    // launch gathers from one slab of x-bar, which then stays in L2.
+    int nblk = 0;  // 0: the plain row kernel (var_a) is used
+    int blk_var = PM_SPMV_VECTOR4;
+    DevBuf<int> rpb, cib;
+    DevBuf<float> vb, accb;
+
     const void* orig_a() const { return exact_f32 ? static_cast<const void*>(oa32.get()) : oa64.get(); }
 };
 
@@ -247,6 +257,77 @@ void host_transpose(int m, int n, const int* rp, const int* ci, const T* v, std:
         }
 }
 
+// dst[q] = src[perm[q]]: the block-major copy of FP32 values.
+__global__ __launch_bounds__(kBlock) void k_gather_f32(int64_t n, const int* __restrict__ perm,
+                                                       const float* __restrict__ src, float* __restrict__ dst) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q < n) dst[q] = src[perm[q]];
+}
+
+// A x-bar through the column blocks with epilogue epi on the row totals.
+template <class Epi>
+void launch_blocked(pm_ctx* c, int nblk, int var, const float* x, const Epi& epi, hipStream_t s) {
+    const int m = c->m;
+    for (int b = 0; b < nblk; ++b)
+        launch_rows<float>(var, m, c->rpb.get() + static_cast<size_t>(b) * (m + 1), c->cib.get(), c->vb.get(), x,
+                           EpiBlock<Epi>{c->accb.get(), epi, b == 0, b == nblk - 1}, s);
+}
+
+// The dual half of a PDLP iteration: A x-bar with the fused dual update.
+void launch_dual_half(pm_ctx* c, const EpiDualHal& ed, hipStream_t s) {
+    if (c->nblk > 0) launch_blocked(c, c->nblk, c->blk_var, c->xe.get(), ed, s);
+    else launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed, s);
+}
+
+// Builds the column-blocked copy of the scaled A with nblk blocks (the
+// values gathered on the device from va through a permutation).
+void build_blocks(pm_ctx* c, const pm_lp* lp, int nblk) {
+    const int m = c->m, n = c->n, nnz = c->nnz;
+    const int* rp = lp->row_ptr;
+    const int* ci = lp->col_idx;
+    std::vector<int> rpb(static_cast<size_t>(nblk) * (m + 1)), cib(nnz), perm(nnz);
+    std::vector<int64_t> cnt(nblk, 0);
+    auto block_of = [&](int j) { return static_cast<int>(static_cast<int64_t>(j) * nblk / n); };
+    for (int k = 0; k < nnz; ++k) ++cnt[block_of(ci[k])];
+    std::vector<int64_t> base(nblk + 1, 0);
+    for (int b = 0; b < nblk; ++b) base[b + 1] = base[b] + cnt[b];
+    std::vector<int64_t> cur(base.begin(), base.end() - 1);
+    for (int i = 0; i < m; ++i) {
+        for (int b = 0; b < nblk; ++b) rpb[static_cast<size_t>(b) * (m + 1) + i] = static_cast<int>(cur[b]);
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {  // columns ascending: blocks in order
+            const int64_t q = cur[block_of(ci[k])]++;
+            cib[q] = ci[k];
+            perm[q] = k;
+        }
+    }
+    for (int b = 0; b < nblk; ++b) rpb[static_cast<size_t>(b) * (m + 1) + m] = static_cast<int>(cur[b]);
+    hipStream_t s = c->stream;
+    c->rpb.upload(rpb, s);
+    c->cib.upload(cib, s);
+    DevBuf<int> d_perm;
+    d_perm.upload(perm, s);
+    c->vb.resize(std::max(nnz, 1));
+    c->accb.resize(std::max(m, 1));
+    hipLaunchKernelGGL(k_gather_f32, dim3((nnz + kBlock - 1) / kBlock), dim3(kBlock), 0, s, static_cast<int64_t>(nnz),
+                       d_perm.get(), c->va.get(), c->vb.get());
+    PM_LAUNCHED();
+    sync(c);
+    c->nblk = nblk;
+}
+
+// Times the column-blocked A x (mean of 5 after 2 warm-ups).
+float time_blocked(pm_ctx* c, int nblk, int var, const float* x, float* y) {
+    EpiStore<float> epi{y};
+    for (int r = 0; r < 2; ++r) launch_blocked(c, nblk, var, x, epi, c->stream);
+    PM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < 5; ++r) launch_blocked(c, nblk, var, x, epi, c->stream);
+    PM_HIP(hipEventRecord(c->ev1, c->stream));
+    PM_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    return ms / 5.0f;
+}
+
 // Times one SpMV variant (mean of 5 launches after 2 warm-ups).
 float time_variant(pm_ctx* c, int variant, int nrows, const int* rp, const int* ci, const float* v,
                    const float* x, float* y) {
@@ -263,7 +344,8 @@ float time_variant(pm_ctx* c, int variant, int nrows, const int* rp, const int* 
 
 int autotune(pm_ctx* c, int nrows, int ncols, int nnz, const int* rp, const int* ci, const float* v,
              const float* x, float* y) {
-    if (c->opts.spmv_variant != PM_SPMV_AUTO) return c->opts.spmv_variant;
+    if (c->opts.spmv_variant != PM_SPMV_AUTO && c->opts.spmv_variant != PM_SPMV_COLBLOCK)
+        return c->opts.spmv_variant;
     if (nnz < (1 << 18) || nrows == 0) return heuristic_variant(nrows, nnz);
     const auto key = std::make_tuple(nrows, ncols, nnz);
     {
@@ -481,6 +563,37 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
         launch_elem(m, FnFill<float>{c->y.get(), 1.0f}, s);
         c->var_a = autotune(c, m, n, nnz, c->rp.get(), c->ci.get(), c->va.get(), xs, c->tmp_m.get());
         c->var_at = autotune(c, n, m, nnz, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), c->tmp_n.get());
+        // Column blocks for the iteration's A x-bar: tried on large LPs,
+        // where x-bar outgrows what an XCD's L2 keeps beside the matrix
+        // stream (the 500k x 1M LP fetched 4.2x its algorithmic bytes in
+        // plain A x-bar, most of it lines brought in for one gathered value).
+        c->nblk = 0;
+        const bool forced = opts.spmv_variant == PM_SPMV_COLBLOCK;
+        if (opts.algorithm == PM_ALG_PDLP && m > 0 && nnz > 0 &&
+            (forced || (opts.spmv_variant == PM_SPMV_AUTO && n >= (1 << 19) && nnz >= (1 << 22)))) {
+            const int slab = std::max(2, static_cast<int>((static_cast<int64_t>(n) * 4 + (1 << 20) - 1) >> 20));
+            float best = forced ? 1e30f : time_variant(c, c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), xs,
+                                                       c->tmp_m.get());
+            int best_nb = 0, best_var = PM_SPMV_VECTOR4;
+            std::vector<int> counts{2, slab, 2 * slab};  // x-bar in halves, in slabs <= 1 MB, <= 512 KB
+            counts.erase(std::unique(counts.begin(), counts.end()), counts.end());
+            for (int nb : counts) {
+                build_blocks(c, lp, nb);
+                for (int v_ : {PM_SPMV_VECTOR2, PM_SPMV_VECTOR4, PM_SPMV_VECTOR8}) {
+                    const float ms = time_blocked(c, nb, v_, xs, c->tmp_m.get());
+                    if (ms < best) { best = ms; best_nb = nb; best_var = v_; }
+                }
+            }
+            if (best_nb > 0) {
+                if (best_nb != c->nblk) build_blocks(c, lp, best_nb);
+                c->blk_var = best_var;
+            } else {
+                c->nblk = 0;
+                for (auto* bf : {&c->rpb, &c->cib}) bf->reset();
+                c->vb.reset();
+                c->accb.reset();
+            }
+        }
     }
 
     if (opts.algorithm == PM_ALG_PDLP && m > 0 && nnz > 0) {
@@ -783,7 +896,7 @@ void hal_iteration(pm_ctx* c, int k, double omega, bool keep_t, bool use_g) {
     if (c->m > 0) {
         EpiDualHal ed{sig, lam, rho, c->lr32.get(), c->ur32.get(), use_g ? c->g32.get() : nullptr, c->y0.get(),
                       c->y.get(), keep_t ? c->yt.get() : nullptr};
-        launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed, s);
+        launch_dual_half(c, ed, s);
     }
 }
 
@@ -1075,7 +1188,7 @@ int pm_ctx_scaling(pm_ctx* c, double* row_scale, double* col_scale) {
 int pm_ctx_spmv_variants(pm_ctx* c, int32_t* va, int32_t* vat) {
     return pm::guarded([&] {
         if (!c || !va || !vat) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
-        *va = c->var_a;
+        *va = c->nblk > 0 ? PM_SPMV_COLBLOCK : c->var_a;
         *vat = c->var_at;
     });
 }
@@ -1097,8 +1210,7 @@ int pm_ctx_time_kernel(pm_ctx* c, int32_t which, int32_t reps, double* avg_ms, d
                 if (pdlp) {
                     EpiDualHal ed{static_cast<float>(c->eta * omega), 0.5f, 1.0f, c->lr32.get(), c->ur32.get(),
                                   nullptr, c->y0.get(), c->y.get(), nullptr};
-                    launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed,
-                                       c->stream);
+                    launch_dual_half(c, ed, c->stream);
                 } else {
                     launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(),
                                        EpiDualRef{c->sig32.get(), c->lr32.get(), c->ur32.get(), c->y.get()}, c->stream);

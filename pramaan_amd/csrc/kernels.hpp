@@ -210,6 +210,23 @@ struct EpiStore {
     __device__ void operator()(int i, AT s) const { y[i] = static_cast<T>(s); }
 };
 
+// Column-blocked product: the launches of one product walk the column
+// blocks in order; each adds its row sums to acc[] (the first one starts
+// it), and the last hands the row total to the inner epilogue.
+template <class Inner>
+struct EpiBlock {
+    float* acc;
+    Inner inner;
+    int first, last;
+    template <class AT>
+    __device__ void operator()(int i, AT s) const {
+        float t = static_cast<float>(s);
+        if (!first) t = acc[i] + t;
+        if (last) inner(i, t);
+        else acc[i] = t;
+    }
+};
+
 // Reference dual update fused into A x-bar (dual_update_kernel,
 // pdhg_kernels.cuh:72-89): w = y/sigma + Ax; y = sigma (w - clamp(w,l,u)),
 // rows with sigma <= 0 untouched.
