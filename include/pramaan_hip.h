@@ -88,10 +88,17 @@ extern "C" {
 #define PM_SPMV_VECTOR16    6
 #define PM_SPMV_VECTOR32    7
 #define PM_SPMV_VECTOR64    8
-#define PM_SPMV_STREAM      9  /* LDS-staged row segments (CSR-stream)        */
-#define PM_SPMV_SLICED      10 /* sliced ELL (batch engine): 64-row slices,
-                                  rows sorted by length within 256, one lane
-                                  per row, every load coalesced              */
+#define PM_SPMV_STREAM      9  /* LDS-staged row segments (CSR-stream), batch
+                                  engine                                     */
+#define PM_SPMV_SLICED      10 /* sliced ELL: 64-row slices, rows sorted by
+                                  length within a window, one lane per row,
+                                  every matrix load coalesced.  Batch engine:
+                                  16-bit column indexes inside each group's
+                                  column window.  Single-LP PDLP engine: both
+                                  products of the iteration over 32-bit
+                                  copies of A and A^T (windows of 1024 rows);
+                                  AUTO times them beside the row kernels on
+                                  LPs of at least 2^18 nonzeros.             */
 #define PM_SPMV_STAGED      11 /* sliced ELL with the gathered vector of each
                                   LP staged in LDS (batch engine)            */
 #define PM_SPMV_COLBLOCK    12 /* single-LP PDLP engine: A x-bar in column

@@ -77,6 +77,17 @@ void launch_rows(int variant, int nrows, const int* rp, const int* ci, const VT*
     PM_LAUNCHED();
 }
 
+// Sliced ELL product over nslices slices of 64 rows (k_sell).
+template <class Epi>
+void launch_sell(int nslices, const int* sptr, const int* slen, const int* srow, const int* col, const float* val,
+                 const float* x, const Epi& epi, hipStream_t s) {
+    if (nslices <= 0) return;
+    const int per = kBlock / 64;
+    hipLaunchKernelGGL((k_sell<Epi>), dim3((nslices + per - 1) / per), dim3(kBlock), 0, s, nslices, sptr, slen, srow,
+                       col, val, x, epi);
+    PM_LAUNCHED();
+}
+
 // Row kernel with block-partial reduction, followed by the final reduce of
 // its partials into out[0 .. NS+NM).  `partials` must hold
 // grid_for(nrows, lanes) * (NS+NM) doubles.

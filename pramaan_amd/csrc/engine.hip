@@ -152,6 +152,20 @@ int heuristic_variant(int nrows, int nnz) {
     }
 }
 
+// Sliced ELL copy of one matrix (k_sell): slices of 64 rows, rows sorted
+// by length (longest first) inside windows of kSellWindow rows.
+constexpr int kSellWindow = 1024;
+struct SellMat {
+    int slices = 0;
+    DevBuf<int> ptr, len, row, col;
+    DevBuf<float> val;
+    void reset() {
+        slices = 0;
+        for (auto* b : {&ptr, &len, &row, &col}) b->reset();
+        val.reset();
+    }
+};
+
 }  // namespace pm
 
 using namespace pm;
@@ -205,6 +219,10 @@ struct pm_ctx {
     int blk_var = PM_SPMV_VECTOR4;
     DevBuf<int> rpb, cib;
     DevBuf<float> vb, accb;
+
+    // Sliced ELL copies of the scaled A and A^T (PM_SPMV_SLICED, PDLP):
+    // used for the iteration's products when slices > 0.
+    SellMat sell_a, sell_at;
 
     const void* orig_a() const { return exact_f32 ? static_cast<const void*>(oa32.get()) : oa64.get(); }
 };
@@ -273,10 +291,107 @@ void launch_blocked(pm_ctx* c, int nblk, int var, const float* x, const Epi& epi
                            EpiBlock<Epi>{c->accb.get(), epi, b == 0, b == nblk - 1}, s);
 }
 
+template <class Epi>
+void launch_sell_mat(const SellMat& S, const float* x, const Epi& epi, hipStream_t s) {
+    launch_sell(S.slices, S.ptr.get(), S.len.get(), S.row.get(), S.col.get(), S.val.get(), x, epi, s);
+}
+
 // The dual half of a PDLP iteration: A x-bar with the fused dual update.
 void launch_dual_half(pm_ctx* c, const EpiDualHal& ed, hipStream_t s) {
     if (c->nblk > 0) launch_blocked(c, c->nblk, c->blk_var, c->xe.get(), ed, s);
+    else if (c->sell_a.slices > 0) launch_sell_mat(c->sell_a, c->xe.get(), ed, s);
     else launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed, s);
+}
+
+// The primal half of a PDLP iteration: A^T y with the fused primal update.
+void launch_primal_half(pm_ctx* c, const EpiPrimalHal& ep, hipStream_t s) {
+    if (c->sell_at.slices > 0) launch_sell_mat(c->sell_at, c->y.get(), ep, s);
+    else launch_rows<float>(c->var_at, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep, s);
+}
+
+// dst[q] = src[perm[q]], or 0 where perm[q] < 0 (padding).
+__global__ __launch_bounds__(kBlock) void k_gather_pad_f32(int64_t n, const int* __restrict__ perm,
+                                                           const float* __restrict__ src, float* __restrict__ dst) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q < n) {
+        const int k = perm[q];
+        dst[q] = k >= 0 ? src[k] : 0.0f;
+    }
+}
+
+// Builds the sliced ELL copy S of a matrix of nrows rows held as a host CSR
+// structure (rp, ci) whose FP32 values lie on the device in CSR order (dv).
+// Returns false (S left empty) when the padded entries would not fit int32
+// offsets.
+bool build_sell(pm_ctx* c, int nrows, const int* rp, const int* ci, const float* dv, SellMat& S) {
+    S.reset();
+    const int ns = (nrows + 63) / 64;
+    std::vector<int> order(nrows);
+    std::iota(order.begin(), order.end(), 0);
+    auto rlen = [&](int r) { return rp[r + 1] - rp[r]; };
+    for (int w0 = 0; w0 < nrows; w0 += kSellWindow) {
+        const int w1 = std::min(nrows, w0 + kSellWindow);
+        std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) { return rlen(a) > rlen(b); });
+    }
+    std::vector<int> len(ns, 0), row(static_cast<size_t>(ns) * 64, -1), ptr(ns);
+    int64_t total = 0;
+    for (int sl = 0; sl < ns; ++sl) {
+        int L = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int q = sl * 64 + l;
+            if (q < nrows) {
+                row[q] = order[q];
+                L = std::max(L, rlen(order[q]));
+            }
+        }
+        len[sl] = L;
+        ptr[sl] = static_cast<int>(std::min<int64_t>(total, INT32_MAX));
+        total += int64_t(64) * L;
+    }
+    if (total >= INT32_MAX) return false;
+    std::vector<int> col(std::max<int64_t>(total, 1), 0), perm(std::max<int64_t>(total, 1), -1);
+    for (int sl = 0; sl < ns; ++sl)
+        for (int l = 0; l < 64; ++l) {
+            const int r = row[sl * 64 + l];
+            const int rl = r >= 0 ? rlen(r) : 0;
+            const int pad_col = rl > 0 ? ci[rp[r + 1] - 1] : 0;
+            for (int j = 0; j < len[sl]; ++j) {
+                const int64_t q = ptr[sl] + int64_t(j) * 64 + l;
+                if (j < rl) {
+                    col[q] = ci[rp[r] + j];
+                    perm[q] = rp[r] + j;
+                } else {
+                    col[q] = pad_col;
+                }
+            }
+        }
+    hipStream_t s = c->stream;
+    S.ptr.upload(ptr, s);
+    S.len.upload(len, s);
+    S.row.upload(row, s);
+    S.col.upload(col, s);
+    DevBuf<int> d_perm;
+    d_perm.upload(perm, s);
+    S.val.resize(col.size());
+    hipLaunchKernelGGL(k_gather_pad_f32, dim3((col.size() + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       static_cast<int64_t>(col.size()), d_perm.get(), dv, S.val.get());
+    PM_LAUNCHED();
+    sync(c);
+    S.slices = ns;
+    return true;
+}
+
+// Times a sliced ELL product y = S x (mean of 5 after 2 warm-ups).
+float time_sell(pm_ctx* c, const SellMat& S, const float* x, float* y) {
+    EpiStore<float> epi{y};
+    for (int r = 0; r < 2; ++r) launch_sell_mat(S, x, epi, c->stream);
+    PM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < 5; ++r) launch_sell_mat(S, x, epi, c->stream);
+    PM_HIP(hipEventRecord(c->ev1, c->stream));
+    PM_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    return ms / 5.0f;
 }
 
 // Builds the column-blocked copy of the scaled A with nblk blocks (the
@@ -344,8 +459,8 @@ float time_variant(pm_ctx* c, int variant, int nrows, const int* rp, const int* 
 
 int autotune(pm_ctx* c, int nrows, int ncols, int nnz, const int* rp, const int* ci, const float* v,
              const float* x, float* y) {
-    if (c->opts.spmv_variant != PM_SPMV_AUTO && c->opts.spmv_variant != PM_SPMV_COLBLOCK)
-        return c->opts.spmv_variant;
+    const int ov = c->opts.spmv_variant;
+    if (ov != PM_SPMV_AUTO && ov != PM_SPMV_COLBLOCK && ov != PM_SPMV_SLICED) return ov;
     if (nnz < (1 << 18) || nrows == 0) return heuristic_variant(nrows, nnz);
     const auto key = std::make_tuple(nrows, ncols, nnz);
     {
@@ -391,6 +506,11 @@ void check_options(const pm_pdhg_options& opts) {
     if (opts.max_iterations < 0) fail(PM_ERR_INVALID_ARG, "max_iterations must be non-negative");
     if (opts.spmv_variant < 0 || opts.spmv_variant >= PM_SPMV_NUM_VARIANTS)
         fail(PM_ERR_INVALID_ARG, "unknown spmv_variant");
+    if (opts.spmv_variant == PM_SPMV_STREAM || opts.spmv_variant == PM_SPMV_STAGED)
+        fail(PM_ERR_INVALID_ARG, "spmv_variant PM_SPMV_STREAM and PM_SPMV_STAGED are batch-engine kernels");
+    if ((opts.spmv_variant == PM_SPMV_SLICED || opts.spmv_variant == PM_SPMV_COLBLOCK) &&
+        opts.algorithm != PM_ALG_PDLP)
+        fail(PM_ERR_INVALID_ARG, "spmv_variant PM_SPMV_SLICED and PM_SPMV_COLBLOCK need PM_ALG_PDLP");
     if (opts.algorithm != PM_ALG_REFERENCE && opts.algorithm != PM_ALG_PDLP)
         fail(PM_ERR_INVALID_ARG, "unknown algorithm");
     if (opts.termination != PM_TERM_REFERENCE && opts.termination != PM_TERM_RELATIVE_KKT)
@@ -563,6 +683,29 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
         launch_elem(m, FnFill<float>{c->y.get(), 1.0f}, s);
         c->var_a = autotune(c, m, n, nnz, c->rp.get(), c->ci.get(), c->va.get(), xs, c->tmp_m.get());
         c->var_at = autotune(c, n, m, nnz, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), c->tmp_n.get());
+        // Sliced ELL copies for the iteration's products (PDLP): built and
+        // timed beside the row kernels on LPs of at least 2^18 nonzeros,
+        // kept where faster (always under PM_SPMV_SLICED).
+        c->sell_a.reset();
+        c->sell_at.reset();
+        float t_a = -1.0f;  // time of A x-bar by the kernel kept, when measured
+        const bool want_sell = opts.spmv_variant == PM_SPMV_SLICED;
+        if (opts.algorithm == PM_ALG_PDLP && m > 0 && nnz > 0 &&
+            (want_sell || ((opts.spmv_variant == PM_SPMV_AUTO || opts.spmv_variant == PM_SPMV_COLBLOCK) &&
+                           nnz >= (1 << 18)))) {
+            if (build_sell(c, n, cp.data(), ri.data(), c->vat.get(), c->sell_at) && !want_sell) {
+                const float tr = time_variant(c, c->var_at, n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(),
+                                              c->tmp_n.get());
+                if (time_sell(c, c->sell_at, c->y.get(), c->tmp_n.get()) >= tr) c->sell_at.reset();
+            }
+            if (build_sell(c, m, lp->row_ptr, lp->col_idx, c->va.get(), c->sell_a) && !want_sell) {
+                const float tr = time_variant(c, c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), xs,
+                                              c->tmp_m.get());
+                const float ts = time_sell(c, c->sell_a, xs, c->tmp_m.get());
+                t_a = std::min(tr, ts);
+                if (ts >= tr) c->sell_a.reset();
+            }
+        }
         // Column blocks for the iteration's A x-bar: tried on large LPs,
         // where x-bar outgrows what an XCD's L2 keeps beside the matrix
         // stream (the 500k x 1M LP fetched 4.2x its algorithmic bytes in
@@ -572,8 +715,10 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
         if (opts.algorithm == PM_ALG_PDLP && m > 0 && nnz > 0 &&
             (forced || (opts.spmv_variant == PM_SPMV_AUTO && n >= (1 << 19) && nnz >= (1 << 22)))) {
             const int slab = std::max(2, static_cast<int>((static_cast<int64_t>(n) * 4 + (1 << 20) - 1) >> 20));
-            float best = forced ? 1e30f : time_variant(c, c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), xs,
-                                                       c->tmp_m.get());
+            float best = forced ? 1e30f
+                         : t_a >= 0.0f ? t_a
+                                       : time_variant(c, c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), xs,
+                                                      c->tmp_m.get());
             int best_nb = 0, best_var = PM_SPMV_VECTOR4;
             std::vector<int> counts{2, slab, 2 * slab};  // x-bar in halves, in slabs <= 1 MB, <= 512 KB
             counts.erase(std::unique(counts.begin(), counts.end()), counts.end());
@@ -587,6 +732,7 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
             if (best_nb > 0) {
                 if (best_nb != c->nblk) build_blocks(c, lp, best_nb);
                 c->blk_var = best_var;
+                c->sell_a.reset();
             } else {
                 c->nblk = 0;
                 for (auto* bf : {&c->rpb, &c->cib}) bf->reset();
@@ -892,7 +1038,7 @@ void hal_iteration(pm_ctx* c, int k, double omega, bool keep_t, bool use_g) {
     const float rho = 1.0f;
     EpiPrimalHal ep{tau, lam, rho, c->c32.get(), c->lx32.get(), c->ux32.get(), c->x0.get(), c->x.get(), c->xe.get(),
                     keep_t ? c->xt.get() : nullptr};
-    launch_rows<float>(c->var_at, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep, s);
+    launch_primal_half(c, ep, s);
     if (c->m > 0) {
         EpiDualHal ed{sig, lam, rho, c->lr32.get(), c->ur32.get(), use_g ? c->g32.get() : nullptr, c->y0.get(),
                       c->y.get(), keep_t ? c->yt.get() : nullptr};
@@ -1188,8 +1334,8 @@ int pm_ctx_scaling(pm_ctx* c, double* row_scale, double* col_scale) {
 int pm_ctx_spmv_variants(pm_ctx* c, int32_t* va, int32_t* vat) {
     return pm::guarded([&] {
         if (!c || !va || !vat) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
-        *va = c->nblk > 0 ? PM_SPMV_COLBLOCK : c->var_a;
-        *vat = c->var_at;
+        *va = c->nblk > 0 ? PM_SPMV_COLBLOCK : c->sell_a.slices > 0 ? PM_SPMV_SLICED : c->var_a;
+        *vat = c->sell_at.slices > 0 ? PM_SPMV_SLICED : c->var_at;
     });
 }
 
@@ -1220,8 +1366,7 @@ int pm_ctx_time_kernel(pm_ctx* c, int32_t which, int32_t reps, double* avg_ms, d
                 if (pdlp) {
                     EpiPrimalHal ep{static_cast<float>(c->eta / omega), 0.5f, 1.0f, c->c32.get(), c->lx32.get(),
                                     c->ux32.get(), c->x0.get(), c->x.get(), c->xe.get(), nullptr};
-                    launch_rows<float>(c->var_at, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep,
-                                       c->stream);
+                    launch_primal_half(c, ep, c->stream);
                 } else {
                     launch_rows<float>(c->var_at, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(),
                                        EpiPrimalRef{c->tau32.get(), c->c32.get(), c->lx32.get(), c->ux32.get(),

@@ -210,6 +210,42 @@ struct EpiStore {
     __device__ void operator()(int i, AT s) const { y[i] = static_cast<T>(s); }
 };
 
+// Sliced ELL row kernel of one LP (rows of A or of A^T): one wavefront per
+// slice of 64 rows, lane l owning row srow[s * 64 + l] (-1: a padding
+// lane).  The slice's entries are stored column by column, entry j of the
+// lane's row at sptr[s] + j * 64 + l, so each load of the matrix is one
+// 256-byte coalesced access and the lane walks its row with no row pointer
+// and no cross-lane reduction; padding entries hold value 0 and a column
+// of their row.  Four entries (eight loads) are in flight per lane.
+template <class Epi>
+__global__ __launch_bounds__(kBlock) void k_sell(int nslices, const int* __restrict__ sptr,
+                                                 const int* __restrict__ slen, const int* __restrict__ srow,
+                                                 const int* __restrict__ col, const float* __restrict__ val,
+                                                 const float* __restrict__ x, Epi epi) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = __builtin_amdgcn_readfirstlane(b * (kBlock / 64) + static_cast<int>(threadIdx.x) / 64);
+    if (s >= nslices) return;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int len = __builtin_amdgcn_readfirstlane(slen[s]);
+    const int base = __builtin_amdgcn_readfirstlane(sptr[s]);
+    const int* __restrict__ c = col + base + lane;
+    const float* __restrict__ v = val + base + lane;
+    float acc = 0.0f;
+    int j = 0;
+    for (; j + 3 < len; j += 4, c += 256, v += 256) {
+        const int c0 = c[0], c1 = c[64], c2 = c[128], c3 = c[192];
+        const float v0 = v[0], v1 = v[64], v2 = v[128], v3 = v[192];
+        const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        acc = __builtin_fmaf(v0, x0, acc);
+        acc = __builtin_fmaf(v1, x1, acc);
+        acc = __builtin_fmaf(v2, x2, acc);
+        acc = __builtin_fmaf(v3, x3, acc);
+    }
+    for (; j < len; ++j, c += 64, v += 64) acc = __builtin_fmaf(v[0], x[c[0]], acc);
+    const int r = srow[s * 64 + lane];
+    if (r >= 0) epi(r, acc);
+}
+
 // Column-blocked product: the launches of one product walk the column
 // blocks in order; each adds its row sums to acc[] (the first one starts
 // it), and the last hands the row total to the inner epilogue.

@@ -3,7 +3,9 @@ before any device is touched: the reference algorithm stops only on the
 reference's own test and starts only from its own point
 (pdhg_solver.cu:288-297,356), so asking it for relative-KKT termination or a
 warm start is an argument error, as are unknown algorithms and
-terminations.  Valid pairs get past the checks (on a machine without a GPU
+terminations, the batch engine's LDS kernels (PM_SPMV_STREAM,
+PM_SPMV_STAGED) and the PDLP-only layouts (PM_SPMV_SLICED,
+PM_SPMV_COLBLOCK) under the reference algorithm.  Valid pairs get past the checks (on a machine without a GPU
 they then fail for want of a device, not of arguments)."""
 import numpy as np
 import pytest
@@ -22,6 +24,10 @@ def tiny(capi):
     (dict(algorithm=7), "unknown algorithm"),
     (dict(algorithm="pdlp", termination=5), "unknown termination"),
     (dict(check_frequency=0), "check_frequency"),
+    (dict(algorithm="pdlp", spmv_variant="stream"), "batch-engine kernels"),
+    (dict(algorithm="pdlp", spmv_variant="staged"), "batch-engine kernels"),
+    (dict(algorithm="reference", spmv_variant="sliced"), "need PM_ALG_PDLP"),
+    (dict(algorithm="reference", spmv_variant="colblock"), "need PM_ALG_PDLP"),
 ])
 def test_rejected_pairs(capi, kw, why):
     with pytest.raises(capi.PramaanError) as e:
@@ -34,6 +40,8 @@ def test_rejected_pairs(capi, kw, why):
     dict(algorithm="pdlp"),
     dict(algorithm="pdlp", termination="relative_kkt"),
     dict(algorithm="pdlp", x0=np.ones(2), y0=np.zeros(1)),
+    dict(algorithm="pdlp", spmv_variant="sliced"),
+    dict(algorithm="pdlp", spmv_variant="colblock"),
 ])
 def test_accepted_pairs_pass_the_checks(capi, kw):
     if capi.device_count() > 0:
