@@ -370,7 +370,7 @@ def bench_batch(args, P) -> dict:
         "objective_rel_error_vs_closed_form_max": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "kernel_us": ms * 1e3,
-                     "alg_bytes_per_launch": by, "spmv_variants": ctx_variants},
+                     "alg_bytes_per_launch": by},
     }
 
 
