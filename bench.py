@@ -38,6 +38,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload ruiz       (Ruiz scaling of the 500k x 1M LP)
        python bench.py --workload train      (network training on 20,000 windows)
        python bench.py --workload traffic    (dispatches for rocprofv3 counters)
+       python bench.py --workload starts     (solve time by warm start: learned, exact, noisy, active set)
 """
 from __future__ import annotations
 
@@ -522,12 +523,81 @@ def bench_train(args, P) -> dict:
     }
 
 
+def bench_starts(args, P) -> dict:
+    """What a warm start can give on the rolling windows: one resident batch of
+    the windows, solved to the tolerance from starts built out of each
+    window's own cold solution (perturbed, or mixed with the network's
+    prediction, or snapped onto its true active set: columns at their
+    bound, zero duals on slack rows).  One run per start; the first (cold)
+    also pays the graph capture.  Each line: device seconds, mean and max
+    iterations, windows optimal."""
+    from pramaan_amd import rolling as R
+    from pramaan_amd import warmstart as W
+    rh = P.RollingHorizon()
+    nw = min(args.windows, rh.windows)
+    lps = [rh.window(w) for w in range(nw)]
+    graphs = [W.graph_of(lp, rh.cols_per_period, rh.rows_per_period) for lp in lps]
+    net = R.load(WEIGHTS)
+    net_starts, _ = R.predict(net, lps, graphs)
+    rng = np.random.default_rng(0)
+    out = {}
+    with P.Batch(lps, tolerance=args.tol, max_iterations=args.max_iters, check_frequency=64) as b:
+        def run(name, st=None, want=False):
+            for i in range(nw):
+                b.set_warm_start(i, *(st[i] if st is not None else (None, None)))
+            res, secs, _ = b.run(want_vectors=want)
+            it = np.array([r.iterations for r in res])
+            out[name] = {"seconds": secs, "iterations_mean": float(it.mean()), "iterations_max": int(it.max()),
+                         "optimal": int(sum(r.status == "OPTIMAL" for r in res))}
+            log(f"{name}: {json.dumps(out[name])}")
+            return res
+        cold = run("cold", want=True)
+        xs, ys = [r.x.copy() for r in cold], [r.y.copy() for r in cold]
+        run("learned", net_starts)
+        run("exact", list(zip(xs, ys)))
+        for nz in (0.01, 0.1, 0.3):
+            run(f"exact_noise_{nz}", [(np.clip(x * (1 + nz * rng.uniform(-1, 1, len(x))), lp.var_lower,
+                                               lp.var_upper), y * (1 + nz * rng.uniform(-1, 1, len(y))))
+                                      for x, y, lp in zip(xs, ys, lps)])
+        run("true_x_network_y", [(x, s[1]) for s, x in zip(net_starts, xs)])
+        run("network_x_true_y", [(s[0], y) for s, y in zip(net_starts, ys)])
+        classes, snapped = [], []
+        for (x0, y0), x, y, lp in zip(net_starts, xs, ys, lps):
+            atl = np.abs(x - lp.var_lower) <= 1e-7 * (1 + np.abs(lp.var_lower))
+            atu = np.abs(x - lp.var_upper) <= 1e-7 * (1 + np.abs(lp.var_upper))
+            cl = np.where(atl, 0, np.where(atu, 2, 1))
+            classes.append(cl)
+            snapped.append((np.where(atl, lp.var_lower, np.where(atu, lp.var_upper, x0)),
+                            np.where(np.abs(y) <= 1e-9 * (1 + np.abs(y).max()), 0.0, y0)))
+        run("network_on_true_active_set", snapped)
+        run("network_x_on_true_active_set", [(s[0], n[1]) for s, n in zip(snapped, net_starts)])
+        for q in (0.05, 0.1):
+            st = []
+            for (x0, y0), cl, lp in zip(net_starts, classes, lps):
+                c2 = np.where(rng.random(len(cl)) < q, (cl + rng.integers(1, 3, len(cl))) % 3, cl)
+                lo = np.where(np.abs(lp.var_lower) < 1e29, lp.var_lower, x0)
+                hi = np.where(np.abs(lp.var_upper) < 1e29, lp.var_upper, x0)
+                st.append((np.where(c2 == 0, lo, np.where(c2 == 2, hi, x0)), y0))
+            run(f"network_x_on_active_set_accuracy_{1 - q:.2f}", st)
+    allc = np.concatenate(classes)
+    return {
+        "metric": "device s to rel KKT, rolling-horizon batch, by warm start", "value": out["learned"]["seconds"],
+        "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": out["learned"]["seconds"] * 1e3,
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{nw} rolling-horizon windows, one batch, one solve per start", "windows": nw,
+                   "tolerance": args.tol},
+        "column_classes": {"at_lower": float((allc == 0).mean()), "interior": float((allc == 1).mean()),
+                           "at_upper": float((allc == 2).mean())},
+        "starts": out, "roofline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["rolling500", "lp1", "batch4096", "ruiz", "train", "traffic"],
+    ap.add_argument("--workload", choices=["rolling500", "lp1", "batch4096", "ruiz", "train", "traffic", "starts"],
                     default="rolling500")
     ap.add_argument("--plans", type=int, default=40, help="training plans of the train workload (500 windows each)")
     ap.add_argument("--cpu-windows", type=int, default=0,
@@ -561,6 +631,8 @@ def main() -> int:
         out = bench_train(args, P)
     elif args.workload == "traffic":
         out = bench_traffic(args, P)
+    elif args.workload == "starts":
+        out = bench_starts(args, P)
     else:
         out = bench_lp1(args, P)
     print(json.dumps(out), flush=True)
