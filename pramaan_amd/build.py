@@ -72,7 +72,8 @@ def build_library(jobs: int = 8) -> Path:
             f.result()
     objs = [str(_obj_for(s)) for s in HIP_SOURCES + HOST_SOURCES if (CSRC / s).exists()]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs], "link")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-L/opt/rocm/lib",
+          "-lrocprofiler-sdk-roctx"], "link")
     os.replace(tmp, LIB)
     return LIB
 

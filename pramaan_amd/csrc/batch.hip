@@ -1800,6 +1800,7 @@ int pm_batch_time_kernel(pm_batch* b, int32_t which, int32_t reps, double* avg_m
 }
 
 int pm_batch_create(int32_t device, int32_t count, const pm_lp* lps, const pm_pdhg_options* opts, pm_batch** out) {
+    pm::Range range_("pm_batch_create");
     return pm::guarded([&] {
         if (!out) pm::fail(PM_ERR_INVALID_ARG, "out is NULL");
         *out = nullptr;
@@ -1859,6 +1860,7 @@ int pm_batch_scaling(pm_batch* b, int32_t i, double* row_scale, double* col_scal
 }
 
 int pm_batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
+    pm::Range range_("pm_batch_run");
     return pm::guarded([&] {
         if (!b) pm::fail(PM_ERR_INVALID_ARG, "batch is NULL");
         PM_HIP(hipSetDevice(b->device));

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -39,6 +40,16 @@ inline void hip_check(hipError_t e, const char* file, int line, const char* what
 
 #define PM_HIP(call) ::pm::hip_check((call), __FILE__, __LINE__, #call)
 #define PM_LAUNCHED() ::pm::hip_check(hipGetLastError(), __FILE__, __LINE__, "kernel launch")
+
+// A named ROCTx range for the lifetime of the object: entry points and load
+// phases show up on rocprofv3's marker trace (--marker-trace); without a
+// tool attached a push or pop is a call that returns at once.
+struct Range {
+    explicit Range(const char* name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+};
 
 void set_last_error(const std::string& msg);
 void clear_last_error();

@@ -678,6 +678,7 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
 
     // SpMV variant per orientation.
     {
+        Range range_("pm_ctx_load: kernel choice");
         float* xs = c->x.get();
         launch_elem(n, FnFill<float>{xs, 1.0f}, s);
         launch_elem(m, FnFill<float>{c->y.get(), 1.0f}, s);
@@ -743,6 +744,7 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
     }
 
     if (opts.algorithm == PM_ALG_PDLP && m > 0 && nnz > 0) {
+        Range range_("pm_ctx_load: power iteration");
         // Power iteration for ||A_hat||_2 on A^T A.  Its estimate approaches
         // sigma_max from below, so it runs until the estimate settles (in
         // blocks of 16 steps after the first 64, relative change of
@@ -1281,6 +1283,7 @@ int pm_ctx_destroy(pm_ctx* c) {
 }
 
 int pm_ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* opts) {
+    pm::Range range_("pm_ctx_load");
     return pm::guarded([&] {
         if (!c) pm::fail(PM_ERR_INVALID_ARG, "ctx is NULL");
         PM_HIP(hipSetDevice(c->device));
@@ -1289,6 +1292,7 @@ int pm_ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* opts) {
 }
 
 int pm_ctx_solve(pm_ctx* c, pm_pdhg_result* res) {
+    pm::Range range_("pm_ctx_solve");
     return pm::guarded([&] {
         if (!c) pm::fail(PM_ERR_INVALID_ARG, "ctx is NULL");
         PM_HIP(hipSetDevice(c->device));
@@ -1297,6 +1301,7 @@ int pm_ctx_solve(pm_ctx* c, pm_pdhg_result* res) {
 }
 
 int pm_pdhg_solve(const pm_lp* lp, const pm_pdhg_options* opts, pm_pdhg_result* res) {
+    pm::Range range_("pm_pdhg_solve");
     // The LP and the options are checked on the host before a device is
     // touched, so bad arguments fail the same way with or without a GPU.
     int rc = pm::guarded([&] {

@@ -300,6 +300,7 @@ extern "C" {
 
 int pm_batch_learned_warm_start(pm_batch* b, pm_gnn* g, int32_t period_cols, int32_t period_rows,
                                 double* seconds) {
+    pm::Range range_("pm_batch_learned_warm_start");
     return pm::guarded([&] {
         if (!b || !g) pm::fail(PM_ERR_INVALID_ARG, "NULL batch or network");
         if (b->device != g->device) pm::fail(PM_ERR_INVALID_ARG, "batch and network live on different devices");

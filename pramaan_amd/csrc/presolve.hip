@@ -351,6 +351,7 @@ int pm_ruiz_scale_noalloc(int32_t m, int32_t n, int32_t nnz, const int32_t* row_
 int pm_ruiz_scale(int32_t m, int32_t n, const int32_t* row_ptr, const int32_t* col_idx, double* values, double* c,
                   double* row_lower, double* row_upper, double* var_lower, double* var_upper, int32_t passes,
                   double* row_scale, double* col_scale) {
+    pm::Range range_("pm_ruiz_scale");
     return pm::guarded([&] {
         pm::check_inputs(m, n, row_ptr, col_idx, values, c, row_lower, row_upper, var_lower, var_upper, passes,
                          row_scale, col_scale);
