@@ -170,7 +170,12 @@ typedef struct pm_pdhg_result {
     double  dual_objective;
     double  primal_weight;
     double  device_seconds;    /* wall time of the device solve loop         */
-    double* x;                 /* caller-owned [n] or NULL                   */
+    double* x;                 /* caller-owned [n] or NULL.  PM_ALG_PDLP
+                                  without convergence (iteration limit,
+                                  numerical failure): the checked candidate
+                                  of lowest rel_kkt, which every field here
+                                  describes; PM_ALG_REFERENCE: the last
+                                  iterate, as the reference returns it     */
     double* y;                 /* caller-owned [m] or NULL (reference sign:
                                   reduced cost is c + A^T y)                 */
     double* row_activity;      /* caller-owned [m] or NULL: A x in FP64      */

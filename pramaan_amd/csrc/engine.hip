@@ -204,6 +204,7 @@ struct pm_ctx {
     DevBuf<float> x, y, xe, x0, y0, xt, yt, xl, yl, tmp_n, tmp_m;
     // FP64 vectors.
     DevBuf<double> xtot, ytot, xbase, ybase, act64;
+    DevBuf<double> xbest, ybest;  // PDLP: the checked candidate of lowest relative KKT error
     DevBuf<double> partials, red;
     PinnedBuf<double> hred;
 
@@ -592,8 +593,8 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
     c->hred.resize(64);
     for (auto* b : {&c->x, &c->xe, &c->x0, &c->xt, &c->xl, &c->tmp_n}) b->resize(n);
     for (auto* b : {&c->y, &c->y0, &c->yt, &c->yl, &c->tmp_m}) b->resize(std::max(m, 1));
-    for (auto* b : {&c->xtot, &c->xbase}) b->resize(n);
-    for (auto* b : {&c->ytot, &c->ybase, &c->act64}) b->resize(std::max(m, 1));
+    for (auto* b : {&c->xtot, &c->xbase, &c->xbest}) b->resize(n);
+    for (auto* b : {&c->ytot, &c->ybase, &c->act64, &c->ybest}) b->resize(std::max(m, 1));
 
     if (opts.algorithm == PM_ALG_REFERENCE) {
         // Canonicalisation and diagonal steps as pdhg_solver.cu:196-250.
@@ -1100,6 +1101,10 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
     const auto t0 = std::chrono::steady_clock::now();
     bool have_candidate = false;
     bool round_start = true;
+    // A solve that ends without meeting the tolerance returns the checked
+    // candidate of lowest relative KKT error, not the last one: late in a
+    // stalled solve the error can rise again after a restart.
+    double best_rel = INFINITY;
     for (;;) {
         if (round_start) {
             round_data(c, ctl.DP, ctl.DD);
@@ -1119,6 +1124,11 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
                         s);
         have_candidate = true;
         const Kkt kk = eval_kkt(c, c->xtot.get(), c->ytot.get());
+        if (kk.rel < best_rel) {
+            best_rel = kk.rel;
+            launch_elem(n, FnCopy<double>{c->xbest.get(), c->xtot.get()}, s);
+            if (m > 0) launch_elem(m, FnCopy<double>{c->ybest.get(), c->ytot.get()}, s);
+        }
         const int act = ctl.on_check(kk.rel, kk.rel_p, kk.rel_d, kk.rel_g);
         if (o.verbose)
             fprintf(stderr, "[pramaan] it %d round %d kkt %.3e (p %.2e d %.2e g %.2e) omega %.3g restarts %d\n",
@@ -1164,7 +1174,10 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
     res->restarts = ctl.restarts;
     res->refinements = ctl.rounds;
     res->primal_weight = ctl.omega;
-    if (!have_candidate) {
+    if (res->status != PM_SOLVE_OPTIMAL && best_rel < INFINITY) {
+        launch_elem(n, FnCopy<double>{c->xtot.get(), c->xbest.get()}, s);
+        if (m > 0) launch_elem(m, FnCopy<double>{c->ytot.get(), c->ybest.get()}, s);
+    } else if (!have_candidate) {
         // No iteration since the last base point: report the base point.
         launch_elem(n, FnCopy<double>{c->xtot.get(), c->xbase.get()}, s);
         if (m > 0) launch_elem(m, FnCopy<double>{c->ytot.get(), c->ybase.get()}, s);
