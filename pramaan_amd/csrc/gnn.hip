@@ -2,7 +2,8 @@
 // (include/pramaan_gnn.h).
 //
 // Activations are N x 64 FP32 row-major (one 256-byte row per node).  The
-// 64x64 dense layers run on MFMA (v_mfma_f32_32x32x8_bf16): one wavefront
+// 64x64 dense layers run on gfx950's MFMA (v_mfma_f32_32x32x16_bf16, K = 16
+// per instruction): one wavefront
 // computes a 32-row x 64-column output tile as two 32x32 accumulators,
 // converting FP32 activations to bfloat16 (round to nearest even) as it
 // loads them and reading weights from a pre-converted bfloat16 copy laid
@@ -27,7 +28,8 @@ namespace {
 constexpr int kH = 64, kF = PM_GNN_FEATURES;
 constexpr float kOmegaLossWeight = 0.01f;  // weight of the primal-weight head's term in the loss
 static_assert(kF % 4 == 0, "features are loaded four at a time");
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ short f2bf(float f) {
@@ -39,6 +41,13 @@ __device__ __forceinline__ short f2bf(float f) {
 // Row of a 32x32 MFMA accumulator element e held by `lane`
 // (column = lane % 32).
 __device__ __forceinline__ int acc_row(int e, int lane) { return 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3); }
+
+// One 32x32x16 step: lane l supplies A[l % 32][8 (l / 32) .. + 7] and
+// B[8 (l / 32) .. + 7][l % 32], eight bfloat16 values each.
+__device__ __forceinline__ f32x16 mfma16(s16x8 a, s16x8 b, f32x16 acc) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                   0, 0, 0);
+}
 
 // Out[N][64] = act(A1 B1 (+ A2 B2) (+ bias)) (+ Out when ACC), with
 // B given as BT[c][k] = B[k][c] in bfloat16.  Block = 4 waves = 128 rows.
@@ -58,16 +67,21 @@ __global__ __launch_bounds__(256) void k_dense(int N, const float* __restrict__ 
         const float* A = in == 0 ? A1 : A2;
         const short* BT = in == 0 ? B1T : B2T;
 #pragma unroll
-        for (int ks = 0; ks < kH / 8; ++ks) {
-            const int k0 = ks * 8 + 4 * kg;
-            float4 av = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (r < N) av = *reinterpret_cast<const float4*>(A + static_cast<size_t>(r) * kH + k0);
-            bf16x4 a;
+        for (int ks = 0; ks < kH / 16; ++ks) {
+            const int k0 = ks * 16 + 8 * kg;
+            float4 av = make_float4(0.f, 0.f, 0.f, 0.f), aw = av;
+            if (r < N) {
+                const float4* ap = reinterpret_cast<const float4*>(A + static_cast<size_t>(r) * kH + k0);
+                av = ap[0];
+                aw = ap[1];
+            }
+            s16x8 a;
             a[0] = f2bf(av.x); a[1] = f2bf(av.y); a[2] = f2bf(av.z); a[3] = f2bf(av.w);
-            const bf16x4 b0 = *reinterpret_cast<const bf16x4*>(BT + c * kH + k0);
-            const bf16x4 b1 = *reinterpret_cast<const bf16x4*>(BT + (32 + c) * kH + k0);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+            a[4] = f2bf(aw.x); a[5] = f2bf(aw.y); a[6] = f2bf(aw.z); a[7] = f2bf(aw.w);
+            const s16x8 b0 = *reinterpret_cast<const s16x8*>(BT + c * kH + k0);
+            const s16x8 b1 = *reinterpret_cast<const s16x8*>(BT + (32 + c) * kH + k0);
+            acc0 = mfma16(a, b0, acc0);
+            acc1 = mfma16(a, b1, acc1);
         }
     }
 #pragma unroll
@@ -96,16 +110,16 @@ __global__ __launch_bounds__(256) void k_wgrad(int N, int rows_per, const float*
     f32x16 acc;
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-    for (int r0 = rb; r0 < re; r0 += 8) {
-        bf16x4 a, b;
+    for (int r0 = rb; r0 < re; r0 += 16) {
+        s16x8 a, b;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int r = r0 + 4 * kg + q;
+        for (int q = 0; q < 8; ++q) {
+            const int r = r0 + 8 * kg + q;
             const bool ok = r < re;
             a[q] = f2bf(ok ? A[static_cast<size_t>(r) * kH + 32 * ti + c] : 0.0f);
             b[q] = f2bf(ok ? D[static_cast<size_t>(r) * kH + 32 * tj + c] : 0.0f);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        acc = mfma16(a, b, acc);
     }
     float* out = P + static_cast<size_t>(blockIdx.x) * kH * kH;
 #pragma unroll
