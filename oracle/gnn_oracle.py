@@ -18,9 +18,12 @@ Network (bipartite row/column message passing, hidden H = 64):
           Adam scales each parameter's step, so the head learns at full speed
           while its term moves the shared layers little)
 Ar is A with rows scaled by 1/max|row| (m x n); Ac is A^T with rows scaled
-by 1/max|column| (n x m).  With bf16=True every 64x64 product rounds its two
-inputs to bfloat16 (round to nearest even) and accumulates in FP64, as the
-MFMA kernels do with FP32 accumulation.
+by 1/max|column| (n x m).  With bf16=True every activation (Hc, Hr, Zr, Zc)
+is stored rounded to bfloat16 (round to nearest even) where it is produced,
+as the HIP kernels store them, and every 64x64 product rounds its two inputs
+to bfloat16 and accumulates in FP64, as the MFMA kernels do with FP32
+accumulation.  Gradients stay unrounded except as inputs of those products
+(the rounding passes gradients through unchanged).
 """
 from __future__ import annotations
 
@@ -109,15 +112,16 @@ def _mm(a, b, use_bf16):
 
 def forward(p, g: Graph, Xc, Xr, use_bf16=True, seg=None):
     relu = lambda t: np.maximum(t, 0.0)
+    st = bf16 if use_bf16 else (lambda t: t)  # activation storage
     c = {}
-    c["Hc0"] = relu(Xc @ p["We_c"] + p["be_c"])
-    c["Hr0"] = relu(Xr @ p["We_r"] + p["be_r"])
+    c["Hc0"] = st(relu(Xc @ p["We_c"] + p["be_c"]))
+    c["Hr0"] = st(relu(Xr @ p["We_r"] + p["be_r"]))
     Hc, Hr = c["Hc0"], c["Hr0"]
     for l in (1, 2):
-        Zr = g.row_agg(Hc)
-        Hr = relu(_mm(Hr, p[f"Wrr{l}"], use_bf16) + _mm(Zr, p[f"Wrz{l}"], use_bf16) + p[f"br{l}"])
-        Zc = g.col_agg(Hr)
-        Hc = relu(_mm(Hc, p[f"Wcc{l}"], use_bf16) + _mm(Zc, p[f"Wcz{l}"], use_bf16) + p[f"bc{l}"])
+        Zr = st(g.row_agg(Hc))
+        Hr = st(relu(_mm(Hr, p[f"Wrr{l}"], use_bf16) + _mm(Zr, p[f"Wrz{l}"], use_bf16) + p[f"br{l}"]))
+        Zc = st(g.col_agg(Hr))
+        Hc = st(relu(_mm(Hc, p[f"Wcc{l}"], use_bf16) + _mm(Zc, p[f"Wcz{l}"], use_bf16) + p[f"bc{l}"]))
         c[f"Zr{l}"], c[f"Hr{l}"], c[f"Zc{l}"], c[f"Hc{l}"] = Zr, Hr, Zc, Hc
     c["xhat"] = Hc @ p["wx"] + p["bx"][0]
     c["yhat"] = Hr @ p["wy"] + p["by"][0]

@@ -1,17 +1,18 @@
 // gnn.hip -- forward, backward and Adam kernels of the warm-start network
 // (include/pramaan_gnn.h).
 //
-// Activations are N x 64 FP32 row-major (one 256-byte row per node).  The
-// 64x64 dense layers run on gfx950's MFMA (v_mfma_f32_32x32x16_bf16, K = 16
-// per instruction): one wavefront
+// Activations are N x 64 bfloat16 row-major (one 128-byte row per node),
+// rounded to nearest even once, where they are produced; gradients are
+// FP32.  The 64x64 dense layers run on gfx950's MFMA
+// (v_mfma_f32_32x32x16_bf16, K = 16 per instruction): one wavefront
 // computes a 32-row x 64-column output tile as two 32x32 accumulators,
-// converting FP32 activations to bfloat16 (round to nearest even) as it
-// loads them and reading weights from a pre-converted bfloat16 copy laid
-// out k-contiguous per output column.  Weight gradients (64x64, summed over
+// loading eight bfloat16 activations (or FP32 gradients, rounded as they
+// are loaded) per lane and step, and reading weights from a pre-converted
+// bfloat16 copy laid out k-contiguous per output column.  Weight gradients (64x64, summed over
 // all nodes) use the same instruction with K running over nodes, one
 // partial per workgroup, reduced in a fixed order.  Message passing is a
-// sparse-times-dense product with one wavefront per output node and one
-// lane per feature (64 lanes = 64 features, coalesced 256-byte rows).
+// sparse-times-dense product with eight lanes per output node and eight
+// features per lane.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -37,6 +38,32 @@ __device__ __forceinline__ short f2bf(float f) {
     u += 0x7FFFu + ((u >> 16) & 1u);
     return static_cast<short>(u >> 16);
 }
+__device__ __forceinline__ float bf2f(short h) { return __uint_as_float(static_cast<unsigned>(static_cast<unsigned short>(h)) << 16); }
+
+// Eight consecutive values of a row as bfloat16 bits (16-byte load of an
+// activation row, or two 16-byte loads of an FP32 gradient row rounded).
+__device__ __forceinline__ s16x8 load8_bf(const short* p) { return *reinterpret_cast<const s16x8*>(p); }
+__device__ __forceinline__ s16x8 load8_bf(const float* p) {
+    const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+    s16x8 r;
+    r[0] = f2bf(a.x); r[1] = f2bf(a.y); r[2] = f2bf(a.z); r[3] = f2bf(a.w);
+    r[4] = f2bf(b.x); r[5] = f2bf(b.y); r[6] = f2bf(b.z); r[7] = f2bf(b.w);
+    return r;
+}
+// Eight consecutive values of a row as FP32.
+__device__ __forceinline__ void load8_f(const short* p, float (&v)[8]) {
+    const s16x8 h = *reinterpret_cast<const s16x8*>(p);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = bf2f(h[q]);
+}
+__device__ __forceinline__ void load8_f(const float* p, float (&v)[8]) {
+    const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void store_val(short* p, float v) { *p = f2bf(v); }
+__device__ __forceinline__ void store_val(float* p, float v) { *p = v; }
+__device__ __forceinline__ float load_val(const short* p) { return bf2f(*p); }
+__device__ __forceinline__ float load_val(const float* p) { return *p; }
 
 // Row of a 32x32 MFMA accumulator element e held by `lane`
 // (column = lane % 32).
@@ -51,10 +78,11 @@ __device__ __forceinline__ f32x16 mfma16(s16x8 a, s16x8 b, f32x16 acc) {
 
 // Out[N][64] = act(A1 B1 (+ A2 B2) (+ bias)) (+ Out when ACC), with
 // B given as BT[c][k] = B[k][c] in bfloat16.  Block = 4 waves = 128 rows.
-template <bool TWO, bool BIAS, bool RELU, bool ACC>
-__global__ __launch_bounds__(256) void k_dense(int N, const float* __restrict__ A1, const short* __restrict__ B1T,
-                                               const float* __restrict__ A2, const short* __restrict__ B2T,
-                                               const float* __restrict__ bias, float* __restrict__ Out) {
+// AT / OT: short (bfloat16 activations) or float (FP32 gradients).
+template <bool TWO, bool BIAS, bool RELU, bool ACC, class AT, class OT>
+__global__ __launch_bounds__(256) void k_dense(int N, const AT* __restrict__ A1, const short* __restrict__ B1T,
+                                               const AT* __restrict__ A2, const short* __restrict__ B2T,
+                                               const float* __restrict__ bias, OT* __restrict__ Out) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row0 = (blockIdx.x * 4 + w) * 32;
     if (row0 >= N) return;
@@ -64,20 +92,13 @@ __global__ __launch_bounds__(256) void k_dense(int N, const float* __restrict__ 
     for (int e = 0; e < 16; ++e) acc0[e] = acc1[e] = 0.0f;
 #pragma unroll
     for (int in = 0; in < (TWO ? 2 : 1); ++in) {
-        const float* A = in == 0 ? A1 : A2;
+        const AT* A = in == 0 ? A1 : A2;
         const short* BT = in == 0 ? B1T : B2T;
 #pragma unroll
         for (int ks = 0; ks < kH / 16; ++ks) {
             const int k0 = ks * 16 + 8 * kg;
-            float4 av = make_float4(0.f, 0.f, 0.f, 0.f), aw = av;
-            if (r < N) {
-                const float4* ap = reinterpret_cast<const float4*>(A + static_cast<size_t>(r) * kH + k0);
-                av = ap[0];
-                aw = ap[1];
-            }
-            s16x8 a;
-            a[0] = f2bf(av.x); a[1] = f2bf(av.y); a[2] = f2bf(av.z); a[3] = f2bf(av.w);
-            a[4] = f2bf(aw.x); a[5] = f2bf(aw.y); a[6] = f2bf(aw.z); a[7] = f2bf(aw.w);
+            s16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (r < N) a = load8_bf(A + static_cast<size_t>(r) * kH + k0);
             const s16x8 b0 = *reinterpret_cast<const s16x8*>(BT + c * kH + k0);
             const s16x8 b1 = *reinterpret_cast<const s16x8*>(BT + (32 + c) * kH + k0);
             acc0 = mfma16(a, b0, acc0);
@@ -91,16 +112,17 @@ __global__ __launch_bounds__(256) void k_dense(int N, const float* __restrict__ 
         float v0 = acc0[e], v1 = acc1[e];
         if (BIAS) { v0 += bias[c]; v1 += bias[32 + c]; }
         if (RELU) { v0 = fmaxf(v0, 0.0f); v1 = fmaxf(v1, 0.0f); }
-        float* o = Out + static_cast<size_t>(row) * kH;
-        if (ACC) { o[c] += v0; o[32 + c] += v1; }
-        else { o[c] = v0; o[32 + c] = v1; }
+        OT* o = Out + static_cast<size_t>(row) * kH;
+        if (ACC) { v0 += load_val(o + c); v1 += load_val(o + 32 + c); }
+        store_val(o + c, v0);
+        store_val(o + 32 + c, v1);
     }
 }
 
 // Partial weight gradient over rows [b*rows_per, (b+1)*rows_per):
 // P[b][i][j] = sum_r A[r][i] D[r][j] (bfloat16 inputs, FP32 accumulate).
 // Wave w computes the 32x32 tile (w / 2, w % 2).
-__global__ __launch_bounds__(256) void k_wgrad(int N, int rows_per, const float* __restrict__ A,
+__global__ __launch_bounds__(256) void k_wgrad(int N, int rows_per, const short* __restrict__ A,
                                                const float* __restrict__ D, float* __restrict__ P) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ti = w >> 1, tj = w & 1, kg = lane >> 5, c = lane & 31;
@@ -116,7 +138,7 @@ __global__ __launch_bounds__(256) void k_wgrad(int N, int rows_per, const float*
         for (int q = 0; q < 8; ++q) {
             const int r = r0 + 8 * kg + q;
             const bool ok = r < re;
-            a[q] = f2bf(ok ? A[static_cast<size_t>(r) * kH + 32 * ti + c] : 0.0f);
+            a[q] = ok ? A[static_cast<size_t>(r) * kH + 32 * ti + c] : static_cast<short>(0);
             b[q] = f2bf(ok ? D[static_cast<size_t>(r) * kH + 32 * tj + c] : 0.0f);
         }
         acc = mfma16(a, b, acc);
@@ -173,17 +195,19 @@ __global__ __launch_bounds__(256) void k_sum_partials(int nb, int width, const f
 }
 
 // Out[i][:] (+)= sum_k val[k] H[idx[k]][:], eight lanes per row (eight
-// features each, two 16-byte loads), so a wavefront walks eight rows at
-// once.  The grid is 8 x per blocks of 32 rows; block b runs on XCD b % 8,
-// which takes the (b % 8)-th eighth of the rows in order, so the rows of a
-// window (and the H rows they gather) stay in one XCD's L2.  Entries are
-// taken eight at a time (indexes and values, then the gathers, then the
-// FMAs in entry order), so each feature's sum is the same as one entry at a
-// time and up to 64 gathers are in flight per wavefront.
-template <bool ACC>
+// features each: one 16-byte load of a bfloat16 activation row, two of an
+// FP32 gradient row), so a wavefront walks eight rows at once.  The grid
+// is 8 x per blocks of 32 rows; block b runs on XCD b % 8, which takes the
+// (b % 8)-th eighth of the rows in order, so the rows of a window (and the
+// H rows they gather) stay in one XCD's L2.  Entries are taken eight at a
+// time (indexes and values, then the gathers, then the FMAs in entry
+// order), so each feature's sum is the same as one entry at a time and up
+// to 64 gathers are in flight per wavefront.  IT / OT: short (bfloat16
+// activations) or float (FP32 gradients).
+template <bool ACC, class IT, class OT>
 __global__ __launch_bounds__(256) void k_spmm(int nrows, int per, const int* __restrict__ rp,
                                               const int* __restrict__ idx, const float* __restrict__ val,
-                                              const float* __restrict__ Hin, float* __restrict__ Out) {
+                                              const IT* __restrict__ Hin, OT* __restrict__ Out) {
     const int blk = (static_cast<int>(blockIdx.x) & 7) * per + (static_cast<int>(blockIdx.x) >> 3);
     const int row = blk * 32 + (threadIdx.x >> 3), f = (threadIdx.x & 7) * 8;
     if (row >= nrows) return;
@@ -193,50 +217,46 @@ __global__ __launch_bounds__(256) void k_spmm(int nrows, int per, const int* __r
     for (; k + 8 <= k1; k += 8) {
         int i[8];
         float v[8];
-        float4 a[8], b[8];
+        float h[8][8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) { i[u] = idx[k + u]; v[u] = val[k + u]; }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const float4* h = reinterpret_cast<const float4*>(Hin + static_cast<size_t>(i[u]) * kH + f);
-            a[u] = h[0];
-            b[u] = h[1];
-        }
+        for (int u = 0; u < 8; ++u) load8_f(Hin + static_cast<size_t>(i[u]) * kH + f, h[u]);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            s[0] = __builtin_fmaf(v[u], a[u].x, s[0]); s[1] = __builtin_fmaf(v[u], a[u].y, s[1]);
-            s[2] = __builtin_fmaf(v[u], a[u].z, s[2]); s[3] = __builtin_fmaf(v[u], a[u].w, s[3]);
-            s[4] = __builtin_fmaf(v[u], b[u].x, s[4]); s[5] = __builtin_fmaf(v[u], b[u].y, s[5]);
-            s[6] = __builtin_fmaf(v[u], b[u].z, s[6]); s[7] = __builtin_fmaf(v[u], b[u].w, s[7]);
-        }
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s[q] = __builtin_fmaf(v[u], h[u][q], s[q]);
     }
     for (; k < k1; ++k) {
-        const float4* h = reinterpret_cast<const float4*>(Hin + static_cast<size_t>(idx[k]) * kH + f);
-        const float4 a = h[0], b = h[1];
+        float h[8];
+        load8_f(Hin + static_cast<size_t>(idx[k]) * kH + f, h);
         const float vk = val[k];
-        s[0] = __builtin_fmaf(vk, a.x, s[0]); s[1] = __builtin_fmaf(vk, a.y, s[1]);
-        s[2] = __builtin_fmaf(vk, a.z, s[2]); s[3] = __builtin_fmaf(vk, a.w, s[3]);
-        s[4] = __builtin_fmaf(vk, b.x, s[4]); s[5] = __builtin_fmaf(vk, b.y, s[5]);
-        s[6] = __builtin_fmaf(vk, b.z, s[6]); s[7] = __builtin_fmaf(vk, b.w, s[7]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[q] = __builtin_fmaf(vk, h[q], s[q]);
     }
-    float4* o = reinterpret_cast<float4*>(Out + static_cast<size_t>(row) * kH + f);
-    if (ACC) {
-        float4 t0 = o[0], t1 = o[1];
-        t0.x += s[0]; t0.y += s[1]; t0.z += s[2]; t0.w += s[3];
-        t1.x += s[4]; t1.y += s[5]; t1.z += s[6]; t1.w += s[7];
-        o[0] = t0;
-        o[1] = t1;
+    OT* o = Out + static_cast<size_t>(row) * kH + f;
+    if constexpr (ACC) {
+        float t[8];
+        load8_f(o, t);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[q] += t[q];
+    }
+    if constexpr (sizeof(OT) == 2) {
+        s16x8 r;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r[q] = f2bf(s[q]);
+        *reinterpret_cast<s16x8*>(o) = r;
     } else {
-        o[0] = make_float4(s[0], s[1], s[2], s[3]);
-        o[1] = make_float4(s[4], s[5], s[6], s[7]);
+        reinterpret_cast<float4*>(o)[0] = make_float4(s[0], s[1], s[2], s[3]);
+        reinterpret_cast<float4*>(o)[1] = make_float4(s[4], s[5], s[6], s[7]);
     }
 }
 
-// Input layer: Out = relu(X W + b), X [N][kF], W [kF][64] FP32.  Sixteen
-// lanes per row, four outputs each (one 16-byte store), so a wavefront
-// writes four rows.
+// Input layer: Out = relu(X W + b), X [N][kF], W [kF][64] FP32, Out in
+// bfloat16.  Sixteen lanes per row, four outputs each (one 8-byte store),
+// so a wavefront writes four rows.
 __global__ __launch_bounds__(256) void k_input(int N, const float* __restrict__ X, const float* __restrict__ W,
-                                               const float* __restrict__ b, float* __restrict__ Out) {
+                                               const float* __restrict__ b, short* __restrict__ Out) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     const int64_t r = t >> 4;
     const int c = static_cast<int>(t & 15) * 4;
@@ -260,20 +280,23 @@ __global__ __launch_bounds__(256) void k_input(int N, const float* __restrict__ 
         s[2] = __builtin_fmaf(xs[k], w.z, s[2]);
         s[3] = __builtin_fmaf(xs[k], w.w, s[3]);
     }
-    *reinterpret_cast<float4*>(Out + r * kH + c) =
-        make_float4(fmaxf(s[0], 0.0f), fmaxf(s[1], 0.0f), fmaxf(s[2], 0.0f), fmaxf(s[3], 0.0f));
+    typedef short s16x4 __attribute__((ext_vector_type(4)));
+    s16x4 o;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = f2bf(fmaxf(s[q], 0.0f));
+    *reinterpret_cast<s16x4*>(Out + r * kH + c) = o;
 }
 
 // Primal-weight head, one workgroup per segment (LP) k: pool[k][c] = mean
 // of H[j][c] over the segment's columns (four column groups summed in LDS
 // in a fixed order), what[k] = pool[k] . w + b.
-__global__ __launch_bounds__(256) void k_pool_head(const int* __restrict__ seg, const float* __restrict__ H,
+__global__ __launch_bounds__(256) void k_pool_head(const int* __restrict__ seg, const short* __restrict__ H,
                                                    const float* __restrict__ w, const float* __restrict__ b,
                                                    float* __restrict__ pool, float* __restrict__ what) {
     const int k = blockIdx.x, c = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int j0 = seg[k], j1 = seg[k + 1];
     float s = 0.0f;
-    for (int j = j0 + q; j < j1; j += 4) s += H[static_cast<size_t>(j) * kH + c];
+    for (int j = j0 + q; j < j1; j += 4) s += bf2f(H[static_cast<size_t>(j) * kH + c]);
     __shared__ float red[4][kH];
     __shared__ float dot[kH];
     red[q][c] = s;
@@ -340,19 +363,18 @@ __global__ __launch_bounds__(256) void k_omega_bwd(int n, int count, const int* 
 }
 
 // Head: out[r] = H[r] . w + b.
-__global__ __launch_bounds__(256) void k_head(int N, const float* __restrict__ Hin, const float* __restrict__ w,
+__global__ __launch_bounds__(256) void k_head(int N, const short* __restrict__ Hin, const float* __restrict__ w,
                                               const float* __restrict__ b, float* __restrict__ out) {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= N) return;
-    const float4* h = reinterpret_cast<const float4*>(Hin + static_cast<size_t>(r) * kH);
+    const short* h = Hin + static_cast<size_t>(r) * kH;
     float s = 0.0f;
 #pragma unroll
-    for (int q = 0; q < kH / 4; ++q) {
-        const float4 v = h[q];
-        s = __builtin_fmaf(v.x, w[4 * q], s);
-        s = __builtin_fmaf(v.y, w[4 * q + 1], s);
-        s = __builtin_fmaf(v.z, w[4 * q + 2], s);
-        s = __builtin_fmaf(v.w, w[4 * q + 3], s);
+    for (int q = 0; q < kH / 8; ++q) {
+        float v[8];
+        load8_f(h + 8 * q, v);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s = __builtin_fmaf(v[e], w[8 * q + e], s);
     }
     out[r] = s + b[0];
 }
@@ -387,15 +409,15 @@ __global__ __launch_bounds__(256) void k_head_bwd(int N, const float* __restrict
 }
 
 // dP = dH * (Hout > 0), in place.
-__global__ __launch_bounds__(256) void k_relu_bwd(size_t total, const float* __restrict__ Hout, float* __restrict__ dH) {
+__global__ __launch_bounds__(256) void k_relu_bwd(size_t total, const short* __restrict__ Hout, float* __restrict__ dH) {
     const size_t t = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
     if (t >= total) return;
-    if (!(Hout[t] > 0.0f)) dH[t] = 0.0f;
+    if (!(bf2f(Hout[t]) > 0.0f)) dH[t] = 0.0f;
 }
 
 // Partial d-weighted row sums for the head weights: P[b][c] = sum_r d[r] H[r][c].
 __global__ __launch_bounds__(256) void k_head_wgrad(int N, int rows_per, const float* __restrict__ d,
-                                                    const float* __restrict__ Hin, float* __restrict__ P,
+                                                    const short* __restrict__ Hin, float* __restrict__ P,
                                                     float* __restrict__ Pb) {
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int rb = blockIdx.x * rows_per;
@@ -403,7 +425,7 @@ __global__ __launch_bounds__(256) void k_head_wgrad(int N, int rows_per, const f
     if (re > N) re = N;
     float s = 0.0f, sb = 0.0f;
     for (int r = rb + g; r < re; r += 4) {
-        s += d[r] * Hin[static_cast<size_t>(r) * kH + c];
+        s += d[r] * bf2f(Hin[static_cast<size_t>(r) * kH + c]);
         sb += d[r];
     }
     __shared__ float red[4][kH];
@@ -506,35 +528,35 @@ void prep_weights(pm_gnn* g) {
 const short* WT(pm_gnn* g, int l, int w) { return g->wt.get() + widx(l, w) * kH * kH; }
 const short* WN(pm_gnn* g, int l, int w) { return g->wn.get() + widx(l, w) * kH * kH; }
 
-void dense(pm_gnn* g, int N, const float* A1, const short* B1, const float* A2, const short* B2, const float* bias,
-           float* out, bool relu, bool acc) {
+// Forward layer: out = relu(A1 B1 + A2 B2 + bias), bfloat16 in and out.
+void dense(pm_gnn* g, int N, const short* A1, const short* B1, const short* A2, const short* B2, const float* bias,
+           short* out) {
     if (N <= 0) return;
-    const dim3 grid(cdiv(N, 128)), blk(256);
-    if (A2 && bias && relu && !acc)
-        hipLaunchKernelGGL((k_dense<true, true, true, false>), grid, blk, 0, g->s, N, A1, B1, A2, B2, bias, out);
-    else if (!A2 && !bias && !relu && !acc)
-        hipLaunchKernelGGL((k_dense<false, false, false, false>), grid, blk, 0, g->s, N, A1, B1, A2, B2, bias, out);
-    else if (!A2 && !bias && !relu && acc)
-        hipLaunchKernelGGL((k_dense<false, false, false, true>), grid, blk, 0, g->s, N, A1, B1, A2, B2, bias, out);
-    else
-        fail(PM_ERR_INTERNAL, "unsupported dense-layer configuration");
+    hipLaunchKernelGGL((k_dense<true, true, true, false, short, short>), dim3(cdiv(N, 128)), dim3(256), 0, g->s, N,
+                       A1, B1, A2, B2, bias, out);
     PM_LAUNCHED();
 }
 
-void spmm(pm_gnn* g, int nrows, const int* rp, const int* idx, const float* val, const float* Hin, float* out,
-          bool acc) {
+// Input gradient: out = D B, FP32 in and out.
+void dense_grad(pm_gnn* g, int N, const float* D, const short* B, float* out) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL((k_dense<false, false, false, false, float, float>), dim3(cdiv(N, 128)), dim3(256), 0, g->s,
+                       N, D, B, nullptr, nullptr, nullptr, out);
+    PM_LAUNCHED();
+}
+
+template <bool ACC, class IT, class OT>
+void spmm(pm_gnn* g, int nrows, const int* rp, const int* idx, const float* val, const IT* Hin, OT* out) {
     if (nrows <= 0) return;
     const int per = cdiv(cdiv(nrows, 32), 8);
-    const dim3 grid(8 * per);
-    if (acc) hipLaunchKernelGGL(k_spmm<true>, grid, dim3(256), 0, g->s, nrows, per, rp, idx, val, Hin, out);
-    else hipLaunchKernelGGL(k_spmm<false>, grid, dim3(256), 0, g->s, nrows, per, rp, idx, val, Hin, out);
+    hipLaunchKernelGGL((k_spmm<ACC, IT, OT>), dim3(8 * per), dim3(256), 0, g->s, nrows, per, rp, idx, val, Hin, out);
     PM_LAUNCHED();
 }
 
 constexpr int kRowsPer = 2048;  // rows per workgroup in the gradient reductions
 
 // grad[off .. off+4096) = A^T D summed over all N rows.
-void wgrad(pm_gnn* g, int N, const float* A, const float* D, int off) {
+void wgrad(pm_gnn* g, int N, const short* A, const float* D, int off) {
     if (N <= 0) return fill(g, g->grad.get() + off, kH * kH, 0.0f);
     const int nb = cdiv(N, kRowsPer);
     hipLaunchKernelGGL(k_wgrad, dim3(nb), dim3(256), 0, g->s, N, kRowsPer, A, D, g->part.get());
@@ -565,7 +587,7 @@ void colsum(pm_gnn* g, int N, const float* D, const float* X, int boff, int xoff
     }
 }
 
-void head_grads(pm_gnn* g, int N, const float* d, const float* Hin, int woff_, int boff) {
+void head_grads(pm_gnn* g, int N, const float* d, const short* Hin, int woff_, int boff) {
     if (N <= 0) {
         fill(g, g->grad.get() + woff_, kH, 0.0f);
         fill(g, g->grad.get() + boff, 1, 0.0f);
@@ -582,7 +604,7 @@ void head_grads(pm_gnn* g, int N, const float* d, const float* Hin, int woff_, i
     PM_LAUNCHED();
 }
 
-void relu_bwd(pm_gnn* g, int N, const float* Hout, float* dH) {
+void relu_bwd(pm_gnn* g, int N, const short* Hout, float* dH) {
     if (N <= 0) return;
     const size_t total = static_cast<size_t>(N) * kH;
     hipLaunchKernelGGL(k_relu_bwd, dim3(cdiv(static_cast<long long>(total), 256)), dim3(256), 0, g->s, total, Hout, dH);
@@ -600,12 +622,10 @@ void forward(pm_gnn* g) {
                            g->xr.get(), P + kOff.We_r, P + kOff.be_r, g->Hr[0].get());
     PM_LAUNCHED();
     for (int l = 1; l <= 2; ++l) {
-        spmm(g, m, g->rp.get(), g->ci.get(), g->ar.get(), g->Hc[l - 1].get(), g->Zr[l].get(), false);
-        dense(g, m, g->Hr[l - 1].get(), WT(g, l, 0), g->Zr[l].get(), WT(g, l, 1), P + kOff.br[l], g->Hr[l].get(), true,
-              false);
-        spmm(g, n, g->cp.get(), g->ri.get(), g->ac.get(), g->Hr[l].get(), g->Zc[l].get(), false);
-        dense(g, n, g->Hc[l - 1].get(), WT(g, l, 2), g->Zc[l].get(), WT(g, l, 3), P + kOff.bc[l], g->Hc[l].get(), true,
-              false);
+        spmm<false>(g, m, g->rp.get(), g->ci.get(), g->ar.get(), g->Hc[l - 1].get(), g->Zr[l].get());
+        dense(g, m, g->Hr[l - 1].get(), WT(g, l, 0), g->Zr[l].get(), WT(g, l, 1), P + kOff.br[l], g->Hr[l].get());
+        spmm<false>(g, n, g->cp.get(), g->ri.get(), g->ac.get(), g->Hr[l].get(), g->Zc[l].get());
+        dense(g, n, g->Hc[l - 1].get(), WT(g, l, 2), g->Zc[l].get(), WT(g, l, 3), P + kOff.bc[l], g->Hc[l].get());
     }
     if (n > 0)
         hipLaunchKernelGGL(k_head, dim3(cdiv(n, 256)), dim3(256), 0, g->s, n, g->Hc[2].get(), P + kOff.wx, P + kOff.bx,
@@ -667,17 +687,17 @@ float backward(pm_gnn* g) {
         wgrad(g, n, g->Hc[l - 1].get(), dHc, kOff.Wcc[l]);
         wgrad(g, n, g->Zc[l].get(), dHc, kOff.Wcz[l]);
         colsum(g, n, dHc, nullptr, kOff.bc[l], 0);
-        dense(g, n, dHc, WN(g, l, 2), nullptr, nullptr, nullptr, dHc2, false, false);  // dHc(l-1)
-        dense(g, n, dHc, WN(g, l, 3), nullptr, nullptr, nullptr, dZ, false, false);    // dZc
-        spmm(g, m, g->rp.get(), g->ci.get(), g->ac_t.get(), dZ, dHr, true);            // dHr += Ac^T dZc
+        dense_grad(g, n, dHc, WN(g, l, 2), dHc2);                                       // dHc(l-1)
+        dense_grad(g, n, dHc, WN(g, l, 3), dZ);                                         // dZc
+        spmm<true>(g, m, g->rp.get(), g->ci.get(), g->ac_t.get(), dZ, dHr);             // dHr += Ac^T dZc
         // row update of layer l: dHr holds dL/dHr(l)
         relu_bwd(g, m, g->Hr[l].get(), dHr);
         wgrad(g, m, g->Hr[l - 1].get(), dHr, kOff.Wrr[l]);
         wgrad(g, m, g->Zr[l].get(), dHr, kOff.Wrz[l]);
         colsum(g, m, dHr, nullptr, kOff.br[l], 0);
-        dense(g, m, dHr, WN(g, l, 1), nullptr, nullptr, nullptr, dZ, false, false);    // dZr
-        dense(g, m, dHr, WN(g, l, 0), nullptr, nullptr, nullptr, g->dT.get(), false, false);  // dHr(l-1)
-        spmm(g, n, g->cp.get(), g->ri.get(), g->ar_t.get(), dZ, dHc2, true);            // dHc(l-1) += Ar^T dZr
+        dense_grad(g, m, dHr, WN(g, l, 1), dZ);                                         // dZr
+        dense_grad(g, m, dHr, WN(g, l, 0), g->dT.get());                                // dHr(l-1)
+        spmm<true>(g, n, g->cp.get(), g->ri.get(), g->ar_t.get(), dZ, dHc2);            // dHc(l-1) += Ar^T dZr
         std::swap(g->dHc, g->dHc2);
         std::swap(g->dHr, g->dT);
         dHc = g->dHc.get();

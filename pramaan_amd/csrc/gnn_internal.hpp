@@ -17,8 +17,11 @@ struct pm_gnn {
     pm::DevBuf<short> wt, wn;  // 8 dense weights x 4096 bf16 each, two layouts
     pm::DevBuf<int> rp, ci, cp, ri;
     pm::DevBuf<float> ar, ac, ar_t, ac_t, xr, xc;
-    // activations: Hc0..2, Hr0..2, Zr1..2, Zc1..2, and gradients
-    pm::DevBuf<float> Hc[3], Hr[3], Zr[3], Zc[3], dHc, dHr, dHc2, dZ, dT, xhat, yhat, dx, dy, tx, ty, part;
+    // activations Hc0..2, Hr0..2, Zr1..2, Zc1..2 stored as bfloat16 bits
+    // (every consumer reads them in bfloat16: the MFMA layers, the message
+    // passing, the heads); gradients and everything else in FP32
+    pm::DevBuf<short> Hc[3], Hr[3], Zr[3], Zc[3];
+    pm::DevBuf<float> dHc, dHr, dHc2, dZ, dT, xhat, yhat, dx, dy, tx, ty, part;
     // primal-weight head: the graph's LPs own the columns [seg[k], seg[k+1]);
     // pool[k][64] = mean of their final column embeddings, what[k] the
     // prediction, tw[k] its training target (used when has_tw), dw[k] the
