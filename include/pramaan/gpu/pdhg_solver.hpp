@@ -30,6 +30,7 @@ struct PdhgOptions {
     bool relative_kkt = false;       // terminate on the FP64 relative KKT error
     int spmv_variant = 0;            // PM_SPMV_* (0 = autotuned)
     int max_refinements = 8;
+    bool feasibility_polishing = true;  // PDLP: polish primal and dual feasibility once the gap is small
     int device = 0;
     std::vector<double> x0, y0;      // optional warm start (y: reduced cost c + A^T y)
 };

@@ -151,6 +151,11 @@ typedef struct pm_pdhg_options {
                                   (PM_ALG_PDLP only: the reference
                                   algorithm rejects a warm start)          */
     const double* y0;          /* optional warm start [m], reference sign    */
+    int32_t feasibility_polishing; /* PDLP: 1 (default) = once the gap is small,
+                                  solve the primal and the dual feasibility
+                                  problems from the candidate and accept
+                                  the pair when it meets the tolerance;
+                                  0 = off                                  */
 } pm_pdhg_options;
 
 /* ---- result: SolveResult (simplex.hpp:46-76) plus PDHG diagnostics ------ */

@@ -44,7 +44,8 @@ class pm_pdhg_options(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int32), ("check_frequency", C.c_int32),
                 ("algorithm", C.c_int32), ("termination", C.c_int32), ("spmv_variant", C.c_int32),
                 ("ruiz_iterations", C.c_int32), ("max_refinements", C.c_int32), ("device", C.c_int32),
-                ("verbose", C.c_int32), ("primal_weight", C.c_double), ("x0", _d), ("y0", _d)]
+                ("verbose", C.c_int32), ("primal_weight", C.c_double), ("x0", _d), ("y0", _d),
+                ("feasibility_polishing", C.c_int32)]
 
 
 class pm_pdhg_result(C.Structure):

@@ -196,6 +196,10 @@ struct BatchArrays {
     float *c32, *lr32, *ur32, *lx32, *ux32, *g32;
     float *x, *xe, *x0, *xt, *xl, *y, *y0, *yt, *yl;
     double *xtot, *ytot, *xbase, *ybase;
+    double *xsave, *ysave, *xpol;  // feasibility polishing: saved candidate, polished primal
+    float* hold32;                 // the main solve's state during a polish (pm_batch::hold32)
+    double* hold64;
+    int64_t cols_total, rows_total;
     BatchDesc* desc;
     const int* order;
     const int* grp;     // global row pointer of A   (nz offsets of the concatenation)
@@ -211,7 +215,7 @@ struct BatchArrays {
     const int *sa_ptr, *sa_len, *sa_pos, *st_ptr, *st_len, *st_pos;
     const int2 *sa_win, *st_win;
     double tol;
-    int max_iterations, check_frequency, max_refinements;
+    int max_iterations, check_frequency, max_refinements, polish;
 };
 
 // ---- block helpers -------------------------------------------------------
@@ -320,7 +324,9 @@ struct View {
     const VT *oa, *oat;
     const double *cmin, *lr, *ur, *lx, *ux, *dr, *dc, *wx, *wy;
     float *c32, *lr32, *ur32, *lx32, *ux32, *g32, *x, *xe, *x0, *xt, *xl, *y, *y0, *yt, *yl;
-    double *xtot, *ytot, *xbase, *ybase;
+    double *xtot, *ytot, *xbase, *ybase, *xsave, *ysave, *xpol;
+    float *hx[6], *hy[6];  // held main state: x, x0, xl, c32, lx32, ux32 / y, y0, yl, lr32, ur32, g32
+    double *hxb, *hyb;     // held x_base, y_base
 };
 
 template <class VT>
@@ -343,16 +349,24 @@ __device__ __forceinline__ View<VT> view_of(const BatchArrays<VT>& A, const Batc
     v.lr = A.lr + ro; v.ur = A.ur + ro; v.dr = A.dr + ro; v.wy = A.wy + ro;
     v.c32 = A.c32 + co; v.lx32 = A.lx32 + co; v.ux32 = A.ux32 + co;
     v.x = A.x + co; v.xe = A.xe + co; v.x0 = A.x0 + co; v.xt = A.xt + co; v.xl = A.xl + co;
-    v.xtot = A.xtot + co; v.xbase = A.xbase + co;
+    v.xtot = A.xtot + co; v.xbase = A.xbase + co; v.xsave = A.xsave + co; v.xpol = A.xpol + co;
     v.lr32 = A.lr32 + ro; v.ur32 = A.ur32 + ro; v.g32 = A.g32 + ro;
     v.y = A.y + ro; v.y0 = A.y0 + ro; v.yt = A.yt + ro; v.yl = A.yl + ro;
-    v.ytot = A.ytot + ro; v.ybase = A.ybase + ro;
+    v.ytot = A.ytot + ro; v.ybase = A.ybase + ro; v.ysave = A.ysave + ro;
+    for (int q = 0; q < 6; ++q) {
+        v.hx[q] = A.hold32 + q * A.cols_total + co;
+        v.hy[q] = A.hold32 + 6 * A.cols_total + q * A.rows_total + ro;
+    }
+    v.hxb = A.hold64 + co;
+    v.hyb = A.hold64 + A.cols_total + ro;
     return v;
 }
 
 // Solver state of one LP between the phase kernels.
 struct BatchState {
     Control ctl;
+    Control ctl_held;  // the main solve's control state while a polish runs
+    Polish pol;
     double eta;
     int32_t status;
     int32_t active;       // 1 while the LP is being solved
@@ -367,7 +381,7 @@ __device__ __forceinline__ void publish_iter(LpIter* it, int lp, const BatchStat
     const double om = st.ctl.omega;
     r.tau = static_cast<float>(st.eta / om);
     r.sig = static_cast<float>(st.eta * om);
-    r.k0 = st.ctl.k;
+    r.k0 = st.pol.phase != Polish::MAIN ? st.pol.k : st.ctl.k;
     r.chunk = st.active ? st.ctl.chunk(K) : 0;
     r.use_g = st.ctl.use_g ? 1 : 0;
     r.row_off = static_cast<int32_t>(d.row_off);
@@ -495,6 +509,7 @@ __global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S
         d.warm_accepted = (d.warm_x || d.warm_y) ? (warm ? 1 : 0) : -1;
         d.start_kkt = (d.warm_x || d.warm_y) ? (warm ? rel_s[0] : rel_s[1]) : -1.0;
         st.ctl.init(A.tol, A.max_iterations, A.max_refinements, om, warm && d.warm_y != 0 && m > 0);
+        st.pol.init(A.polish != 0);
         st.eta = d.eta;
         st.status = PM_SOLVE_ITERATION_LIMIT;
         st.active = 1;
@@ -930,8 +945,10 @@ __global__ __launch_bounds__(kBlock) void k_b_items(BatchArrays<VT> A, const Bat
         if constexpr (PASS == kPassRound) {
             if (!S[lp].round_start) continue;  // uniform over the workgroup
             const double dp = S[lp].ctl.DP, dd = S[lp].ctl.DD;
+            const int ph = S[lp].pol.phase;
+            const int mode = ph == Polish::PRIMAL ? kRoundPrimalFeas : (ph == Polish::DUAL ? kRoundDualFeas : kRoundLp);
             if (rows) {
-                const EpiRoundRow epi{v.lr, v.ur, v.ybase, v.dr, dp, dd, v.lr32, v.ur32, v.g32};
+                const EpiRoundRow epi{v.lr, v.ur, v.ybase, v.dr, dp, dd, v.lr32, v.ur32, v.g32, mode};
                 item_rows_any(v.la, cnt, [&](int r, int lane, auto LC) {
                     const int i = first + r;
                     const double ax =
@@ -942,7 +959,7 @@ __global__ __launch_bounds__(kBlock) void k_b_items(BatchArrays<VT> A, const Bat
                     }
                 });
             } else {
-                const EpiRoundCol epi{v.cmin, v.lx, v.ux, v.xbase, v.dc, dp, dd, v.c32, v.lx32, v.ux32};
+                const EpiRoundCol epi{v.cmin, v.lx, v.ux, v.xbase, v.dc, dp, dd, v.c32, v.lx32, v.ux32, mode};
                 item_rows_any(v.lat, cnt, [&](int r, int lane, auto LC) {
                     const int j = first + r;
                     const double aty =
@@ -1011,8 +1028,23 @@ __global__ __launch_bounds__(kBlock) void k_b_items(BatchArrays<VT> A, const Bat
     }
 }
 
+// What a check does to the arrays of an LP besides the Control actions:
+// the steps of a feasibility polish (Polish in control.hpp).
+enum PolishEvent {
+    kEvNone = 0,
+    kEvStartPrimal,   // save the candidate; primal phase from x_cand
+    kEvStartDual,     // save the candidate (x_cand is already polished); dual phase from y_cand
+    kEvPrimalToDual,  // keep the polished x; dual phase from the saved y
+    kEvCombine,       // polished x with the saved y (primal phase done, dual not needed)
+    kEvCombineDual,   // polished x with the polished y (dual phase done)
+    kEvResume,        // back to the main solve, restored as it was when the polish began
+    kEvPhaseRestart,  // restart inside a phase
+    kEvLimit          // budget spent inside a phase: the saved candidate is the result
+};
+
 // Phase 3, per LP: the KKT error from its blocks' partial sums -> Control
-// decision -> restart / refine / finish.  Counts the LPs still active.
+// decision -> restart / refine / polish / finish.  Counts the LPs still
+// active.
 template <class VT>
 __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* S, int* active_count) {
     const int lp = A.order[blockIdx.x];
@@ -1022,13 +1054,17 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
     const View<VT> v = view_of(A, d);
     const int m = v.m, n = v.n, tid = threadIdx.x;
     __shared__ double red[16], keep[8];
-    __shared__ int act_s;
+    __shared__ int act_s, ev_s;
     __shared__ Control ctl;
-    if (tid == 0) ctl = st.ctl;
+    __shared__ Polish pol;
+    if (tid == 0) {
+        ctl = st.ctl;
+        pol = st.pol;
+    }
     __syncthreads();
     const int chunk = A.it[lp].chunk;
     if (tid == 0) st.round_start = 0;  // the round pass of this chunk has run
-    int act;
+    int act, ev = kEvNone;
     if (chunk > 0) {
         // The KKT terms of x_tot, y_tot, reduced per block by the check
         // pass, summed over the LP's blocks in order.
@@ -1050,17 +1086,131 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
             red[j] = a;
         }
         __syncthreads();
-        double rel = 0.0;
-        if (tid == 0) rel = kkt_finish(keep, red, d);
         if (tid == 0) {
+            const double rel = kkt_finish(keep, red, d);
             ctl.advance(chunk);
-            act_s = ctl.on_check(rel, d.rel_p, d.rel_d, d.rel_g);
-            st.have_cand = 1;
+            int a = Control::CONTINUE, e = kEvNone;
+            if (pol.phase == Polish::MAIN) {
+                a = ctl.on_check(rel, d.rel_p, d.rel_d, d.rel_g);
+                st.have_cand = 1;
+                if (a == Control::CONTINUE && pol.trigger(A.tol, ctl.total_it, rel, d.rel_g)) {
+                    st.ctl_held = ctl;
+                    pol.save(ctl.total_it, rel, d.rel_p, d.rel_d);
+                    const bool primal = pol.needs(A.tol, d.rel_p);
+                    pol.start(primal ? Polish::PRIMAL : Polish::DUAL, ctl.total_it);
+                    ctl.DP = ctl.DD = Polish::scale(primal ? d.rel_p : d.rel_d, ctl.DP);
+                    if (!primal) ctl.use_g = true;
+                    e = primal ? kEvStartPrimal : kEvStartDual;
+                }
+            } else {
+                const bool dual_phase = pol.phase == Polish::DUAL;
+                const int pa = pol.on_check(A.tol, dual_phase ? d.rel_d : d.rel_p, chunk);
+                const bool spent = ctl.total_it >= ctl.max_iterations;
+                if (pa == Polish::DONE && (dual_phase || !pol.needs(A.tol, pol.save_d))) {
+                    e = dual_phase ? kEvCombineDual : kEvCombine;
+                } else if (spent) {
+                    e = kEvLimit;
+                } else if (pa == Polish::DONE) {
+                    pol.start(Polish::DUAL, ctl.total_it);
+                    ctl.DP = ctl.DD = Polish::scale(pol.save_d, ctl.DP);
+                    ctl.use_g = true;
+                    e = kEvPrimalToDual;
+                } else if (pa == Polish::FAILED) {
+                    e = kEvResume;
+                } else if (pa == Polish::RESTART) {
+                    e = kEvPhaseRestart;
+                }
+            }
+            act_s = a;
+            ev_s = e;
         }
         __syncthreads();
         act = act_s;
+        ev = ev_s;
     } else {
         act = Control::LIMIT;
+        if (pol.phase != Polish::MAIN) ev = kEvLimit;
+    }
+    // Polish steps.
+    if (ev == kEvStartPrimal || ev == kEvStartDual) {
+        // hold the main solve's state (iterates, round data, round base)
+        float* const mx[6] = {v.x, v.x0, v.xl, v.c32, v.lx32, v.ux32};
+        float* const my[6] = {v.y, v.y0, v.yl, v.lr32, v.ur32, v.g32};
+        for (int j = tid; j < n; j += kBB) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v.hx[q][j] = mx[q][j];
+            v.hxb[j] = v.xbase[j];
+        }
+        for (int i = tid; i < m; i += kBB) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v.hy[q][i] = my[q][i];
+            v.hyb[i] = v.ybase[i];
+        }
+        __syncthreads();
+        for (int j = tid; j < n; j += kBB) {
+            const double x = v.xtot[j];
+            v.xsave[j] = x;
+            v.xbase[j] = ev == kEvStartPrimal ? x : 0.0;
+            if (ev == kEvStartDual) v.xpol[j] = x;
+        }
+        for (int i = tid; i < m; i += kBB) {
+            const double y = v.ytot[i];
+            v.ysave[i] = y;
+            v.ybase[i] = ev == kEvStartPrimal ? 0.0 : y;
+        }
+        if (tid == 0) st.round_start = 1;
+    } else if (ev == kEvPrimalToDual) {
+        for (int j = tid; j < n; j += kBB) {
+            v.xpol[j] = v.xtot[j];
+            v.xbase[j] = 0.0;
+        }
+        for (int i = tid; i < m; i += kBB) v.ybase[i] = v.ysave[i];
+        if (tid == 0) st.round_start = 1;
+    } else if (ev == kEvCombine || ev == kEvCombineDual) {
+        // (x_tot is the polished x after a primal phase; y_tot the polished y after a dual one)
+        if (ev == kEvCombine)
+            for (int i = tid; i < m; i += kBB) v.ytot[i] = v.ysave[i];
+        else
+            for (int j = tid; j < n; j += kBB) v.xtot[j] = v.xpol[j];
+        __syncthreads();
+        const double rel = batch_kkt(v, d, red, keep);
+        if (tid == 0) act_s = Polish::accept(A.tol, rel, d.rel_g) ? Control::OPTIMAL : Control::CONTINUE;
+        __syncthreads();
+        act = act_s;
+        if (act != Control::OPTIMAL) ev = ctl.total_it >= ctl.max_iterations ? kEvLimit : kEvResume;
+    } else if (ev == kEvPhaseRestart) {
+        for (int j = tid; j < n; j += kBB) v.x[j] = v.x0[j] = v.xl[j] = v.xt[j];
+        for (int i = tid; i < m; i += kBB) v.y[i] = v.y0[i] = v.yl[i] = v.yt[i];
+    }
+    if (ev == kEvResume || ev == kEvLimit) {
+        for (int j = tid; j < n; j += kBB) v.xtot[j] = v.xsave[j];
+        for (int i = tid; i < m; i += kBB) v.ytot[i] = v.ysave[i];
+        if (ev == kEvResume) {
+            // the main solve continues where it was: the polish cost only its iterations
+            float* const mx[6] = {v.x, v.x0, v.xl, v.c32, v.lx32, v.ux32};
+            float* const my[6] = {v.y, v.y0, v.yl, v.lr32, v.ur32, v.g32};
+            for (int j = tid; j < n; j += kBB) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) mx[q][j] = v.hx[q][j];
+                v.xbase[j] = v.hxb[j];
+            }
+            for (int i = tid; i < m; i += kBB) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) my[q][i] = v.hy[q][i];
+                v.ybase[i] = v.hyb[i];
+            }
+        }
+        __syncthreads();
+        if (ev == kEvLimit) {
+            (void)batch_kkt(v, d, red, keep);  // the result fields describe the saved candidate
+            act = Control::LIMIT;
+        } else if (tid == 0) {
+            const int spent = ctl.total_it;
+            ctl = st.ctl_held;
+            ctl.total_it = spent;
+            st.have_cand = 1;
+        }
+        if (tid == 0) pol.phase = Polish::MAIN;
     }
     bool done = act == Control::OPTIMAL || act == Control::FAIL || act == Control::LIMIT;
     if (act == Control::REFINE) {
@@ -1108,6 +1258,7 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
     }
     if (tid == 0) {
         st.ctl = ctl;
+        st.pol = pol;
         publish_iter(A.it, lp, st, d, A.check_frequency);
         if (!done) atomicAdd(active_count, 1);
     }
@@ -1153,6 +1304,7 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->max_iterations = opts.max_iterations;
     b->check_frequency = opts.check_frequency;
     b->max_refinements = opts.max_refinements;
+    b->polish = opts.feasibility_polishing ? 1 : 0;
     b->count = count;
     b->desc.assign(count, BatchDesc{});
     b->sense.resize(count);
@@ -1413,8 +1565,10 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->chunks_per_graph = std::max(1, std::min(4, 512 / std::max(1, b->check_frequency)));
     for (auto* p : {&b->c32, &b->lx32, &b->ux32, &b->x, &b->xe, &b->x0, &b->xt, &b->xl}) p->resize(Cn);
     for (auto* p : {&b->lr32, &b->ur32, &b->g32, &b->y, &b->y0, &b->yt, &b->yl}) p->resize(R);
-    for (auto* p : {&b->xtot, &b->xbase, &b->wx}) p->resize(Cn);
-    for (auto* p : {&b->ytot, &b->ybase, &b->wy}) p->resize(R);
+    for (auto* p : {&b->xtot, &b->xbase, &b->wx, &b->xsave, &b->xpol}) p->resize(Cn);
+    for (auto* p : {&b->ytot, &b->ybase, &b->wy, &b->ysave}) p->resize(R);
+    b->hold32.resize(6 * (Cn + R));
+    b->hold64.resize(Cn + R);
     b->d_desc.upload(b->desc, s);
     b->state.resize(sizeof(BatchState) * static_cast<size_t>(count));
     b->counter.resize(1);
@@ -1444,6 +1598,9 @@ BatchArrays<VT> batch_arrays(pm_batch* b, const VT* oa, const VT* oat) {
     A.x = b->x.get(); A.xe = b->xe.get(); A.x0 = b->x0.get(); A.xt = b->xt.get(); A.xl = b->xl.get();
     A.y = b->y.get(); A.y0 = b->y0.get(); A.yt = b->yt.get(); A.yl = b->yl.get();
     A.xtot = b->xtot.get(); A.ytot = b->ytot.get(); A.xbase = b->xbase.get(); A.ybase = b->ybase.get();
+    A.xsave = b->xsave.get(); A.ysave = b->ysave.get(); A.xpol = b->xpol.get();
+    A.hold32 = b->hold32.get(); A.hold64 = b->hold64.get();
+    A.cols_total = std::max<int64_t>(b->cols_total, 1); A.rows_total = std::max<int64_t>(b->rows_total, 1);
     A.desc = b->d_desc.get();
     A.order = b->d_order.get();
     A.grp = b->grp.get();
@@ -1460,6 +1617,7 @@ BatchArrays<VT> batch_arrays(pm_batch* b, const VT* oa, const VT* oat) {
     A.max_iterations = b->max_iterations;
     A.check_frequency = b->check_frequency;
     A.max_refinements = b->max_refinements;
+    A.polish = b->polish;
     return A;
 }
 

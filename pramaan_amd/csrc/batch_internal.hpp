@@ -117,7 +117,13 @@ struct pm_batch {
     pm::DevBuf<double> oa64, oat64;
     pm::DevBuf<double> cmin, lr, ur, lx, ux, dr, dc, wx, wy;
     pm::DevBuf<float> c32, lr32, ur32, lx32, ux32, g32, x, xe, x0, xt, xl, y, y0, yt, yl;
-    pm::DevBuf<double> xtot, ytot, xbase, ybase;
+    pm::DevBuf<double> xtot, ytot, xbase, ybase, xsave, ysave, xpol;
+    // The main solve's state while a feasibility polish runs: FP32 blocks
+    // [x | x0 | xl | c32 | lx32 | ux32] of cols_total each, then
+    // [y | y0 | yl | lr32 | ur32 | g32] of rows_total each; FP64 x_base, y_base.
+    pm::DevBuf<float> hold32;
+    pm::DevBuf<double> hold64;
+    int polish = 1;  // feasibility polishing (pm_pdhg_options.feasibility_polishing)
     pm::DevBuf<pm::BatchDesc> d_desc;
     pm::DevBuf<int> grp, gcp, wcnt;
     pm::DevBuf<pm::WorkItem> work_r, work_c;

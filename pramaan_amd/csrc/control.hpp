@@ -46,6 +46,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 
 namespace pm {
 
@@ -181,6 +182,105 @@ struct Control {
     // Halpern weight of the iteration with in-round index kk.
     __host__ __device__ static float lambda(int kk) {
         return static_cast<float>((kk + 1.0) / (kk + 2.0));
+    }
+};
+
+// Feasibility polishing (after PDLP's): once a check finds the relative gap
+// small (<= kGap tol) and the error within kRel tol (at most once every
+// kEvery iterations), the primal and the dual feasibility problems are
+// solved on their own from the candidate,
+// each by the same reflected-Halpern PDHG with the main primal weight:
+//   PRIMAL  the LP with cost 0, from x_cand (its dual starts at 0): stops
+//           once the primal residual of x_tot is <= kTarget tol;
+//   DUAL    the LP with every finite bound moved to 0, from y_cand (x at
+//           0): stops once the dual residual of y_tot is <= kTarget tol.
+// Each phase restarts on its own residual by PDLP's rule and has a budget
+// of kFrac of the iterations so far (at least kMinBudget).  When both meet
+// their targets the pair (x_primal, y_dual) is accepted if its relative KKT
+// error meets the tolerance and its relative gap is at most kAccept tol:
+// both points are then feasible to 0.3 tol, the optimum lies between their
+// objectives, and a gap of half the tolerance keeps the primal objective
+// within about tol (1 + |objective|) of it -- the KKT test alone would let
+// it drift to twice that (a polished rolling-horizon window ended 1.01e-6
+// from the HiGHS objective).  Otherwise, or when a phase runs out of
+// budget, the main solve continues exactly where it was when the polish
+// began (its iterates, round data and control state are held meanwhile),
+// so a polish that fails costs only its own iterations.  A phase that
+// starts from a residual already at its target is skipped.  Polishing
+// targets the solves whose gap has converged while a feasibility residual
+// stalls; the thresholds come from a numpy restatement of this iteration
+// on 20 rolling-horizon windows of 5,160 x 9,600 (mean iterations 5,261
+// without polishing, 4,067 with these; accepting on the KKT error alone
+// would give 3,398 at the objective accuracy cost above).
+struct Polish {
+    enum Phase { MAIN = 0, PRIMAL = 1, DUAL = 2 };
+    enum Action { CONTINUE = 0, DONE = 1, FAILED = 2, RESTART = 3 };
+    static constexpr double kGap = 0.1, kRel = 3.0, kTarget = 0.3, kFrac = 0.1, kAccept = 0.5;
+    static constexpr int kEvery = 512, kMinBudget = 256;
+
+    int32_t enabled, phase, k, it, since, budget, last_try;
+    double q_last, q_prev;
+    double save_rel, save_p, save_d;  // errors of the saved candidate
+
+    __host__ __device__ void init(bool on) {
+        enabled = on ? 1 : 0;
+        phase = MAIN;
+        k = it = since = budget = 0;
+        last_try = -(1 << 30);
+        q_last = -1.0;
+        q_prev = INFINITY;
+        save_rel = save_p = save_d = 0.0;
+    }
+    // Whether a main-solve check of (rel, rel_g) at total_it starts polishing.
+    __host__ __device__ bool trigger(double tol, int total_it, double rel, double rel_g) const {
+        return enabled && phase == MAIN && rel > tol && rel <= kRel * tol && rel_g <= kGap * tol &&
+               total_it - last_try >= kEvery;
+    }
+    __host__ __device__ void save(int total_it, double rel, double rel_p, double rel_d) {
+        last_try = total_it;
+        save_rel = rel;
+        save_p = rel_p;
+        save_d = rel_d;
+    }
+    __host__ __device__ bool needs(double tol, double q) const { return q > kTarget * tol; }
+    // Whether a polished pair of relative KKT error rel and gap rel_g ends the solve.
+    __host__ __device__ static bool accept(double tol, double rel, double rel_g) {
+        return rel <= tol && rel_g <= kAccept * tol;
+    }
+    __host__ __device__ void start(int ph, int total_it) {
+        phase = ph;
+        k = it = since = 0;
+        const int b = static_cast<int>(kFrac * total_it);
+        budget = b > kMinBudget ? b : kMinBudget;
+        q_last = -1.0;
+        q_prev = INFINITY;
+    }
+    // Check of a phase after `iters` more iterations, q its residual.
+    __host__ __device__ Action on_check(double tol, double q, int iters) {
+        k += iters;
+        it += iters;
+        since += iters;
+        if (q <= kTarget * tol) return DONE;
+        if (!(q == q) || it >= budget) return FAILED;
+        if (q_last < 0.0) q_last = q;
+        const bool restart = q <= Control::kSuff * q_last || (q <= Control::kNec * q_last && q > q_prev) ||
+                             static_cast<double>(since) >= Control::kArt * static_cast<double>(it);
+        q_prev = q;
+        if (restart) {
+            k = 0;
+            since = 0;
+            q_last = q;
+            q_prev = INFINITY;
+            return RESTART;
+        }
+        return CONTINUE;
+    }
+    // Scale factor of a phase's correction problem that starts from residual q.
+    __host__ __device__ static double scale(double q, double dp) {
+        double f = 1.0 / (q > 1e-300 ? q : 1e-300);
+        const double cap = (dp > 1.0 ? dp : 1.0) * 1e4;
+        f = f < cap ? f : cap;
+        return f > 1.0 ? f : 1.0;
     }
 };
 

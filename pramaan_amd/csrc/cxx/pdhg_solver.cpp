@@ -92,6 +92,7 @@ SolveResult PdhgSolver::solve(const ModelIR& model) {
     if (options_.relative_kkt) o.termination = PM_TERM_RELATIVE_KKT;
     o.spmv_variant = options_.spmv_variant;
     o.max_refinements = options_.max_refinements;
+    o.feasibility_polishing = options_.feasibility_polishing ? 1 : 0;
     o.device = options_.device;
     if (static_cast<int>(options_.x0.size()) == n && n > 0) o.x0 = options_.x0.data();
     if (static_cast<int>(options_.y0.size()) == m && m > 0) o.y0 = options_.y0.data();

@@ -205,6 +205,12 @@ struct pm_ctx {
     // FP64 vectors.
     DevBuf<double> xtot, ytot, xbase, ybase, act64;
     DevBuf<double> xbest, ybest;  // PDLP: the checked candidate of lowest relative KKT error
+    // PDLP feasibility polishing: the candidate it starts from, the polished
+    // primal, and the main solve's state held meanwhile (FP32 blocks
+    // x, x0, xl, c32, lx32, ux32 of n, then y, y0, yl, lr32, ur32, g32 of m;
+    // FP64 x_base, y_base).
+    DevBuf<double> xsave, ysave, xpol, hold64;
+    DevBuf<float> hold32;
     DevBuf<double> partials, red;
     PinnedBuf<double> hred;
 
@@ -595,6 +601,13 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
     for (auto* b : {&c->y, &c->y0, &c->yt, &c->yl, &c->tmp_m}) b->resize(std::max(m, 1));
     for (auto* b : {&c->xtot, &c->xbase, &c->xbest}) b->resize(n);
     for (auto* b : {&c->ytot, &c->ybase, &c->act64, &c->ybest}) b->resize(std::max(m, 1));
+    if (opts.algorithm == PM_ALG_PDLP && opts.feasibility_polishing) {
+        c->xsave.resize(n);
+        c->xpol.resize(n);
+        c->ysave.resize(std::max(m, 1));
+        c->hold32.resize(6 * (static_cast<size_t>(n) + std::max(m, 1)));
+        c->hold64.resize(static_cast<size_t>(n) + std::max(m, 1));
+    }
 
     if (opts.algorithm == PM_ALG_REFERENCE) {
         // Canonicalisation and diagonal steps as pdhg_solver.cu:196-250.
@@ -1009,18 +1022,18 @@ void finish_result(pm_ctx* c, pm_pdhg_result* res, const double* xd_dev, const d
 // ---------------------------------------------------------------------------
 // PDLP (reflected Halpern) solve
 // ---------------------------------------------------------------------------
-void round_data(pm_ctx* c, double dp, double dd) {
+void round_data(pm_ctx* c, double dp, double dd, int mode = kRoundLp) {
     hipStream_t s = c->stream;
     const int m = c->m, n = c->n;
     EpiRoundCol ec{c->d_cmin.get(), c->d_lx.get(), c->d_ux.get(), c->xbase.get(), c->d_dc.get(), dp, dd,
-                   c->c32.get(), c->lx32.get(), c->ux32.get()};
+                   c->c32.get(), c->lx32.get(), c->ux32.get(), mode};
     if (c->exact_f32)
         launch_rows<double>(c->var_at, n, c->cp.get(), c->ri.get(), c->oat32.get(), c->ybase.get(), ec, s);
     else
         launch_rows<double>(c->var_at, n, c->cp.get(), c->ri.get(), c->oat64.get(), c->ybase.get(), ec, s);
     if (m > 0) {
         EpiRoundRow er{c->d_lr.get(), c->d_ur.get(), c->ybase.get(), c->d_dr.get(), dp, dd,
-                       c->lr32.get(), c->ur32.get(), c->g32.get()};
+                       c->lr32.get(), c->ur32.get(), c->g32.get(), mode};
         if (c->exact_f32)
             launch_rows<double>(c->var_a, m, c->rp.get(), c->ci.get(), c->oa32.get(), c->xbase.get(), er, s);
         else
@@ -1047,6 +1060,134 @@ void hal_iteration(pm_ctx* c, int k, double omega, bool keep_t, bool use_g) {
                       c->y.get(), keep_t ? c->yt.get() : nullptr};
         launch_dual_half(c, ed, s);
     }
+}
+
+// Candidate T(z) of the last iteration of a round with base (x_base, y_base)
+// and scale factors (dp, dd), as a total point in FP64 (xtot, ytot).
+void total_point(pm_ctx* c, double dp, double dd) {
+    hipStream_t s = c->stream;
+    launch_elem(c->n, FnTotalX{c->xbase.get(), c->d_dc.get(), c->xt.get(), 1.0 / dp, c->d_lx.get(), c->d_ux.get(),
+                               c->xtot.get()},
+                s);
+    if (c->m > 0)
+        launch_elem(c->m, FnTotalY{c->ybase.get(), c->d_dr.get(), c->yt.get(), 1.0 / dd, c->d_lr.get(),
+                                   c->d_ur.get(), c->ytot.get()},
+                    s);
+}
+
+// z <- T(z), anchor and last restart point too (restarts inside a round).
+void restart_at_candidate(pm_ctx* c) {
+    hipStream_t s = c->stream;
+    for (float* d : {c->x.get(), c->x0.get(), c->xl.get()}) launch_elem(c->n, FnCopy<float>{d, c->xt.get()}, s);
+    for (float* d : {c->y.get(), c->y0.get(), c->yl.get()}) launch_elem(c->m, FnCopy<float>{d, c->yt.get()}, s);
+}
+
+// The main solve's iterates, round data and round base into hold32 /
+// hold64 (to_hold) or back.
+void hold_state(pm_ctx* c, bool to_hold) {
+    hipStream_t s = c->stream;
+    const size_t n = c->n, m = std::max(c->m, 1);
+    float* const xs[6] = {c->x.get(), c->x0.get(), c->xl.get(), c->c32.get(), c->lx32.get(), c->ux32.get()};
+    float* const ys[6] = {c->y.get(), c->y0.get(), c->yl.get(), c->lr32.get(), c->ur32.get(), c->g32.get()};
+    for (int q = 0; q < 6; ++q) {
+        float* hx = c->hold32.get() + q * n;
+        float* hy = c->hold32.get() + 6 * n + q * m;
+        if (to_hold) {
+            launch_elem(c->n, FnCopy<float>{hx, xs[q]}, s);
+            launch_elem(c->m, FnCopy<float>{hy, ys[q]}, s);
+        } else {
+            launch_elem(c->n, FnCopy<float>{xs[q], hx}, s);
+            launch_elem(c->m, FnCopy<float>{ys[q], hy}, s);
+        }
+    }
+    double* hxb = c->hold64.get();
+    double* hyb = c->hold64.get() + n;
+    if (to_hold) {
+        launch_elem(c->n, FnCopy<double>{hxb, c->xbase.get()}, s);
+        launch_elem(c->m, FnCopy<double>{hyb, c->ybase.get()}, s);
+    } else {
+        launch_elem(c->n, FnCopy<double>{c->xbase.get(), hxb}, s);
+        launch_elem(c->m, FnCopy<double>{c->ybase.get(), hyb}, s);
+    }
+}
+
+// One phase of a feasibility polish (Polish in control.hpp) on the round
+// data already built: reflected Halpern iterations with the main primal
+// weight until the phase's residual (primal: rel_p of x_tot, dual: rel_d
+// of y_tot) meets its target, restarts on it, budget.  Returns whether it
+// met the target; the phase's total point is left in xtot / ytot.
+bool polish_phase(pm_ctx* c, Control& ctl, Polish& pol, int phase, double scale, bool use_g, int K) {
+    pol.start(phase, ctl.total_it);
+    for (;;) {
+        const int chunk = ctl.chunk(K);
+        if (chunk <= 0) return false;
+        for (int i = 0; i < chunk; ++i) hal_iteration(c, pol.k + i, ctl.omega, i == chunk - 1, use_g);
+        ctl.total_it += chunk;
+        total_point(c, scale, scale);
+        const Kkt kk = eval_kkt(c, c->xtot.get(), c->ytot.get());
+        const int pa = pol.on_check(ctl.tol, phase == Polish::PRIMAL ? kk.rel_p : kk.rel_d, chunk);
+        if (pa == Polish::DONE) return true;
+        if (pa == Polish::FAILED) return false;
+        if (pa == Polish::RESTART) restart_at_candidate(c);
+    }
+}
+
+// Feasibility polish from the candidate (xtot, ytot) of errors kk: the
+// primal phase from x_cand, the dual phase from y_cand, then the pair.
+// Returns true when the pair is accepted (it is left in xtot / ytot and
+// *out holds its KKT); otherwise the main solve's state is restored as it
+// was and only the iterations spent remain counted.
+bool polish(pm_ctx* c, Control& ctl, Polish& pol, const Kkt& kk, int K, bool ref_test, Kkt* out) {
+    hipStream_t s = c->stream;
+    const int m = c->m, n = c->n;
+    const double tol = ctl.tol;
+    const Control held = ctl;
+    hold_state(c, true);
+    launch_elem(n, FnCopy<double>{c->xsave.get(), c->xtot.get()}, s);
+    if (m > 0) launch_elem(m, FnCopy<double>{c->ysave.get(), c->ytot.get()}, s);
+    pol.save(ctl.total_it, kk.rel, kk.rel_p, kk.rel_d);
+    bool ok = true;
+    if (pol.needs(tol, kk.rel_p)) {
+        const double f = Polish::scale(kk.rel_p, ctl.DP);
+        launch_elem(n, FnCopy<double>{c->xbase.get(), c->xsave.get()}, s);
+        launch_elem(std::max(m, 1), FnFill<double>{c->ybase.get(), 0.0}, s);
+        round_data(c, f, f, kRoundPrimalFeas);
+        ok = polish_phase(c, ctl, pol, Polish::PRIMAL, f, false, K);
+        if (ok) launch_elem(n, FnCopy<double>{c->xpol.get(), c->xtot.get()}, s);
+    } else {
+        launch_elem(n, FnCopy<double>{c->xpol.get(), c->xsave.get()}, s);
+    }
+    if (ok && pol.needs(tol, kk.rel_d)) {
+        const double f = Polish::scale(kk.rel_d, ctl.DP);
+        launch_elem(n, FnFill<double>{c->xbase.get(), 0.0}, s);
+        if (m > 0) launch_elem(m, FnCopy<double>{c->ybase.get(), c->ysave.get()}, s);
+        round_data(c, f, f, kRoundDualFeas);
+        ok = polish_phase(c, ctl, pol, Polish::DUAL, f, true, K);
+    } else if (ok && m > 0) {
+        launch_elem(m, FnCopy<double>{c->ytot.get(), c->ysave.get()}, s);
+    }
+    pol.phase = Polish::MAIN;
+    if (ok) {
+        launch_elem(n, FnCopy<double>{c->xtot.get(), c->xpol.get()}, s);
+        const Kkt pk = eval_kkt(c, c->xtot.get(), c->ytot.get());
+        bool accept = Polish::accept(tol, pk.rel, pk.rel_g);
+        if (accept && ref_test) {
+            double pv = 0.0, dv = 0.0;
+            ref_style_residuals(c, c->xtot.get(), c->ytot.get(), c->opts.tolerance, pv, dv);
+            accept = pv <= c->opts.tolerance && dv <= c->opts.tolerance;
+        }
+        if (accept) {
+            *out = pk;
+            return true;
+        }
+    }
+    const int spent = ctl.total_it;
+    ctl = held;
+    ctl.total_it = spent;
+    hold_state(c, false);
+    launch_elem(n, FnCopy<double>{c->xtot.get(), c->xsave.get()}, s);
+    if (m > 0) launch_elem(m, FnCopy<double>{c->ytot.get(), c->ysave.get()}, s);
+    return false;
 }
 
 void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
@@ -1092,6 +1233,8 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
 
     Control ctl;
     ctl.init(o.tolerance, o.max_iterations, o.max_refinements, c->omega0, warm && !c->y0_h.empty() && m > 0);
+    Polish pol;
+    pol.init(o.feasibility_polishing != 0);
     res->status = PM_SOLVE_ITERATION_LIMIT;
     // Under PM_TERM_REFERENCE a point is optimal only when the reference's
     // own measures agree (pv, dv <= tolerance, pdhg_solver.cu:356): each
@@ -1115,13 +1258,7 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
         for (int i = 0; i < chunk; ++i) hal_iteration(c, ctl.k + i, ctl.omega, i == chunk - 1, ctl.use_g);
         ctl.advance(chunk);
         // Candidate = T(z) of the last iteration, as a total point in FP64.
-        launch_elem(n, FnTotalX{c->xbase.get(), c->d_dc.get(), c->xt.get(), 1.0 / ctl.DP, c->d_lx.get(),
-                                c->d_ux.get(), c->xtot.get()},
-                    s);
-        if (m > 0)
-            launch_elem(m, FnTotalY{c->ybase.get(), c->d_dr.get(), c->yt.get(), 1.0 / ctl.DD, c->d_lr.get(),
-                                    c->d_ur.get(), c->ytot.get()},
-                        s);
+        total_point(c, ctl.DP, ctl.DD);
         have_candidate = true;
         const Kkt kk = eval_kkt(c, c->xtot.get(), c->ytot.get());
         if (kk.rel < best_rel) {
@@ -1142,6 +1279,17 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
             }
             ctl.tol = std::max(ctl.tol * 0.1, 1e-14);
             continue;
+        }
+        if (act == Control::CONTINUE && pol.trigger(ctl.tol, ctl.total_it, kk.rel, kk.rel_g)) {
+            Kkt pk;
+            if (polish(c, ctl, pol, kk, K, ref_test, &pk)) {
+                if (o.verbose)
+                    fprintf(stderr, "[pramaan] it %d polished kkt %.3e (p %.2e d %.2e g %.2e)\n", ctl.total_it, pk.rel,
+                            pk.rel_p, pk.rel_d, pk.rel_g);
+                res->status = PM_SOLVE_OPTIMAL;
+                break;
+            }
+            continue;  // the main solve goes on where it was
         }
         if (act == Control::FAIL) { res->status = PM_SOLVE_NUMERICAL_FAILURE; break; }
         if (act == Control::LIMIT) break;
@@ -1260,6 +1408,7 @@ void pm_pdhg_options_default(pm_pdhg_options* o) {
     o->primal_weight = 0.0;
     o->x0 = nullptr;
     o->y0 = nullptr;
+    o->feasibility_polishing = 1;
 }
 
 int pm_ctx_create(int32_t device, pm_ctx** out) {

@@ -472,8 +472,14 @@ __device__ __forceinline__ float to_f32_bound(double v) {
     return static_cast<float>(v);
 }
 
+// What a round's correction problem is: the LP itself (refinement rounds),
+// or one of the two feasibility problems of a polish (Polish in
+// control.hpp): the LP with cost 0, or with every finite bound at 0.
+enum RoundMode { kRoundLp = 0, kRoundPrimalFeas = 1, kRoundDualFeas = 2 };
+
 // Correction-LP column data of a refinement round from FP64 A^T y_base:
-// c_hat = D_c DD (c + A^T y_base), bounds DP (l - x_base) / D_c.
+// c_hat = D_c DD (c + A^T y_base), bounds DP (l - x_base) / D_c (cost 0 in
+// kRoundPrimalFeas, finite bounds 0 in kRoundDualFeas, whose x_base is 0).
 struct EpiRoundCol {
     const double* c;
     const double* lo;
@@ -484,13 +490,16 @@ struct EpiRoundCol {
     float* c32;
     float* lo32;
     float* hi32;
+    int mode = kRoundLp;
     __device__ void operator()(int j, double aty) const {
-        c32[j] = to_f32_bound(dc[j] * dd * (c[j] + aty));
-        lo32[j] = lo[j] > -kInf ? to_f32_bound(dp * (lo[j] - xb[j]) / dc[j]) : -kFltInf;
-        hi32[j] = hi[j] < kInf ? to_f32_bound(dp * (hi[j] - xb[j]) / dc[j]) : kFltInf;
+        c32[j] = mode == kRoundPrimalFeas ? 0.0f : to_f32_bound(dc[j] * dd * (c[j] + aty));
+        const bool homog = mode == kRoundDualFeas;
+        lo32[j] = lo[j] > -kInf ? (homog ? 0.0f : to_f32_bound(dp * (lo[j] - xb[j]) / dc[j])) : -kFltInf;
+        hi32[j] = hi[j] < kInf ? (homog ? 0.0f : to_f32_bound(dp * (hi[j] - xb[j]) / dc[j])) : kFltInf;
     }
 };
-// Row data: bounds D_r DP (l - A x_base), shift g = -DD y_base / D_r.
+// Row data: bounds D_r DP (l - A x_base), shift g = -DD y_base / D_r
+// (finite bounds 0 in kRoundDualFeas).
 struct EpiRoundRow {
     const double* lo;
     const double* hi;
@@ -500,9 +509,11 @@ struct EpiRoundRow {
     float* lo32;
     float* hi32;
     float* g32;
+    int mode = kRoundLp;
     __device__ void operator()(int i, double ax) const {
-        lo32[i] = lo[i] > -kInf ? to_f32_bound(dr[i] * dp * (lo[i] - ax)) : -kFltInf;
-        hi32[i] = hi[i] < kInf ? to_f32_bound(dr[i] * dp * (hi[i] - ax)) : kFltInf;
+        const bool homog = mode == kRoundDualFeas;
+        lo32[i] = lo[i] > -kInf ? (homog ? 0.0f : to_f32_bound(dr[i] * dp * (lo[i] - ax))) : -kFltInf;
+        hi32[i] = hi[i] < kInf ? (homog ? 0.0f : to_f32_bound(dr[i] * dp * (hi[i] - ax))) : kFltInf;
         g32[i] = to_f32_bound(-dd * yb[i] / dr[i]);
     }
 };
