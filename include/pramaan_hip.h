@@ -184,6 +184,8 @@ typedef struct pm_pdhg_result {
     double* y;                 /* caller-owned [m] or NULL (reference sign:
                                   reduced cost is c + A^T y)                 */
     double* row_activity;      /* caller-owned [m] or NULL: A x in FP64      */
+    int32_t polish_attempts;   /* PDLP feasibility polishes tried (the last
+                                  accepted when it ended the solve)        */
 } pm_pdhg_result;
 
 typedef struct pm_ctx pm_ctx;

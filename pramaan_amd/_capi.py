@@ -54,7 +54,7 @@ class pm_pdhg_result(C.Structure):
                 ("dual_residual", C.c_double), ("rel_kkt", C.c_double), ("rel_primal", C.c_double),
                 ("rel_dual", C.c_double), ("rel_gap", C.c_double), ("primal_objective", C.c_double),
                 ("dual_objective", C.c_double), ("primal_weight", C.c_double), ("device_seconds", C.c_double),
-                ("x", _d), ("y", _d), ("row_activity", _d)]
+                ("x", _d), ("y", _d), ("row_activity", _d), ("polish_attempts", C.c_int32)]
 
 
 class pm_gen_params(C.Structure):
@@ -269,12 +269,14 @@ class Result:
     x: np.ndarray
     y: np.ndarray
     row_activity: np.ndarray
+    polish_attempts: int = 0
 
 
 def _result(r: pm_pdhg_result, x, y, act) -> Result:
     return Result(SOLVE_STATUS[r.status], r.iterations, r.restarts, r.refinements, r.objective,
                   r.primal_residual, r.dual_residual, r.rel_kkt, r.rel_primal, r.rel_dual, r.rel_gap,
-                  r.primal_objective, r.dual_objective, r.primal_weight, r.device_seconds, x, y, act)
+                  r.primal_objective, r.dual_objective, r.primal_weight, r.device_seconds, x, y, act,
+                  r.polish_attempts)
 
 
 def _opts_with_warm(lp: LP, kw: dict):

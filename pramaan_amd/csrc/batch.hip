@@ -1210,7 +1210,10 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
             ctl.total_it = spent;
             st.have_cand = 1;
         }
-        if (tid == 0) pol.phase = Polish::MAIN;
+        if (tid == 0) {
+            pol.phase = Polish::MAIN;
+            ++pol.fails;
+        }
     }
     bool done = act == Control::OPTIMAL || act == Control::FAIL || act == Control::LIMIT;
     if (act == Control::REFINE) {
@@ -1251,6 +1254,7 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
             d.iterations = ctl.total_it;
             d.restarts = ctl.restarts;
             d.refinements = ctl.rounds;
+            d.polishes = pol.attempts;
             d.omega = ctl.omega;
             d.t_end = wall_clock64();
             st.active = 0;
@@ -1807,6 +1811,7 @@ void batch_run(pm_batch* b, pm_pdhg_result* results, pm_batch_stats* stats) {
         r.iterations = d.iterations;
         r.restarts = d.restarts;
         r.refinements = d.refinements;
+        r.polish_attempts = d.polishes;
         r.rel_kkt = d.rel_kkt;
         r.rel_primal = d.rel_p;
         r.rel_dual = d.rel_d;

@@ -33,7 +33,7 @@ struct BatchDesc {
     // these offsets in the sliced arrays of A and A^T
     int32_t la_off, lt_off;
     // results
-    int32_t status, iterations, restarts, refinements;
+    int32_t status, iterations, restarts, refinements, polishes;
     double rel_kkt, rel_p, rel_d, rel_g, pobj, dobj, omega;
     uint64_t t_start, t_end;
 };

@@ -218,25 +218,28 @@ struct Polish {
     static constexpr double kGap = 0.1, kRel = 3.0, kTarget = 0.3, kFrac = 0.1, kAccept = 0.5;
     static constexpr int kEvery = 512, kMinBudget = 256;
 
-    int32_t enabled, phase, k, it, since, budget, last_try;
+    int32_t enabled, phase, k, it, since, budget, last_try, attempts, fails;
     double q_last, q_prev;
     double save_rel, save_p, save_d;  // errors of the saved candidate
 
     __host__ __device__ void init(bool on) {
         enabled = on ? 1 : 0;
         phase = MAIN;
-        k = it = since = budget = 0;
+        k = it = since = budget = attempts = fails = 0;
         last_try = -(1 << 30);
         q_last = -1.0;
         q_prev = INFINITY;
         save_rel = save_p = save_d = 0.0;
     }
-    // Whether a main-solve check of (rel, rel_g) at total_it starts polishing.
+    // Whether a main-solve check of (rel, rel_g) at total_it starts polishing;
+    // each failed attempt doubles the wait before the next (up to 64x).
     __host__ __device__ bool trigger(double tol, int total_it, double rel, double rel_g) const {
+        const int wait = kEvery << (fails < 6 ? fails : 6);
         return enabled && phase == MAIN && rel > tol && rel <= kRel * tol && rel_g <= kGap * tol &&
-               total_it - last_try >= kEvery;
+               total_it - last_try >= wait;
     }
     __host__ __device__ void save(int total_it, double rel, double rel_p, double rel_d) {
+        ++attempts;
         last_try = total_it;
         save_rel = rel;
         save_p = rel_p;

@@ -1184,6 +1184,7 @@ bool polish(pm_ctx* c, Control& ctl, Polish& pol, const Kkt& kk, int K, bool ref
     const int spent = ctl.total_it;
     ctl = held;
     ctl.total_it = spent;
+    ++pol.fails;
     hold_state(c, false);
     launch_elem(n, FnCopy<double>{c->xtot.get(), c->xsave.get()}, s);
     if (m > 0) launch_elem(m, FnCopy<double>{c->ytot.get(), c->ysave.get()}, s);
@@ -1321,6 +1322,7 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
     res->iterations = ctl.total_it;
     res->restarts = ctl.restarts;
     res->refinements = ctl.rounds;
+    res->polish_attempts = pol.attempts;
     res->primal_weight = ctl.omega;
     if (res->status != PM_SOLVE_OPTIMAL && best_rel < INFINITY) {
         launch_elem(n, FnCopy<double>{c->xtot.get(), c->xbest.get()}, s);
@@ -1348,6 +1350,7 @@ void ctx_solve(pm_ctx* c, pm_pdhg_result* res) {
     res->iterations = 0;
     res->restarts = 0;
     res->refinements = 0;
+    res->polish_attempts = 0;
     res->objective = 0.0;
     res->primal_residual = INFINITY;
     res->dual_residual = INFINITY;
