@@ -383,6 +383,9 @@ __device__ __forceinline__ void publish_iter(LpIter* it, int lp, const BatchStat
     r.sig = static_cast<float>(st.eta * om);
     r.k0 = st.pol.phase != Polish::MAIN ? st.pol.k : st.ctl.k;
     r.chunk = st.active ? st.ctl.chunk(K) : 0;
+    // A polished pair rides one iteration (whose iterates are discarded)
+    // so that the grid-wide passes evaluate its KKT error.
+    if (st.pol.phase == Polish::VERIFY && r.chunk > 1) r.chunk = 1;
     r.use_g = st.ctl.use_g ? 1 : 0;
     r.row_off = static_cast<int32_t>(d.row_off);
     r.col_off = static_cast<int32_t>(d.col_off);
@@ -972,6 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_b_items(BatchArrays<VT> A, const Bat
                 });
             }
         } else if constexpr (PASS == kPassTotals) {
+            if (S[lp].pol.phase == Polish::VERIFY) continue;  // x_tot, y_tot hold the polished pair
             if (rows) {
                 const double invd = 1.0 / S[lp].ctl.DD;
                 for (int r = tid; r < cnt; r += kBlock) {
@@ -1035,8 +1039,8 @@ enum PolishEvent {
     kEvStartPrimal,   // save the candidate; primal phase from x_cand
     kEvStartDual,     // save the candidate (x_cand is already polished); dual phase from y_cand
     kEvPrimalToDual,  // keep the polished x; dual phase from the saved y
-    kEvCombine,       // polished x with the saved y (primal phase done, dual not needed)
-    kEvCombineDual,   // polished x with the polished y (dual phase done)
+    kEvCombine,       // polished x with the saved y (primal phase done, dual not needed): verify next
+    kEvCombineDual,   // polished x with the polished y (dual phase done): verify next
     kEvResume,        // back to the main solve, restored as it was when the polish began
     kEvPhaseRestart,  // restart inside a phase
     kEvLimit          // budget spent inside a phase: the saved candidate is the result
@@ -1102,12 +1106,18 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
                     if (!primal) ctl.use_g = true;
                     e = primal ? kEvStartPrimal : kEvStartDual;
                 }
+            } else if (pol.phase == Polish::VERIFY) {
+                // the polished pair's KKT terms came from the grid-wide passes
+                pol.phase = Polish::MAIN;
+                if (Polish::accept(A.tol, rel, d.rel_g)) a = Control::OPTIMAL;
+                else e = ctl.total_it >= ctl.max_iterations ? kEvLimit : kEvResume;
             } else {
                 const bool dual_phase = pol.phase == Polish::DUAL;
                 const int pa = pol.on_check(A.tol, dual_phase ? d.rel_d : d.rel_p, chunk);
                 const bool spent = ctl.total_it >= ctl.max_iterations;
-                if (pa == Polish::DONE && (dual_phase || !pol.needs(A.tol, pol.save_d))) {
+                if (pa == Polish::DONE && (dual_phase || !pol.needs(A.tol, pol.save_d)) && !spent) {
                     e = dual_phase ? kEvCombineDual : kEvCombine;
+                    pol.phase = Polish::VERIFY;
                 } else if (spent) {
                     e = kEvLimit;
                 } else if (pa == Polish::DONE) {
@@ -1172,12 +1182,6 @@ __global__ __launch_bounds__(kBB) void k_b_check(BatchArrays<VT> A, BatchState* 
             for (int i = tid; i < m; i += kBB) v.ytot[i] = v.ysave[i];
         else
             for (int j = tid; j < n; j += kBB) v.xtot[j] = v.xpol[j];
-        __syncthreads();
-        const double rel = batch_kkt(v, d, red, keep);
-        if (tid == 0) act_s = Polish::accept(A.tol, rel, d.rel_g) ? Control::OPTIMAL : Control::CONTINUE;
-        __syncthreads();
-        act = act_s;
-        if (act != Control::OPTIMAL) ev = ctl.total_it >= ctl.max_iterations ? kEvLimit : kEvResume;
     } else if (ev == kEvPhaseRestart) {
         for (int j = tid; j < n; j += kBB) v.x[j] = v.x0[j] = v.xl[j] = v.xt[j];
         for (int i = tid; i < m; i += kBB) v.y[i] = v.y0[i] = v.yl[i] = v.yt[i];

@@ -213,7 +213,7 @@ struct Control {
 // without polishing, 4,067 with these; accepting on the KKT error alone
 // would give 3,398 at the objective accuracy cost above).
 struct Polish {
-    enum Phase { MAIN = 0, PRIMAL = 1, DUAL = 2 };
+    enum Phase { MAIN = 0, PRIMAL = 1, DUAL = 2, VERIFY = 3 };  // VERIFY: the pair's KKT is being evaluated
     enum Action { CONTINUE = 0, DONE = 1, FAILED = 2, RESTART = 3 };
     static constexpr double kGap = 0.1, kRel = 3.0, kTarget = 0.3, kFrac = 0.1, kAccept = 0.5;
     static constexpr int kEvery = 512, kMinBudget = 256;
