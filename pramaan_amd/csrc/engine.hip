@@ -1137,7 +1137,7 @@ bool polish_phase(pm_ctx* c, Control& ctl, Polish& pol, int phase, double scale,
 // Returns true when the pair is accepted (it is left in xtot / ytot and
 // *out holds its KKT); otherwise the main solve's state is restored as it
 // was and only the iterations spent remain counted.
-bool polish(pm_ctx* c, Control& ctl, Polish& pol, const Kkt& kk, int K, bool ref_test, Kkt* out) {
+bool polish(pm_ctx* c, Control& ctl, Polish& pol, const Kkt& kk, int K, Kkt* out) {
     hipStream_t s = c->stream;
     const int m = c->m, n = c->n;
     const double tol = ctl.tol;
@@ -1170,13 +1170,7 @@ bool polish(pm_ctx* c, Control& ctl, Polish& pol, const Kkt& kk, int K, bool ref
     if (ok) {
         launch_elem(n, FnCopy<double>{c->xtot.get(), c->xpol.get()}, s);
         const Kkt pk = eval_kkt(c, c->xtot.get(), c->ytot.get());
-        bool accept = Polish::accept(tol, pk.rel, pk.rel_g);
-        if (accept && ref_test) {
-            double pv = 0.0, dv = 0.0;
-            ref_style_residuals(c, c->xtot.get(), c->ytot.get(), c->opts.tolerance, pv, dv);
-            accept = pv <= c->opts.tolerance && dv <= c->opts.tolerance;
-        }
-        if (accept) {
+        if (Polish::accept(tol, pk.rel, pk.rel_g)) {
             *out = pk;
             return true;
         }
@@ -1234,14 +1228,18 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
 
     Control ctl;
     ctl.init(o.tolerance, o.max_iterations, o.max_refinements, c->omega0, warm && !c->y0_h.empty() && m > 0);
-    Polish pol;
-    pol.init(o.feasibility_polishing != 0);
     res->status = PM_SOLVE_ITERATION_LIMIT;
     // Under PM_TERM_REFERENCE a point is optimal only when the reference's
     // own measures agree (pv, dv <= tolerance, pdhg_solver.cu:356): each
     // check that meets the relative target but not them tightens the
     // target tenfold and the solve goes on.
     const bool ref_test = o.termination == PM_TERM_REFERENCE;
+    // Feasibility polishing stays off under the reference's test: a polished
+    // primal point is feasible but not complementary entry by entry (values
+    // slightly above a bound where the reduced cost is large), which the
+    // reference's dv counts in full, so its pairs would be rejected.
+    Polish pol;
+    pol.init(o.feasibility_polishing != 0 && !ref_test);
     const auto t0 = std::chrono::steady_clock::now();
     bool have_candidate = false;
     bool round_start = true;
@@ -1283,7 +1281,7 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
         }
         if (act == Control::CONTINUE && pol.trigger(ctl.tol, ctl.total_it, kk.rel, kk.rel_g)) {
             Kkt pk;
-            if (polish(c, ctl, pol, kk, K, ref_test, &pk)) {
+            if (polish(c, ctl, pol, kk, K, &pk)) {
                 if (o.verbose)
                     fprintf(stderr, "[pramaan] it %d polished kkt %.3e (p %.2e d %.2e g %.2e)\n", ctl.total_it, pk.rel,
                             pk.rel_p, pk.rel_d, pk.rel_g);
