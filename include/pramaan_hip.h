@@ -76,6 +76,9 @@ extern "C" {
  * pairs fail with PM_ERR_INVALID_ARG. */
 #define PM_TERM_REFERENCE    0 /* pv <= tol && dv <= tol (pdhg_solver.cu:356) */
 #define PM_TERM_RELATIVE_KKT 1 /* max(rel primal, rel dual, rel gap) <= tol   */
+/* Restart metric of PM_ALG_PDLP (pm_pdhg_options.restart_metric). */
+#define PM_RESTART_KKT 0 /* the FP64 relative KKT error of the candidate   */
+#define PM_RESTART_GAP 1 /* PDLP's normalized duality gap of the candidate */
 
 /* ---- SpMV variants ------------------------------------------------------ */
 #define PM_SPMV_AUTO        0  /* autotuned per matrix                        */
@@ -151,6 +154,10 @@ typedef struct pm_pdhg_options {
                                   (PM_ALG_PDLP only: the reference
                                   algorithm rejects a warm start)          */
     const double* y0;          /* optional warm start [m], reference sign    */
+    int32_t restart_metric;    /* PDLP: what the adaptive restart rule
+                                  compares, PM_RESTART_KKT (0, default) or
+                                  PM_RESTART_GAP (single-LP engine only;
+                                  the batch engine rejects it)              */
     int32_t feasibility_polishing; /* PDLP: 1 (default) = once the gap is small,
                                   solve the primal and the dual feasibility
                                   problems from the candidate and accept

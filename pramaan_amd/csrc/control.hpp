@@ -108,8 +108,11 @@ struct Control {
         it_since += iters;
     }
 
-    // rel = max(rel_p, rel_d, rel_g) of the candidate.
-    __host__ __device__ Action on_check(double rel, double rel_p, double rel_d, double rel_g) {
+    // rel = max(rel_p, rel_d, rel_g) of the candidate.  q >= 0: the quantity
+    // the restart rule compares instead of rel (the normalized duality gap,
+    // for LPs whose every column is boxed); termination, refinement rounds
+    // and stalls always use rel.
+    __host__ __device__ Action on_check(double rel, double rel_p, double rel_d, double rel_g, double q = -1.0) {
         if (!(rel == rel) || rel == INFINITY) return FAIL;
         last_p = rel_p;
         last_d = rel_d;
@@ -142,16 +145,17 @@ struct Control {
             new_round();
             return REFINE;
         }
-        if (k_last < 0.0) k_last = rel;
+        const double r = q >= 0.0 ? q : rel;
+        if (k_last < 0.0) k_last = r;
         const bool stall_restart = stalled && !stall_restarted;
         if (stall_restart) stall_restarted = true;
-        const bool restart = rel <= kSuff * k_last || (rel <= kNec * k_last && rel > k_prev) ||
+        const bool restart = r <= kSuff * k_last || (r <= kNec * k_last && r > k_prev) ||
                              static_cast<double>(it_since) >= kArt * static_cast<double>(total_it) ||
                              stall_restart;
-        k_prev = rel;
+        k_prev = r;
         if (restart) {
             k = 0;
-            k_last = rel;
+            k_last = r;
             k_prev = INFINITY;
             it_since = 0;
             ++restarts;

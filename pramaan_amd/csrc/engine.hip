@@ -131,6 +131,79 @@ struct FnNormalize {
     }
 };
 
+// PDLP's normalized duality gap of a candidate z = (x, y) of the current
+// round's FP32 problem, on the ball of radius r = ||z - z0||_omega around
+// it (||(dx, dy)||^2_omega = omega |dx|^2 + |dy|^2 / omega): the largest
+// Lagrangian gap L(x, y^) - L(x^, y) over the ball, divided by r.  For a
+// step scale t the maximiser of the gap minus ||z^ - z||^2_omega / (2 t) is
+// one prox step from z with steps t / omega (primal) and t omega (dual);
+// its distance grows with t, so the ball's maximiser is that step at the t
+// whose distance is r.  One pass evaluates kNdgT values of t at once:
+// sums [2k] = the squared distance contribution, [2k + 1] = the gap
+// contribution at t[k].  Engine sign: reduced cost c + A^T y, and a row's
+// y - g is >= 0 only with a finite upper bound, <= 0 only with a finite
+// lower one, g the round's dual shift.
+constexpr int kNdgT = 16;
+struct NdgTs {
+    double t[kNdgT];
+};
+struct FnNdgX {
+    static constexpr int NS = 2 * kNdgT, NM = 0;
+    const float* x;    // candidate
+    const float* aty;  // A^T y of the candidate's y
+    const float* c;
+    const float* lo;
+    const float* hi;
+    double omega;
+    NdgTs ts;
+    __device__ void operator()(int j, double* acc) const {
+        const double xc = x[j], g = static_cast<double>(c[j]) + static_cast<double>(aty[j]);
+        const double l = lo[j], u = hi[j];
+#pragma unroll
+        for (int k = 0; k < kNdgT; ++k) {
+            double xh = xc - ts.t[k] / omega * g;
+            xh = xh < l ? l : (xh > u ? u : xh);
+            const double d = xh - xc;
+            acc[2 * k] = omega * d * d;
+            acc[2 * k + 1] = -g * d;
+        }
+    }
+};
+struct FnNdgY {
+    static constexpr int NS = 2 * kNdgT, NM = 0;
+    const float* y;   // candidate
+    const float* ax;  // A x of the candidate's x
+    const float* lo;
+    const float* hi;
+    const float* gs;  // dual shift, or null
+    double omega;
+    NdgTs ts;
+    // support term of the row: (y - g) u when y - g > 0, (y - g) l when < 0
+    __device__ static double supp(double v, double l, double u) { return v > 0.0 ? v * u : (v < 0.0 ? v * l : 0.0); }
+    __device__ void operator()(int i, double* acc) const {
+        const double yc = y[i], a = ax[i], l = lo[i], u = hi[i], g = gs ? static_cast<double>(gs[i]) : 0.0;
+        const double h0 = supp(yc - g, l, u);
+#pragma unroll
+        for (int k = 0; k < kNdgT; ++k) {
+            const double sg = ts.t[k] * omega;
+            const double sv = yc / sg + a - g / sg;
+            const double yh = g + sg * (sv - (sv < l ? l : (sv > u ? u : sv)));
+            const double d = yh - yc;
+            acc[2 * k] = d * d / omega;
+            acc[2 * k + 1] = a * d - (supp(yh - g, l, u) - h0);
+        }
+    }
+};
+struct FnNdgScale {  // sums: [0] |c + A^T y|^2, [1] |A x|^2 over the two sides
+    static constexpr int NS = 1, NM = 0;
+    const float* a;
+    const float* b;  // added to a, or null
+    __device__ void operator()(int i, double* acc) const {
+        const double v = static_cast<double>(a[i]) + (b ? static_cast<double>(b[i]) : 0.0);
+        acc[0] = v * v;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // SpMV autotuning cache: (rows, cols, nnz) -> variant
 // ---------------------------------------------------------------------------
@@ -211,6 +284,11 @@ struct pm_ctx {
     // FP64 x_base, y_base).
     DevBuf<double> xsave, ysave, xpol, hold64;
     DevBuf<float> hold32;
+    // PDLP restarts on the normalized duality gap (PM_RESTART_GAP): block
+    // partials of the multi-t passes.
+    bool restart_on_gap = false;
+    DevBuf<double> ndg_part, ndg_red;
+    PinnedBuf<double> ndg_h;
     DevBuf<double> partials, red;
     PinnedBuf<double> hred;
 
@@ -522,6 +600,8 @@ void check_options(const pm_pdhg_options& opts) {
         fail(PM_ERR_INVALID_ARG, "unknown algorithm");
     if (opts.termination != PM_TERM_REFERENCE && opts.termination != PM_TERM_RELATIVE_KKT)
         fail(PM_ERR_INVALID_ARG, "unknown termination");
+    if (opts.restart_metric != PM_RESTART_KKT && opts.restart_metric != PM_RESTART_GAP)
+        fail(PM_ERR_INVALID_ARG, "unknown restart metric");
     // The reference algorithm is the reference's solver unchanged: its own
     // test and its own starting point (pdhg_solver.cu:288-297,356).
     if (opts.algorithm == PM_ALG_REFERENCE && opts.termination != PM_TERM_REFERENCE)
@@ -601,6 +681,12 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
     for (auto* b : {&c->y, &c->y0, &c->yt, &c->yl, &c->tmp_m}) b->resize(std::max(m, 1));
     for (auto* b : {&c->xtot, &c->xbase, &c->xbest}) b->resize(n);
     for (auto* b : {&c->ytot, &c->ybase, &c->act64, &c->ybest}) b->resize(std::max(m, 1));
+    c->restart_on_gap = opts.algorithm == PM_ALG_PDLP && opts.restart_metric == PM_RESTART_GAP;
+    if (c->restart_on_gap) {
+        c->ndg_part.resize(static_cast<size_t>((std::max(m, n) + kBlock - 1) / kBlock + 1) * 2 * kNdgT + 64);
+        c->ndg_red.resize(4 * kNdgT);
+        c->ndg_h.resize(4 * kNdgT);
+    }
     if (opts.algorithm == PM_ALG_PDLP && opts.feasibility_polishing) {
         c->xsave.resize(n);
         c->xpol.resize(n);
@@ -1185,6 +1271,69 @@ bool polish(pm_ctx* c, Control& ctl, Polish& pol, const Kkt& kk, int K, Kkt* out
     return false;
 }
 
+// The normalized duality gap of the candidate (xt, yt) of the current
+// round on the ball around it of radius ||(xt, yt) - (x0, y0)||_omega
+// (FnNdgX, FnNdgY): three passes of kNdgT step scales each (a logarithmic
+// sweep, then two linear refinements of the bracket where the distance
+// crosses the radius), the gap interpolated at the crossing.  Returns -1
+// (use the KKT error) when the radius is 0.
+double normalized_gap(pm_ctx* c, double omega, bool use_g) {
+    hipStream_t s = c->stream;
+    const int m = c->m, n = c->n;
+    double* red = c->red.get();
+    launch_elem_reduce(n, FnNormDiff{c->xt.get(), c->x0.get()}, c->partials.get(), red + 50, s);
+    launch_elem_reduce(m, FnNormDiff{c->yt.get(), c->y0.get()}, c->partials.get(), red + 51, s);
+    launch_rows<float>(c->var_at, n, c->cp.get(), c->ri.get(), c->vat.get(), c->yt.get(),
+                       EpiStore<float>{c->tmp_n.get()}, s);
+    if (m > 0)
+        launch_rows<float>(c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), c->xt.get(),
+                           EpiStore<float>{c->tmp_m.get()}, s);
+    launch_elem_reduce(n, FnNdgScale{c->tmp_n.get(), c->c32.get()}, c->partials.get(), red + 52, s);
+    launch_elem_reduce(m, FnNdgScale{c->tmp_m.get(), nullptr}, c->partials.get(), red + 53, s);
+    PM_HIP(hipMemcpyAsync(c->hred.get() + 50, red + 50, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    const double* h = c->hred.get();
+    const double r = std::sqrt(omega * h[50] + h[51] / omega);
+    if (!(r > 0.0) || !std::isfinite(r)) return -1.0;
+    const double t0 = r / std::sqrt(h[52] / omega + omega * h[53] + 1e-300);
+    double nrm[kNdgT], val[kNdgT];
+    auto pass = [&](const NdgTs& ts) {
+        double* out = c->ndg_red.get();
+        launch_elem_reduce(n, FnNdgX{c->xt.get(), c->tmp_n.get(), c->c32.get(), c->lx32.get(), c->ux32.get(), omega, ts},
+                           c->ndg_part.get(), out, s);
+        launch_elem_reduce(m, FnNdgY{c->yt.get(), c->tmp_m.get(), c->lr32.get(), c->ur32.get(),
+                                     use_g ? c->g32.get() : nullptr, omega, ts},
+                           c->ndg_part.get(), out + 2 * kNdgT, s);
+        PM_HIP(hipMemcpyAsync(c->ndg_h.get(), out, 4 * kNdgT * sizeof(double), hipMemcpyDeviceToHost, s));
+        sync(c);
+        const double* o = c->ndg_h.get();
+        for (int k = 0; k < kNdgT; ++k) {
+            nrm[k] = std::sqrt(o[2 * k] + o[2 * kNdgT + 2 * k]);
+            val[k] = o[2 * k + 1] + o[2 * kNdgT + 2 * k + 1];
+        }
+    };
+    NdgTs ts;
+    for (int k = 0; k < kNdgT; ++k) ts.t[k] = t0 * std::pow(10.0, k - 5);
+    pass(ts);
+    double lo_t = 0.0, lo_n = 0.0, lo_v = 0.0, hi_t = 0.0, hi_n = 0.0, hi_v = 0.0;
+    for (int round = 0; round < 3; ++round) {
+        int k = 0;
+        while (k < kNdgT && nrm[k] < r) ++k;
+        if (k == kNdgT) {
+            if (round == 0) return val[kNdgT - 1] / r;  // the box ends before the ball: its far point
+            lo_t = ts.t[kNdgT - 1]; lo_n = nrm[kNdgT - 1]; lo_v = val[kNdgT - 1];  // (hi unchanged)
+        } else {
+            if (k > 0) { lo_t = ts.t[k - 1]; lo_n = nrm[k - 1]; lo_v = val[k - 1]; }
+            hi_t = ts.t[k]; hi_n = nrm[k]; hi_v = val[k];
+        }
+        if (round == 2) break;
+        for (int q = 0; q < kNdgT; ++q) ts.t[q] = lo_t + (hi_t - lo_t) * (q + 1) / (kNdgT + 1);
+        pass(ts);
+    }
+    const double w = hi_n > lo_n ? (r - lo_n) / (hi_n - lo_n) : 1.0;
+    return (lo_v + w * (hi_v - lo_v)) / r;
+}
+
 void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
     hipStream_t s = c->stream;
     const int m = c->m, n = c->n;
@@ -1265,10 +1414,14 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
             launch_elem(n, FnCopy<double>{c->xbest.get(), c->xtot.get()}, s);
             if (m > 0) launch_elem(m, FnCopy<double>{c->ybest.get(), c->ytot.get()}, s);
         }
-        const int act = ctl.on_check(kk.rel, kk.rel_p, kk.rel_d, kk.rel_g);
-        if (o.verbose)
-            fprintf(stderr, "[pramaan] it %d round %d kkt %.3e (p %.2e d %.2e g %.2e) omega %.3g restarts %d\n",
+        const double q = c->restart_on_gap && kk.rel > ctl.tol ? normalized_gap(c, ctl.omega, ctl.use_g) : -1.0;
+        const int act = ctl.on_check(kk.rel, kk.rel_p, kk.rel_d, kk.rel_g, q);
+        if (o.verbose) {
+            fprintf(stderr, "[pramaan] it %d round %d kkt %.3e (p %.2e d %.2e g %.2e) omega %.3g restarts %d",
                     ctl.total_it, ctl.rounds, kk.rel, kk.rel_p, kk.rel_d, kk.rel_g, ctl.omega, ctl.restarts);
+            if (q >= 0.0) fprintf(stderr, " ndg %.3e", q);
+            fprintf(stderr, "\n");
+        }
         if (act == Control::OPTIMAL) {
             double pv = 0.0, dv = 0.0;
             if (ref_test) ref_style_residuals(c, c->xtot.get(), c->ytot.get(), o.tolerance, pv, dv);
