@@ -79,6 +79,9 @@ extern "C" {
 /* Restart metric of PM_ALG_PDLP (pm_pdhg_options.restart_metric). */
 #define PM_RESTART_KKT 0 /* the FP64 relative KKT error of the candidate   */
 #define PM_RESTART_GAP 1 /* PDLP's normalized duality gap of the candidate */
+/* Step rule of PM_ALG_PDLP (pm_pdhg_options.step_rule). */
+#define PM_STEP_HALPERN 0  /* reflected Halpern, constant step 0.99/||A||_2  */
+#define PM_STEP_ADAPTIVE 1 /* plain PDHG with PDLP's adaptive step size      */
 
 /* ---- SpMV variants ------------------------------------------------------ */
 #define PM_SPMV_AUTO        0  /* autotuned per matrix                        */
@@ -157,6 +160,9 @@ typedef struct pm_pdhg_options {
     int32_t restart_metric;    /* PDLP: what the adaptive restart rule
                                   compares, PM_RESTART_KKT (0, default) or
                                   PM_RESTART_GAP (single-LP engine only;
+                                  the batch engine rejects it)              */
+    int32_t step_rule;         /* PDLP: PM_STEP_HALPERN (0, default) or
+                                  PM_STEP_ADAPTIVE (single-LP engine only;
                                   the batch engine rejects it)              */
     int32_t feasibility_polishing; /* PDLP: 1 (default) = once the gap is small,
                                   solve the primal and the dual feasibility

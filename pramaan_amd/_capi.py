@@ -45,7 +45,7 @@ class pm_pdhg_options(C.Structure):
                 ("algorithm", C.c_int32), ("termination", C.c_int32), ("spmv_variant", C.c_int32),
                 ("ruiz_iterations", C.c_int32), ("max_refinements", C.c_int32), ("device", C.c_int32),
                 ("verbose", C.c_int32), ("primal_weight", C.c_double), ("x0", _d), ("y0", _d),
-                ("restart_metric", C.c_int32), ("feasibility_polishing", C.c_int32)]
+                ("restart_metric", C.c_int32), ("step_rule", C.c_int32), ("feasibility_polishing", C.c_int32)]
 
 
 class pm_pdhg_result(C.Structure):
@@ -239,6 +239,9 @@ def options(**kw) -> pm_pdhg_options:
     term = kw.pop("termination", None)
     if term is not None:
         o.termination = {"reference": PM_TERM_REFERENCE, "relative_kkt": PM_TERM_RELATIVE_KKT}.get(term, term)
+    sr = kw.pop("step_rule", None)
+    if sr is not None:
+        o.step_rule = {"halpern": 0, "adaptive": 1}.get(sr, sr)
     rm = kw.pop("restart_metric", None)
     if rm is not None:
         o.restart_metric = {"kkt": 0, "gap": 1}.get(rm, rm)

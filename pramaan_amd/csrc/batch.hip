@@ -1305,6 +1305,8 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     pm_pdhg_options_default(&opts);
     if (o) opts = *o;
     if (opts.check_frequency <= 0) fail(PM_ERR_INVALID_ARG, "check_frequency must be positive");
+    if (opts.step_rule != PM_STEP_HALPERN)
+        fail(PM_ERR_INVALID_ARG, "the batch engine runs the reflected Halpern iteration (PM_STEP_HALPERN)");
     if (opts.restart_metric != PM_RESTART_KKT)
         fail(PM_ERR_INVALID_ARG, "the batch engine restarts on the relative KKT error (PM_RESTART_KKT)");
     int ncu = 0;

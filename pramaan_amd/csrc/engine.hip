@@ -194,6 +194,23 @@ struct FnNdgY {
         }
     }
 };
+// dst = a - b (FP32).
+struct FnSubF {
+    float* dst;
+    const float* a;
+    const float* b;
+    __device__ void operator()(int i) const { dst[i] = a[i] - b[i]; }
+};
+// sums: [0] (y - ys)^T (A dx), the interaction term of PDLP's step bound.
+struct FnDyAdx {
+    static constexpr int NS = 1, NM = 0;
+    const float* y;
+    const float* ys;
+    const float* adx;
+    __device__ void operator()(int i, double* acc) const {
+        acc[0] = (static_cast<double>(y[i]) - static_cast<double>(ys[i])) * static_cast<double>(adx[i]);
+    }
+};
 struct FnNdgScale {  // sums: [0] |c + A^T y|^2, [1] |A x|^2 over the two sides
     static constexpr int NS = 1, NM = 0;
     const float* a;
@@ -287,6 +304,10 @@ struct pm_ctx {
     // PDLP restarts on the normalized duality gap (PM_RESTART_GAP): block
     // partials of the multi-t passes.
     bool restart_on_gap = false;
+    // PM_STEP_ADAPTIVE: the current step size and the saved iterate of a try
+    bool adaptive_step = false;
+    double eta_ad = 1.0;
+    DevBuf<float> xs, ys, adx;
     DevBuf<double> ndg_part, ndg_red;
     PinnedBuf<double> ndg_h;
     DevBuf<double> partials, red;
@@ -602,6 +623,8 @@ void check_options(const pm_pdhg_options& opts) {
         fail(PM_ERR_INVALID_ARG, "unknown termination");
     if (opts.restart_metric != PM_RESTART_KKT && opts.restart_metric != PM_RESTART_GAP)
         fail(PM_ERR_INVALID_ARG, "unknown restart metric");
+    if (opts.step_rule != PM_STEP_HALPERN && opts.step_rule != PM_STEP_ADAPTIVE)
+        fail(PM_ERR_INVALID_ARG, "unknown step rule");
     // The reference algorithm is the reference's solver unchanged: its own
     // test and its own starting point (pdhg_solver.cu:288-297,356).
     if (opts.algorithm == PM_ALG_REFERENCE && opts.termination != PM_TERM_REFERENCE)
@@ -681,6 +704,12 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
     for (auto* b : {&c->y, &c->y0, &c->yt, &c->yl, &c->tmp_m}) b->resize(std::max(m, 1));
     for (auto* b : {&c->xtot, &c->xbase, &c->xbest}) b->resize(n);
     for (auto* b : {&c->ytot, &c->ybase, &c->act64, &c->ybest}) b->resize(std::max(m, 1));
+    c->adaptive_step = opts.algorithm == PM_ALG_PDLP && opts.step_rule == PM_STEP_ADAPTIVE;
+    if (c->adaptive_step) {
+        c->xs.resize(n);
+        c->ys.resize(std::max(m, 1));
+        c->adx.resize(std::max(m, 1));
+    }
     c->restart_on_gap = opts.algorithm == PM_ALG_PDLP && opts.restart_metric == PM_RESTART_GAP;
     if (c->restart_on_gap) {
         c->ndg_part.resize(static_cast<size_t>((std::max(m, n) + kBlock - 1) / kBlock + 1) * 2 * kNdgT + 64);
@@ -1148,6 +1177,50 @@ void hal_iteration(pm_ctx* c, int k, double omega, bool keep_t, bool use_g) {
     }
 }
 
+// One step of plain PDHG with PDLP's adaptive step size (PM_STEP_ADAPTIVE):
+// a step of size eta (primal eta / omega, dual eta omega) is kept when eta
+// <= eta_max = ||dz||^2_omega / (2 |dy^T A dx|), the largest step the
+// PDHG convergence condition allows for that move; either way the next
+// size is min((1 - (k+1)^-0.3) eta_max, (1 + (k+1)^-0.6) eta), k >= 1 the
+// steps taken so far, and a rejected step is retried from the saved iterate.
+// The Halpern epilogues with lambda = 1 and no reflection are plain PDHG.
+// Returns the tries (each costs an iteration).
+int adaptive_iteration(pm_ctx* c, double omega, bool keep_t, bool use_g, int k) {
+    hipStream_t s = c->stream;
+    const int m = c->m, n = c->n;
+    double* red = c->red.get();
+    for (int tries = 1;; ++tries) {
+        launch_elem(n, FnCopy<float>{c->xs.get(), c->x.get()}, s);
+        launch_elem(m, FnCopy<float>{c->ys.get(), c->y.get()}, s);
+        const double eta = c->eta_ad;
+        EpiPrimalHal ep{static_cast<float>(eta / omega), 1.0f, 0.0f, c->c32.get(), c->lx32.get(), c->ux32.get(),
+                        c->x0.get(), c->x.get(), c->xe.get(), keep_t ? c->xt.get() : nullptr};
+        launch_primal_half(c, ep, s);
+        if (m > 0) {
+            EpiDualHal ed{static_cast<float>(eta * omega), 1.0f, 0.0f, c->lr32.get(), c->ur32.get(),
+                          use_g ? c->g32.get() : nullptr, c->y0.get(), c->y.get(), keep_t ? c->yt.get() : nullptr};
+            launch_dual_half(c, ed, s);
+        }
+        launch_elem(n, FnSubF{c->tmp_n.get(), c->x.get(), c->xs.get()}, s);
+        if (m > 0)
+            launch_rows<float>(c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), c->tmp_n.get(),
+                               EpiStore<float>{c->adx.get()}, s);
+        launch_elem_reduce(n, FnNormDiff{c->x.get(), c->xs.get()}, c->partials.get(), red + 56, s);
+        launch_elem_reduce(m, FnNormDiff{c->y.get(), c->ys.get()}, c->partials.get(), red + 57, s);
+        launch_elem_reduce(m, FnDyAdx{c->y.get(), c->ys.get(), c->adx.get()}, c->partials.get(), red + 58, s);
+        PM_HIP(hipMemcpyAsync(c->hred.get() + 56, red + 56, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        sync(c);
+        const double* h = c->hred.get();
+        const double inter = std::fabs(h[58]);
+        const double eta_max = inter > 0.0 ? (omega * h[56] + h[57] / omega) / (2.0 * inter) : INFINITY;
+        const double kk = std::max(k, 1) + 1.0;  // k counts from 1: the first factor is 1 - 2^-0.3
+        c->eta_ad = std::min((1.0 - std::pow(kk, -0.3)) * eta_max, (1.0 + std::pow(kk, -0.6)) * eta);
+        if (eta <= eta_max || tries >= 64) return tries;
+        launch_elem(n, FnCopy<float>{c->x.get(), c->xs.get()}, s);
+        launch_elem(m, FnCopy<float>{c->y.get(), c->ys.get()}, s);
+    }
+}
+
 // Candidate T(z) of the last iteration of a round with base (x_base, y_base)
 // and scale factors (dp, dd), as a total point in FP64 (xtot, ytot).
 void total_point(pm_ctx* c, double dp, double dd) {
@@ -1377,6 +1450,7 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
 
     Control ctl;
     ctl.init(o.tolerance, o.max_iterations, o.max_refinements, c->omega0, warm && !c->y0_h.empty() && m > 0);
+    c->eta_ad = c->eta;  // the adaptive rule starts from the constant step
     res->status = PM_SOLVE_ITERATION_LIMIT;
     // Under PM_TERM_REFERENCE a point is optimal only when the reference's
     // own measures agree (pv, dv <= tolerance, pdhg_solver.cu:356): each
@@ -1403,8 +1477,15 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
         }
         const int chunk = ctl.chunk(K);
         if (chunk <= 0) break;
-        for (int i = 0; i < chunk; ++i) hal_iteration(c, ctl.k + i, ctl.omega, i == chunk - 1, ctl.use_g);
+        int extra = 0;  // rejected adaptive steps, each an iteration's work
+        for (int i = 0; i < chunk; ++i) {
+            if (c->adaptive_step)
+                extra += adaptive_iteration(c, ctl.omega, i == chunk - 1, ctl.use_g, ctl.total_it + extra + i) - 1;
+            else
+                hal_iteration(c, ctl.k + i, ctl.omega, i == chunk - 1, ctl.use_g);
+        }
         ctl.advance(chunk);
+        ctl.total_it += extra;
         // Candidate = T(z) of the last iteration, as a total point in FP64.
         total_point(c, ctl.DP, ctl.DD);
         have_candidate = true;
@@ -1420,6 +1501,7 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
             fprintf(stderr, "[pramaan] it %d round %d kkt %.3e (p %.2e d %.2e g %.2e) omega %.3g restarts %d",
                     ctl.total_it, ctl.rounds, kk.rel, kk.rel_p, kk.rel_d, kk.rel_g, ctl.omega, ctl.restarts);
             if (q >= 0.0) fprintf(stderr, " ndg %.3e", q);
+            if (c->adaptive_step) fprintf(stderr, " step %.3e (constant %.3e)", c->eta_ad, c->eta);
             fprintf(stderr, "\n");
         }
         if (act == Control::OPTIMAL) {
