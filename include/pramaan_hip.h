@@ -116,7 +116,15 @@ extern "C" {
                                   checks autotuned as AUTO.  AUTO times it
                                   beside the row kernels on LPs with
                                   n >= 2^19 and nnz >= 2^22.              */
-#define PM_SPMV_NUM_VARIANTS 13
+#define PM_SPMV_MERGE       13 /* merge-path CSR (the product split evenly
+                                  over rows + nonzeros, whatever the row
+                                  lengths; a second pass adds the parts of
+                                  split rows in order and applies the fused
+                                  update).  Single-LP PDLP engine: both
+                                  products of the iteration; AUTO times it
+                                  beside the row kernels on LPs of at least
+                                  2^18 nonzeros.  Also pm_launch_spmv.     */
+#define PM_SPMV_NUM_VARIANTS 14
 
 /* ---- the LP, in the layout of ModelIR (include/pramaan/ir.hpp:62-119) ---
  * optimize sense: c^T x + offset  s.t.  row_lower <= A x <= row_upper,

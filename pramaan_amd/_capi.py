@@ -24,7 +24,7 @@ PM_ALG_REFERENCE, PM_ALG_PDLP = 0, 1
 PM_TERM_REFERENCE, PM_TERM_RELATIVE_KKT = 0, 1
 SPMV = {"auto": 0, "sequential": 1, "scalar": 2, "vector2": 3, "vector4": 4, "vector8": 5,
         "vector16": 6, "vector32": 7, "vector64": 8, "stream": 9, "sliced": 10,
-        "staged": 11, "colblock": 12}
+        "staged": 11, "colblock": 12, "merge": 13}
 SPMV_NAMES = {v: k for k, v in SPMV.items()}
 PM_KERNEL_SPMV_A, PM_KERNEL_SPMV_AT, PM_KERNEL_ITERATION = 0, 1, 2
 INF = 1.0e30

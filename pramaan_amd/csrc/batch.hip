@@ -1566,6 +1566,8 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->part_c.resize(b->ck_c.size() * kPart);
     b->wrange.resize(static_cast<size_t>(count));
     b->wcnt.resize(4);
+    if (opts.spmv_variant == PM_SPMV_MERGE || opts.spmv_variant == PM_SPMV_COLBLOCK)
+        fail(PM_ERR_INVALID_ARG, "spmv_variant PM_SPMV_MERGE and PM_SPMV_COLBLOCK are single-LP layouts");
     const int fixed = variant_kernel(opts.spmv_variant);
     if (fixed != kAutoKernel) b->lanes_a = b->lanes_at = fixed;
     if (fixed == kStagedKernel && std::max(b->win_a, b->win_at) > kStagedMaxVec)

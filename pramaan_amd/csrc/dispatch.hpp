@@ -111,6 +111,37 @@ void launch_rows_reduce(int variant, int nrows, const int* rp, const int* ci, co
     (void)NA;
 }
 
+// Tiles of a merge-path product over nrows rows and nnz nonzeros (the size
+// of its carry and head buffers; its partition holds one more).
+inline int merge_tiles(int nrows, int nnz) {
+    return static_cast<int>((static_cast<int64_t>(nrows) + nnz + kMergeTile - 1) / kMergeTile);
+}
+
+// Tile starts of a merge-path product (prow: merge_tiles + 1 ints), once
+// per matrix.
+inline void launch_merge_partition(int nrows, int nnz, const int* rp, int* prow, hipStream_t s) {
+    const int nt = merge_tiles(nrows, nnz);
+    hipLaunchKernelGGL(k_merge_partition<kMergeTile>, dim3((nt + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nrows,
+                       nnz, nt, rp, prow);
+    PM_LAUNCHED();
+}
+
+// Merge-path product with a fused epilogue (k_merge_tile, k_merge_fix) over
+// a partition from launch_merge_partition; carry and head hold
+// merge_tiles(nrows, nnz) floats.
+template <class Epi>
+void launch_merge(int nrows, int nnz, const int* rp, const int* ci, const float* v, const float* x, const Epi& epi,
+                  const int* prow, float* carry, float* head, hipStream_t s) {
+    if (nrows <= 0) return;
+    const int nt = merge_tiles(nrows, nnz);
+    hipLaunchKernelGGL((k_merge_tile<Epi>), dim3(nt), dim3(kBlock), 0, s, nrows, nnz, rp, ci, v, x, prow, carry, head,
+                       epi);
+    if (nt > 1)
+        hipLaunchKernelGGL((k_merge_fix<Epi>), dim3((nt + kBlock / kWave - 1) / (kBlock / kWave)), dim3(kBlock), 0, s,
+                           nt, rp, prow, carry, head, epi);
+    PM_LAUNCHED();
+}
+
 template <class F>
 void launch_elem(int n, const F& f, hipStream_t s) {
     if (n <= 0) return;

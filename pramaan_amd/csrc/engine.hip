@@ -330,6 +330,12 @@ struct pm_ctx {
     // used for the iteration's products when slices > 0.
     SellMat sell_a, sell_at;
 
+    // Merge-path products of the iteration (PM_SPMV_MERGE, PDLP): tile
+    // partitions of A and A^T, per-tile carries and head parts.
+    bool merge_a = false, merge_at = false;
+    DevBuf<int> mpart_a, mpart_at;
+    DevBuf<float> mcarry, mhead;
+
     const void* orig_a() const { return exact_f32 ? static_cast<const void*>(oa32.get()) : oa64.get(); }
 };
 
@@ -405,13 +411,19 @@ void launch_sell_mat(const SellMat& S, const float* x, const Epi& epi, hipStream
 // The dual half of a PDLP iteration: A x-bar with the fused dual update.
 void launch_dual_half(pm_ctx* c, const EpiDualHal& ed, hipStream_t s) {
     if (c->nblk > 0) launch_blocked(c, c->nblk, c->blk_var, c->xe.get(), ed, s);
+    else if (c->merge_a)
+        launch_merge(c->m, c->nnz, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed, c->mpart_a.get(),
+                     c->mcarry.get(), c->mhead.get(), s);
     else if (c->sell_a.slices > 0) launch_sell_mat(c->sell_a, c->xe.get(), ed, s);
     else launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed, s);
 }
 
 // The primal half of a PDLP iteration: A^T y with the fused primal update.
 void launch_primal_half(pm_ctx* c, const EpiPrimalHal& ep, hipStream_t s) {
-    if (c->sell_at.slices > 0) launch_sell_mat(c->sell_at, c->y.get(), ep, s);
+    if (c->merge_at)
+        launch_merge(c->n, c->nnz, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep, c->mpart_at.get(),
+                     c->mcarry.get(), c->mhead.get(), s);
+    else if (c->sell_at.slices > 0) launch_sell_mat(c->sell_at, c->y.get(), ep, s);
     else launch_rows<float>(c->var_at, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep, s);
 }
 
@@ -500,6 +512,21 @@ float time_sell(pm_ctx* c, const SellMat& S, const float* x, float* y) {
     return ms / 5.0f;
 }
 
+float time_merge(pm_ctx* c, int nrows, const int* rp, const int* ci, const float* v, const float* x,
+                 const int* part, float* y) {
+    EpiStore<float> epi{y};
+    for (int r = 0; r < 2; ++r)
+        launch_merge(nrows, c->nnz, rp, ci, v, x, epi, part, c->mcarry.get(), c->mhead.get(), c->stream);
+    PM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < 5; ++r)
+        launch_merge(nrows, c->nnz, rp, ci, v, x, epi, part, c->mcarry.get(), c->mhead.get(), c->stream);
+    PM_HIP(hipEventRecord(c->ev1, c->stream));
+    PM_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    return ms / 5.0f;
+}
+
 // Builds the column-blocked copy of the scaled A with nblk blocks (the
 // values gathered on the device from va through a permutation).
 void build_blocks(pm_ctx* c, const pm_lp* lp, int nblk) {
@@ -566,7 +593,7 @@ float time_variant(pm_ctx* c, int variant, int nrows, const int* rp, const int* 
 int autotune(pm_ctx* c, int nrows, int ncols, int nnz, const int* rp, const int* ci, const float* v,
              const float* x, float* y) {
     const int ov = c->opts.spmv_variant;
-    if (ov != PM_SPMV_AUTO && ov != PM_SPMV_COLBLOCK && ov != PM_SPMV_SLICED) return ov;
+    if (ov != PM_SPMV_AUTO && ov != PM_SPMV_COLBLOCK && ov != PM_SPMV_SLICED && ov != PM_SPMV_MERGE) return ov;
     if (nnz < (1 << 18) || nrows == 0) return heuristic_variant(nrows, nnz);
     const auto key = std::make_tuple(nrows, ncols, nnz);
     {
@@ -614,9 +641,10 @@ void check_options(const pm_pdhg_options& opts) {
         fail(PM_ERR_INVALID_ARG, "unknown spmv_variant");
     if (opts.spmv_variant == PM_SPMV_STREAM || opts.spmv_variant == PM_SPMV_STAGED)
         fail(PM_ERR_INVALID_ARG, "spmv_variant PM_SPMV_STREAM and PM_SPMV_STAGED are batch-engine kernels");
-    if ((opts.spmv_variant == PM_SPMV_SLICED || opts.spmv_variant == PM_SPMV_COLBLOCK) &&
+    if ((opts.spmv_variant == PM_SPMV_SLICED || opts.spmv_variant == PM_SPMV_COLBLOCK ||
+         opts.spmv_variant == PM_SPMV_MERGE) &&
         opts.algorithm != PM_ALG_PDLP)
-        fail(PM_ERR_INVALID_ARG, "spmv_variant PM_SPMV_SLICED and PM_SPMV_COLBLOCK need PM_ALG_PDLP");
+        fail(PM_ERR_INVALID_ARG, "spmv_variant PM_SPMV_SLICED, PM_SPMV_COLBLOCK and PM_SPMV_MERGE need PM_ALG_PDLP");
     if (opts.algorithm != PM_ALG_REFERENCE && opts.algorithm != PM_ALG_PDLP)
         fail(PM_ERR_INVALID_ARG, "unknown algorithm");
     if (opts.termination != PM_TERM_REFERENCE && opts.termination != PM_TERM_RELATIVE_KKT)
@@ -836,6 +864,47 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
                 if (ts >= tr) c->sell_a.reset();
             }
         }
+        // Merge-path products: timed against what was kept for each product
+        // (or forced by PM_SPMV_MERGE); they even out skewed row lengths.
+        c->merge_a = c->merge_at = false;
+        const bool want_merge = opts.spmv_variant == PM_SPMV_MERGE;
+        if (opts.algorithm == PM_ALG_PDLP && m > 0 && nnz > 0 &&
+            (want_merge || (opts.spmv_variant == PM_SPMV_AUTO && nnz >= (1 << 18)))) {
+            const size_t tiles = static_cast<size_t>(std::max(merge_tiles(m, nnz), merge_tiles(n, nnz)));
+            c->mcarry.resize(tiles);
+            c->mhead.resize(tiles);
+            c->mpart_a.resize(static_cast<size_t>(merge_tiles(m, nnz)) + 1);
+            c->mpart_at.resize(static_cast<size_t>(merge_tiles(n, nnz)) + 1);
+            launch_merge_partition(m, nnz, c->rp.get(), c->mpart_a.get(), s);
+            launch_merge_partition(n, nnz, c->cp.get(), c->mpart_at.get(), s);
+            const float tm_at = time_merge(c, n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(),
+                                           c->mpart_at.get(), c->tmp_n.get());
+            const float tc_at = want_merge ? 1e30f
+                                : c->sell_at.slices > 0 ? time_sell(c, c->sell_at, c->y.get(), c->tmp_n.get())
+                                                        : time_variant(c, c->var_at, n, c->cp.get(), c->ri.get(),
+                                                                       c->vat.get(), c->y.get(), c->tmp_n.get());
+            if (tm_at < tc_at) {
+                c->merge_at = true;
+                c->sell_at.reset();
+            }
+            const float tm_a = time_merge(c, m, c->rp.get(), c->ci.get(), c->va.get(), xs, c->mpart_a.get(),
+                                          c->tmp_m.get());
+            const float tc_a = want_merge ? 1e30f
+                               : t_a >= 0.0f ? t_a
+                                             : time_variant(c, c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), xs,
+                                                            c->tmp_m.get());
+            if (tm_a < tc_a) {
+                c->merge_a = true;
+                c->sell_a.reset();
+                t_a = tm_a;
+            }
+            if (!c->merge_a) c->mpart_a.reset();
+            if (!c->merge_at) c->mpart_at.reset();
+            if (!c->merge_a && !c->merge_at) {
+                c->mcarry.reset();
+                c->mhead.reset();
+            }
+        }
         // Column blocks for the iteration's A x-bar: tried on large LPs,
         // where x-bar outgrows what an XCD's L2 keeps beside the matrix
         // stream (the 500k x 1M LP fetched 4.2x its algorithmic bytes in
@@ -863,6 +932,8 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
                 if (best_nb != c->nblk) build_blocks(c, lp, best_nb);
                 c->blk_var = best_var;
                 c->sell_a.reset();
+                c->merge_a = false;
+                c->mpart_a.reset();
             } else {
                 c->nblk = 0;
                 for (auto* bf : {&c->rpb, &c->cib}) bf->reset();
@@ -1737,8 +1808,8 @@ int pm_ctx_scaling(pm_ctx* c, double* row_scale, double* col_scale) {
 int pm_ctx_spmv_variants(pm_ctx* c, int32_t* va, int32_t* vat) {
     return pm::guarded([&] {
         if (!c || !va || !vat) pm::fail(PM_ERR_INVALID_ARG, "NULL argument");
-        *va = c->nblk > 0 ? PM_SPMV_COLBLOCK : c->sell_a.slices > 0 ? PM_SPMV_SLICED : c->var_a;
-        *vat = c->sell_at.slices > 0 ? PM_SPMV_SLICED : c->var_at;
+        *va = c->nblk > 0 ? PM_SPMV_COLBLOCK : c->merge_a ? PM_SPMV_MERGE : c->sell_a.slices > 0 ? PM_SPMV_SLICED : c->var_a;
+        *vat = c->merge_at ? PM_SPMV_MERGE : c->sell_at.slices > 0 ? PM_SPMV_SLICED : c->var_at;
     });
 }
 

@@ -477,6 +477,137 @@ __device__ __forceinline__ float to_f32_bound(double v) {
 // control.hpp): the LP with cost 0, or with every finite bound at 0.
 enum RoundMode { kRoundLp = 0, kRoundPrimalFeas = 1, kRoundDualFeas = 2 };
 
+// ---------------------------------------------------------------------------
+// Merge-path CSR product (after Merrill and Garland's merge-based SpMV).
+// The walk that interleaves the row ends rp[1..nrows] with the nonzero
+// indices 0..nnz-1 has nrows + nnz steps (a step either consumes a nonzero
+// or ends a row); it is cut into tiles of kMergeTile steps, one workgroup
+// each, so every workgroup does the same work however long the rows are.
+//  * k_merge_partition (once per matrix): prow[b] = rows ended before step
+//    b * kMergeTile, by binary search on the walk; the tile then starts at
+//    nonzero b * kMergeTile - prow[b].
+//  * k_merge_tile: the workgroup stages its nonzeros' products a_k x_ci[k]
+//    and its row ends in LDS with coalesced loads; each thread walks
+//    kMergeItems steps of the tile (its start found by binary search in
+//    LDS), a segmented scan over the threads (keyed by the row each thread
+//    stops inside) hands every thread the part of its first row that the
+//    threads before it summed, and the rows that end in the tile go through
+//    the epilogue in one coalesced pass.  The row the tile stops inside
+//    leaves its part in carry[b]; the first row, when it started in an
+//    earlier tile, leaves its part in head[b] instead of being finished.
+//  * k_merge_fix: one wave per tile whose first row came in from earlier
+//    tiles adds that row's carries (the tiles whose end lies inside it) to
+//    head[b] and applies the epilogue.
+// Fixed summation order, no atomics: the results repeat bit for bit.
+constexpr int kMergeItems = 8;
+constexpr int kMergeTile = kBlock * kMergeItems;
+constexpr int kWave = 64;
+
+template <int TILE>
+__global__ __launch_bounds__(kBlock) void k_merge_partition(int nrows, int nnz, int ntiles,
+                                                            const int* __restrict__ rp, int* __restrict__ prow) {
+    const int b = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+    if (b > ntiles) return;
+    const int64_t total = static_cast<int64_t>(nrows) + nnz;
+    const int64_t d = static_cast<int64_t>(b) * TILE < total ? static_cast<int64_t>(b) * TILE : total;
+    // rows ended before step d: the first r with rp[r + 1] > d - r - 1
+    int64_t lo = d - nnz > 0 ? d - nnz : 0, hi = d < nrows ? d : nrows;
+    while (lo < hi) {
+        const int64_t piv = (lo + hi) >> 1;
+        if (rp[piv + 1] <= d - piv - 1) lo = piv + 1;
+        else hi = piv;
+    }
+    prow[b] = static_cast<int>(lo);
+}
+
+template <class Epi>
+__global__ __launch_bounds__(kBlock) void k_merge_tile(int nrows, int nnz, const int* __restrict__ rp,
+                                                       const int* __restrict__ ci, const float* __restrict__ v,
+                                                       const float* __restrict__ x, const int* __restrict__ prow,
+                                                       float* __restrict__ carry, float* __restrict__ head, Epi epi) {
+    __shared__ float s_prod[kMergeTile];
+    __shared__ int s_end[kMergeTile + 1];
+    __shared__ float s_sum[kMergeTile];
+    __shared__ float s_cv[kBlock];
+    __shared__ int s_cr[kBlock];
+    const int b = static_cast<int>(blockIdx.x), tid = static_cast<int>(threadIdx.x);
+    const int64_t total = static_cast<int64_t>(nrows) + nnz;
+    const int64_t d0 = static_cast<int64_t>(b) * kMergeTile;
+    const int64_t d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
+    const int r0 = prow[b], r1 = prow[b + 1];
+    const int k0 = static_cast<int>(d0 - r0), k1 = static_cast<int>(d1 - r1);
+    const int nr = r1 - r0, nk = k1 - k0;  // rows ended in the tile, nonzeros consumed
+    for (int j = tid; j < nk; j += kBlock) s_prod[j] = v[k0 + j] * x[ci[k0 + j]];
+    for (int j = tid; j < nr; j += kBlock) s_end[j] = rp[r0 + 1 + j] - k0;
+    if (tid == 0) s_end[nr] = 0x7fffffff;  // the row the tile stops inside does not end here
+    __syncthreads();
+    const int len = nr + nk;
+    const int dl = tid * kMergeItems < len ? tid * kMergeItems : len;
+    const int de = dl + kMergeItems < len ? dl + kMergeItems : len;
+    int lo = dl - nk > 0 ? dl - nk : 0, hi = dl < nr ? dl : nr;
+    while (lo < hi) {
+        const int piv = (lo + hi) >> 1;
+        if (s_end[piv] <= dl - piv - 1) lo = piv + 1;
+        else hi = piv;
+    }
+    const int first = lo;
+    int row = lo, k = dl - lo;
+    float s = 0.0f;
+    for (int d = dl; d < de; ++d) {
+        if (k < s_end[row]) {
+            s += s_prod[k];
+            ++k;
+        } else {
+            s_sum[row] = s;
+            s = 0.0f;
+            ++row;
+        }
+    }
+    s_cv[tid] = s;
+    s_cr[tid] = row;
+    __syncthreads();
+    // inclusive scan of the carries, segmented by the row they belong to
+    // (threads stopping inside one row are consecutive)
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const float add = tid >= off && s_cr[tid - off] == s_cr[tid] ? s_cv[tid - off] : 0.0f;
+        __syncthreads();
+        s_cv[tid] += add;
+        __syncthreads();
+    }
+    if (row > first && tid > 0 && s_cr[tid - 1] == first) s_sum[first] += s_cv[tid - 1];
+    __syncthreads();
+    const bool head_in = nr > 0 && k0 > rp[r0];  // the first row started in an earlier tile
+    for (int j = tid; j < nr; j += kBlock) {
+        if (j == 0 && head_in) head[b] = s_sum[0];
+        else epi(r0 + j, s_sum[j]);
+    }
+    if (tid == kBlock - 1) carry[b] = s_cv[kBlock - 1];
+}
+
+template <class Epi>
+__global__ __launch_bounds__(kBlock) void k_merge_fix(int ntiles, const int* __restrict__ rp,
+                                                      const int* __restrict__ prow, const float* __restrict__ carry,
+                                                      const float* __restrict__ head, Epi epi) {
+    const int b = static_cast<int>((blockIdx.x * kBlock + threadIdx.x) / kWave);
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    if (b >= ntiles) return;
+    const int r0 = prow[b];
+    if (prow[b + 1] == r0) return;  // no row ends in tile b
+    const int64_t k0 = static_cast<int64_t>(b) * kMergeTile - r0;
+    if (k0 <= rp[r0]) return;  // tile b's first row started in it
+    // tiles bb < b whose end lies inside row r0: prow[bb + 1] == r0, the run
+    // just before b
+    float s = 0.0f;
+    for (int base = b - 1; base >= 0; base -= kWave) {
+        const int bb = base - lane;
+        const bool in = bb >= 0 && prow[bb + 1] == r0;
+        if (in) s += carry[bb];
+        if (!__all(in)) break;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) epi(r0, head[b] + s);
+}
+
 // Correction-LP column data of a refinement round from FP64 A^T y_base:
 // c_hat = D_c DD (c + A^T y_base), bounds DP (l - x_base) / D_c (cost 0 in
 // kRoundPrimalFeas, finite bounds 0 in kRoundDualFeas, whose x_base is 0).

@@ -63,6 +63,17 @@ void run_spmv(const int32_t* rp, const int32_t* ci, const float* v, int32_t nnz,
     d_x.upload(x, cols, s);
     if (cols == 0) d_x.resize(1);
     const int var = pick_variant(variant, rows, nnz);
+    if (var == PM_SPMV_MERGE) {
+        const size_t nt = static_cast<size_t>(merge_tiles(rows, nnz));
+        DevBuf<int> prow(nt + 1);
+        DevBuf<float> carry(nt), head(nt);
+        launch_merge_partition(rows, nnz, d_rp.get(), prow.get(), s);
+        launch_merge(rows, nnz, d_rp.get(), d_ci.get(), d_v.get(), d_x.get(), EpiStore<float>{d_y.get()}, prow.get(),
+                     carry.get(), head.get(), s);
+        d_y.download(y, rows, s);
+        PM_HIP(hipStreamSynchronize(s));
+        return;
+    }
     launch_rows<float>(var, rows, d_rp.get(), d_ci.get(), d_v.get(), d_x.get(), EpiStore<float>{d_y.get()}, s);
     d_y.download(y, rows, s);
     PM_HIP(hipStreamSynchronize(s));

@@ -5,7 +5,7 @@ reference's own test and starts only from its own point
 warm start is an argument error, as are unknown algorithms and
 terminations, the batch engine's LDS kernels (PM_SPMV_STREAM,
 PM_SPMV_STAGED) and the PDLP-only layouts (PM_SPMV_SLICED,
-PM_SPMV_COLBLOCK) under the reference algorithm.  Valid pairs get past the checks (on a machine without a GPU
+PM_SPMV_COLBLOCK, PM_SPMV_MERGE) under the reference algorithm.  Valid pairs get past the checks (on a machine without a GPU
 they then fail for want of a device, not of arguments)."""
 import numpy as np
 import pytest
@@ -28,6 +28,7 @@ def tiny(capi):
     (dict(algorithm="pdlp", spmv_variant="staged"), "batch-engine kernels"),
     (dict(algorithm="reference", spmv_variant="sliced"), "need PM_ALG_PDLP"),
     (dict(algorithm="reference", spmv_variant="colblock"), "need PM_ALG_PDLP"),
+    (dict(algorithm="reference", spmv_variant="merge"), "need PM_ALG_PDLP"),
 ])
 def test_rejected_pairs(capi, kw, why):
     with pytest.raises(capi.PramaanError) as e:
@@ -42,6 +43,7 @@ def test_rejected_pairs(capi, kw, why):
     dict(algorithm="pdlp", x0=np.ones(2), y0=np.zeros(1)),
     dict(algorithm="pdlp", spmv_variant="sliced"),
     dict(algorithm="pdlp", spmv_variant="colblock"),
+    dict(algorithm="pdlp", spmv_variant="merge"),
 ])
 def test_accepted_pairs_pass_the_checks(capi, kw):
     if capi.device_count() > 0:

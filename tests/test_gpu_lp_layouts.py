@@ -4,7 +4,9 @@
     sums accumulated across blocks (A^T y autotuned);
   * PM_SPMV_SLICED: both products over sliced ELL copies of A and A^T
     (64-row slices, rows sorted by length within windows of 1,024 rows,
-    padding lanes and padding entries).
+    padding lanes and padding entries);
+  * PM_SPMV_MERGE: both products merge-path (each thread 8 steps of the
+    rows + nonzeros path, split rows finished by a second pass).
 
 Forced on the 50,000 x 100,000 LP of the single-large-LP generator, each
 solve must reach the same optimum as the plain row kernels: the oracle's
@@ -13,8 +15,9 @@ objective within the tolerance x (1 + |p*|) of the closed form (the
 relative gap bounds the objective error at that scale).  The layouts add a row's entries in a
 different order than a row walk, so the trajectories differ in rounding
 only.  On the Netlib files (27 and 56 rows: a single partial slice, rows of
-1 to 20+ entries, padding everywhere) the sliced solve must match the
-reference's published optimum."""
+1 to 20+ entries, padding everywhere; for merge-path, rows split across
+threads) the sliced and merge-path solves must match the reference's
+published optimum."""
 import json
 from pathlib import Path
 
@@ -25,7 +28,7 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
-@pytest.mark.parametrize("layout", ["colblock", "sliced"])
+@pytest.mark.parametrize("layout", ["colblock", "sliced", "merge"])
 def test_layout_solve_matches_plain(capi, oracle, layout):
     lp = capi.generate(2, seed=1)
     tol = 1e-5
@@ -34,7 +37,7 @@ def test_layout_solve_matches_plain(capi, oracle, layout):
         ctx.load(lp, spmv_variant=layout, **kw)
         va, vat = ctx.spmv_variants()
         assert va == layout
-        assert vat == "sliced" if layout == "sliced" else vat != "colblock"
+        assert vat == layout if layout != "colblock" else vat != "colblock"
         rb = ctx.solve()
     rp = capi.solve(lp, spmv_variant="vector8", **kw)
     print(f"{layout}: {rb.status} it={rb.iterations} obj={rb.objective:.10g}; vector8: {rp.status} it={rp.iterations} "
@@ -45,7 +48,7 @@ def test_layout_solve_matches_plain(capi, oracle, layout):
         assert oracle.relative_kkt(lp, r.x, r.y)["rel"] <= 1.2 * tol
 
 
-@pytest.mark.parametrize("layout", ["colblock", "sliced"])
+@pytest.mark.parametrize("layout", ["colblock", "sliced", "merge"])
 def test_layout_full_size_tracks_row_kernels(capi, layout):
     """The 500,000 x 1,000,000 LP (12M nonzeros, x-bar 4 MB): 640 iterations
     with the layout forced and with the 8-lane row kernel forced walk the
@@ -70,13 +73,65 @@ def test_layout_full_size_tracks_row_kernels(capi, layout):
     assert abs(a.rel_kkt - b.rel_kkt) <= 1e-4 * b.rel_kkt
 
 
+@pytest.mark.parametrize("layout", ["sliced", "merge"])
 @pytest.mark.parametrize("name", ["afiro", "adlittle"])
-def test_sliced_netlib(capi, name):
+def test_layout_netlib(capi, name, layout):
     from pramaan_amd.mps import read_mps
     lp = read_mps(GOLDEN / f"{name}.mps")
     r = capi.solve(lp, algorithm="pdlp", termination="relative_kkt", tolerance=1e-8, max_iterations=400000,
-                   check_frequency=64, spmv_variant="sliced")
-    print(f"{name} sliced: {r.status} it={r.iterations} obj={r.objective:.12g}", flush=True)
+                   check_frequency=64, spmv_variant=layout)
+    print(f"{name} {layout}: {r.status} it={r.iterations} obj={r.objective:.12g}", flush=True)
     assert r.status == "OPTIMAL"
     want = json.loads((GOLDEN / "fixtures.json").read_text())["netlib"][name]
     assert abs(r.objective - want) <= 1e-6 * (1 + abs(want))
+
+
+def test_batch_rejects_single_lp_layouts(capi):
+    """Merge-path and column blocks are layouts of the single-LP engine; the
+    batch engine refuses them instead of silently running another kernel."""
+    lp = capi.generate(0, seed=1)
+    for layout in ("merge", "colblock"):
+        with pytest.raises(capi.PramaanError) as e:
+            capi.Batch([lp], tolerance=1e-6, spmv_variant=layout)
+        assert "single-LP layouts" in str(e.value)
+
+
+def with_dense_rows(capi, lp, k):
+    """lp with k rows over every column (sum of x <= 2 sum x* + 1): never
+    binding at the optimum, so x*, y* (with zero duals on the new rows) and
+    p* stay those of lp."""
+    import numpy as np
+    n = lp.n
+    rp = np.concatenate([lp.row_ptr, lp.row_ptr[-1] + n * np.arange(1, k + 1)])
+    ci = np.concatenate([lp.col_idx, np.tile(np.arange(n), k)])
+    v = np.concatenate([lp.values, np.ones(n * k)])
+    cap = 2.0 * float(np.sum(lp.x_star)) + 1.0
+    return capi.LP(lp.c, rp, ci, v, np.concatenate([lp.row_lower, np.full(k, -1e30)]),
+                   np.concatenate([lp.row_upper, np.full(k, cap)]), lp.var_lower, lp.var_upper, offset=lp.offset,
+                   p_star=lp.p_star)
+
+
+def test_merge_dense_rows(capi):
+    """The 50,000 x 100,000 LP with 16 dense rows (100,000 entries each among
+    rows of about 24): row kernels leave one wave per dense row, merge-path
+    splits them over tiles.  AUTO must keep merge-path for A x-bar (measured
+    16 us against 194 us for the best row kernel), and 640 iterations with
+    merge-path and with the 8-lane row kernel forced must walk the same
+    trajectory up to summation order (bounds as for the 500,000 x 1,000,000
+    LP above)."""
+    lp = with_dense_rows(capi, capi.generate(2, seed=1), 16)
+    kw = dict(algorithm="pdlp", termination="relative_kkt", tolerance=1e-9, max_iterations=640, check_frequency=64)
+    with capi.Context(0) as ctx:
+        ctx.load(lp, **kw)
+        assert ctx.spmv_variants()[0] == "merge"
+    out = {}
+    for var in ("merge", "vector8"):
+        with capi.Context(0) as ctx:
+            ctx.load(lp, spmv_variant=var, **kw)
+            out[var] = ctx.solve(want_vectors=False)
+    a, b = out["merge"], out["vector8"]
+    print(f"merge obj={a.objective:.12g} kkt={a.rel_kkt:.6e}; vector8 obj={b.objective:.12g} kkt={b.rel_kkt:.6e}",
+          flush=True)
+    assert a.status == b.status == "ITERATION_LIMIT"
+    assert abs(a.objective - b.objective) <= 2e-5 * (1 + abs(b.objective))
+    assert abs(a.rel_kkt - b.rel_kkt) <= 1e-4 * b.rel_kkt
