@@ -33,7 +33,9 @@ tolerance and their iteration counts are reported beside it, and a
 single-thread run of one window.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-       python bench.py --workload lp1        (the single 500k x 1M LP instead)
+       python bench.py --workload lp1        (the single 500k x 1M LP instead; --lp tenth for
+                                              50k x 100k, --dense-rows K appends K rows over
+                                              every column)
        python bench.py --workload batch4096  (4,096 generated LPs in one batch)
        python bench.py --workload ruiz       (Ruiz scaling of the 500k x 1M LP)
        python bench.py --workload train      (network training on 20,000 windows)
@@ -376,7 +378,9 @@ def bench_batch(args, P) -> dict:
 
 
 def bench_lp1(args, P) -> dict:
-    lp = P.generate(1, seed=1)
+    lp = P.generate(1 if args.lp == "full" else 2, seed=1)
+    if args.dense_rows > 0:
+        lp = P.with_dense_rows(lp, args.dense_rows)
     ctx = P.Context(0)
     t0 = time.perf_counter()
     ctx.load(lp, algorithm="pdlp", termination="relative_kkt", tolerance=args.tol, max_iterations=args.max_iters,
@@ -610,6 +614,10 @@ def main() -> int:
     ap.add_argument("--max-iters", type=int, default=200000)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--spmv", default="auto", help="A x-bar / A^T y kernel of the lp1 workload (a name of _capi.SPMV)")
+    ap.add_argument("--lp", choices=["full", "tenth"], default="full",
+                    help="lp1 workload: the 500k x 1M LP or its 50k x 100k tenth")
+    ap.add_argument("--dense-rows", type=int, default=0,
+                    help="lp1 workload: rows over every column appended (skewed row lengths)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lp1", action="store_true", help="skip the 500k x 1M SpMV measurement")

@@ -761,6 +761,21 @@ class RollingHorizon:
             pass
 
 
+def with_dense_rows(lp: LP, k: int) -> LP:
+    """lp with k rows over every column appended (sum of x <= 2 sum x* + 1):
+    rows of n entries among rows of a few dozen, never binding at the
+    optimum, so x*, y* (zero duals on the new rows) and p* stay those of lp."""
+    n = lp.n
+    rp = np.concatenate([lp.row_ptr, lp.row_ptr[-1] + n * np.arange(1, k + 1)])
+    ci = np.concatenate([lp.col_idx, np.tile(np.arange(n), k)])
+    v = np.concatenate([lp.values, np.ones(n * k)])
+    cap = 2.0 * float(np.sum(lp.x_star)) + 1.0
+    y_star = None if lp.y_star is None else np.concatenate([lp.y_star, np.zeros(k)])
+    return LP(lp.c, rp, ci, v, np.concatenate([lp.row_lower, np.full(k, -INF)]),
+              np.concatenate([lp.row_upper, np.full(k, cap)]), lp.var_lower, lp.var_upper, lp.sense, lp.offset,
+              lp.x_star, y_star, lp.p_star)
+
+
 def generate(which: int | None = None, seed: int = 1, **params) -> LP:
     """Synthetic LP with known optimum (pramaan_gen.h).  `which` names a
     workload (0: 2k x 5k, 1: 500k x 1M, 2: 50k x 100k, 3: batch instance);

@@ -96,21 +96,6 @@ def test_batch_rejects_single_lp_layouts(capi):
         assert "single-LP layouts" in str(e.value)
 
 
-def with_dense_rows(capi, lp, k):
-    """lp with k rows over every column (sum of x <= 2 sum x* + 1): never
-    binding at the optimum, so x*, y* (with zero duals on the new rows) and
-    p* stay those of lp."""
-    import numpy as np
-    n = lp.n
-    rp = np.concatenate([lp.row_ptr, lp.row_ptr[-1] + n * np.arange(1, k + 1)])
-    ci = np.concatenate([lp.col_idx, np.tile(np.arange(n), k)])
-    v = np.concatenate([lp.values, np.ones(n * k)])
-    cap = 2.0 * float(np.sum(lp.x_star)) + 1.0
-    return capi.LP(lp.c, rp, ci, v, np.concatenate([lp.row_lower, np.full(k, -1e30)]),
-                   np.concatenate([lp.row_upper, np.full(k, cap)]), lp.var_lower, lp.var_upper, offset=lp.offset,
-                   p_star=lp.p_star)
-
-
 def test_merge_dense_rows(capi):
     """The 50,000 x 100,000 LP with 16 dense rows (100,000 entries each among
     rows of about 24): row kernels leave one wave per dense row, merge-path
@@ -119,7 +104,7 @@ def test_merge_dense_rows(capi):
     merge-path and with the 8-lane row kernel forced must walk the same
     trajectory up to summation order (bounds as for the 500,000 x 1,000,000
     LP above)."""
-    lp = with_dense_rows(capi, capi.generate(2, seed=1), 16)
+    lp = capi.with_dense_rows(capi.generate(2, seed=1), 16)
     kw = dict(algorithm="pdlp", termination="relative_kkt", tolerance=1e-9, max_iterations=640, check_frequency=64)
     with capi.Context(0) as ctx:
         ctx.load(lp, **kw)
