@@ -117,6 +117,20 @@ inline int merge_tiles(int nrows, int nnz) {
     return static_cast<int>((static_cast<int64_t>(nrows) + nnz + kMergeTile - 1) / kMergeTile);
 }
 
+// Rows of a merge-path product that k_merge_fix finishes, from the host row
+// pointers: more than kMergeShort entries and a tile edge inside the row
+// (between its first nonzero's step rp[r] + r and its end step rp[r+1] + r).
+inline std::vector<int4> merge_long_rows(int nrows, const int* rp) {
+    std::vector<int4> out;
+    for (int r = 0; r < nrows; ++r) {
+        if (rp[r + 1] - rp[r] <= kMergeShort) continue;
+        const int b0 = static_cast<int>((static_cast<int64_t>(rp[r]) + r) / kMergeTile);
+        const int b1 = static_cast<int>((static_cast<int64_t>(rp[r + 1]) + r) / kMergeTile);
+        if (b1 > b0) out.push_back(make_int4(r, b0, b1, 0));
+    }
+    return out;
+}
+
 // Tile starts of a merge-path product (prow: merge_tiles + 1 ints), once
 // per matrix.
 inline void launch_merge_partition(int nrows, int nnz, const int* rp, int* prow, hipStream_t s) {
@@ -126,19 +140,20 @@ inline void launch_merge_partition(int nrows, int nnz, const int* rp, int* prow,
     PM_LAUNCHED();
 }
 
-// Merge-path product with a fused epilogue (k_merge_tile, k_merge_fix) over
-// a partition from launch_merge_partition; carry and head hold
-// merge_tiles(nrows, nnz) floats.
+// Merge-path product with a fused epilogue (k_merge_tile, then k_merge_fix
+// when the matrix has long rows crossing tile edges) over a partition from
+// launch_merge_partition and the list from merge_long_rows (nlong entries
+// on the device); carry and head hold merge_tiles(nrows, nnz) floats.
 template <class Epi>
 void launch_merge(int nrows, int nnz, const int* rp, const int* ci, const float* v, const float* x, const Epi& epi,
-                  const int* prow, float* carry, float* head, hipStream_t s) {
+                  const int* prow, const int4* longs, int nlong, float* carry, float* head, hipStream_t s) {
     if (nrows <= 0) return;
     const int nt = merge_tiles(nrows, nnz);
     hipLaunchKernelGGL((k_merge_tile<Epi>), dim3(nt), dim3(kBlock), 0, s, nrows, nnz, rp, ci, v, x, prow, carry, head,
                        epi);
-    if (nt > 1)
-        hipLaunchKernelGGL((k_merge_fix<Epi>), dim3((nt + kBlock / kWave - 1) / (kBlock / kWave)), dim3(kBlock), 0, s,
-                           nt, rp, prow, carry, head, epi);
+    if (nlong > 0)
+        hipLaunchKernelGGL((k_merge_fix<Epi>), dim3((nlong + kBlock / kWave - 1) / (kBlock / kWave)), dim3(kBlock), 0,
+                           s, nlong, longs, carry, head, epi);
     PM_LAUNCHED();
 }
 

@@ -334,6 +334,8 @@ struct pm_ctx {
     // partitions of A and A^T, per-tile carries and head parts.
     bool merge_a = false, merge_at = false;
     DevBuf<int> mpart_a, mpart_at;
+    DevBuf<int4> mlong_a, mlong_at;  // long rows crossing tile edges
+    int nlong_a = 0, nlong_at = 0;
     DevBuf<float> mcarry, mhead;
 
     const void* orig_a() const { return exact_f32 ? static_cast<const void*>(oa32.get()) : oa64.get(); }
@@ -413,7 +415,7 @@ void launch_dual_half(pm_ctx* c, const EpiDualHal& ed, hipStream_t s) {
     if (c->nblk > 0) launch_blocked(c, c->nblk, c->blk_var, c->xe.get(), ed, s);
     else if (c->merge_a)
         launch_merge(c->m, c->nnz, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed, c->mpart_a.get(),
-                     c->mcarry.get(), c->mhead.get(), s);
+                     c->mlong_a.get(), c->nlong_a, c->mcarry.get(), c->mhead.get(), s);
     else if (c->sell_a.slices > 0) launch_sell_mat(c->sell_a, c->xe.get(), ed, s);
     else launch_rows<float>(c->var_a, c->m, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed, s);
 }
@@ -422,7 +424,7 @@ void launch_dual_half(pm_ctx* c, const EpiDualHal& ed, hipStream_t s) {
 void launch_primal_half(pm_ctx* c, const EpiPrimalHal& ep, hipStream_t s) {
     if (c->merge_at)
         launch_merge(c->n, c->nnz, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep, c->mpart_at.get(),
-                     c->mcarry.get(), c->mhead.get(), s);
+                     c->mlong_at.get(), c->nlong_at, c->mcarry.get(), c->mhead.get(), s);
     else if (c->sell_at.slices > 0) launch_sell_mat(c->sell_at, c->y.get(), ep, s);
     else launch_rows<float>(c->var_at, c->n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep, s);
 }
@@ -513,13 +515,13 @@ float time_sell(pm_ctx* c, const SellMat& S, const float* x, float* y) {
 }
 
 float time_merge(pm_ctx* c, int nrows, const int* rp, const int* ci, const float* v, const float* x,
-                 const int* part, float* y) {
+                 const int* part, const int4* longs, int nlong, float* y) {
     EpiStore<float> epi{y};
     for (int r = 0; r < 2; ++r)
-        launch_merge(nrows, c->nnz, rp, ci, v, x, epi, part, c->mcarry.get(), c->mhead.get(), c->stream);
+        launch_merge(nrows, c->nnz, rp, ci, v, x, epi, part, longs, nlong, c->mcarry.get(), c->mhead.get(), c->stream);
     PM_HIP(hipEventRecord(c->ev0, c->stream));
     for (int r = 0; r < 5; ++r)
-        launch_merge(nrows, c->nnz, rp, ci, v, x, epi, part, c->mcarry.get(), c->mhead.get(), c->stream);
+        launch_merge(nrows, c->nnz, rp, ci, v, x, epi, part, longs, nlong, c->mcarry.get(), c->mhead.get(), c->stream);
     PM_HIP(hipEventRecord(c->ev1, c->stream));
     PM_HIP(hipEventSynchronize(c->ev1));
     float ms = 0.0f;
@@ -877,8 +879,13 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
             c->mpart_at.resize(static_cast<size_t>(merge_tiles(n, nnz)) + 1);
             launch_merge_partition(m, nnz, c->rp.get(), c->mpart_a.get(), s);
             launch_merge_partition(n, nnz, c->cp.get(), c->mpart_at.get(), s);
+            const std::vector<int4> la = merge_long_rows(m, lp->row_ptr), lat = merge_long_rows(n, cp.data());
+            c->nlong_a = static_cast<int>(la.size());
+            c->nlong_at = static_cast<int>(lat.size());
+            c->mlong_a.upload(la, s);
+            c->mlong_at.upload(lat, s);
             const float tm_at = time_merge(c, n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(),
-                                           c->mpart_at.get(), c->tmp_n.get());
+                                           c->mpart_at.get(), c->mlong_at.get(), c->nlong_at, c->tmp_n.get());
             const float tc_at = want_merge ? 1e30f
                                 : c->sell_at.slices > 0 ? time_sell(c, c->sell_at, c->y.get(), c->tmp_n.get())
                                                         : time_variant(c, c->var_at, n, c->cp.get(), c->ri.get(),
@@ -888,7 +895,7 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
                 c->sell_at.reset();
             }
             const float tm_a = time_merge(c, m, c->rp.get(), c->ci.get(), c->va.get(), xs, c->mpart_a.get(),
-                                          c->tmp_m.get());
+                                          c->mlong_a.get(), c->nlong_a, c->tmp_m.get());
             const float tc_a = want_merge ? 1e30f
                                : t_a >= 0.0f ? t_a
                                              : time_variant(c, c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), xs,

@@ -489,18 +489,22 @@ enum RoundMode { kRoundLp = 0, kRoundPrimalFeas = 1, kRoundDualFeas = 2 };
 //  * k_merge_tile: the workgroup stages its nonzeros' products a_k x_ci[k]
 //    and its row ends in LDS with coalesced loads; each thread walks
 //    kMergeItems steps of the tile (its start found by binary search in
-//    LDS), a segmented scan over the threads (keyed by the row each thread
-//    stops inside) hands every thread the part of its first row that the
+//    LDS); a segmented scan over the threads (keyed by the row each thread
+//    stops inside: within each wavefront by shuffles, then across the
+//    wavefronts) hands every thread the part of its first row that the
 //    threads before it summed, and the rows that end in the tile go through
-//    the epilogue in one coalesced pass.  The row the tile stops inside
-//    leaves its part in carry[b]; the first row, when it started in an
-//    earlier tile, leaves its part in head[b] instead of being finished.
-//  * k_merge_fix: one wave per tile whose first row came in from earlier
-//    tiles adds that row's carries (the tiles whose end lies inside it) to
-//    head[b] and applies the epilogue.
+//    the epilogue in one coalesced pass.
+//  * A row of at most kMergeShort entries crosses at most one tile edge; the
+//    tile it starts in finishes it (one wavefront sums the entries past the
+//    edge), and the next tile skips it.  A longer row that crosses edges
+//    leaves its part of each tile it runs through in carry[b] and its part
+//    of the tile it ends in in head[b]; k_merge_fix takes the list of such
+//    rows (built once per matrix), one wavefront per row, adds the parts in
+//    tile order and applies the epilogue.
 // Fixed summation order, no atomics: the results repeat bit for bit.
 constexpr int kMergeItems = 8;
 constexpr int kMergeTile = kBlock * kMergeItems;
+constexpr int kMergeShort = 128;
 constexpr int kWave = 64;
 
 template <int TILE>
@@ -530,7 +534,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_tile(int nrows, int nnz, const
     __shared__ float s_sum[kMergeTile];
     __shared__ float s_cv[kBlock];
     __shared__ int s_cr[kBlock];
+    __shared__ float s_wv[kBlock / kWave];
+    __shared__ int s_wr[kBlock / kWave], s_wf[kBlock / kWave];
     const int b = static_cast<int>(blockIdx.x), tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & (kWave - 1), w = tid / kWave;
     const int64_t total = static_cast<int64_t>(nrows) + nnz;
     const int64_t d0 = static_cast<int64_t>(b) * kMergeTile;
     const int64_t d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
@@ -563,49 +570,72 @@ __global__ __launch_bounds__(kBlock) void k_merge_tile(int nrows, int nnz, const
             ++row;
         }
     }
-    s_cv[tid] = s;
+    // inclusive scan of the carries, segmented by the row they belong to
+    // (threads stopping inside one row are consecutive): within the
+    // wavefront, then the runs that reach back into earlier wavefronts
+    float cv = s;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const float vu = __shfl_up(cv, off);
+        const int ru = __shfl_up(row, off);
+        if (lane >= off && ru == row) cv += vu;
+    }
+    const int row_l0 = __shfl(row, 0);
+    if (lane == kWave - 1) {
+        s_wv[w] = cv;
+        s_wr[w] = row;
+        s_wf[w] = row_l0;
+    }
+    __syncthreads();
+    if (row == row_l0) {  // this thread's run starts at the wavefront's first lane
+        for (int w2 = w - 1; w2 >= 0 && s_wr[w2] == row; --w2) {
+            cv += s_wv[w2];
+            if (s_wf[w2] != row) break;
+        }
+    }
+    s_cv[tid] = cv;
     s_cr[tid] = row;
     __syncthreads();
-    // inclusive scan of the carries, segmented by the row they belong to
-    // (threads stopping inside one row are consecutive)
-    for (int off = 1; off < kBlock; off <<= 1) {
-        const float add = tid >= off && s_cr[tid - off] == s_cr[tid] ? s_cv[tid - off] : 0.0f;
-        __syncthreads();
-        s_cv[tid] += add;
-        __syncthreads();
-    }
     if (row > first && tid > 0 && s_cr[tid - 1] == first) s_sum[first] += s_cv[tid - 1];
     __syncthreads();
-    const bool head_in = nr > 0 && k0 > rp[r0];  // the first row started in an earlier tile
+    // the first row, when it came in from the previous tile: a short one was
+    // finished there, a long one leaves its part here for k_merge_fix
+    const bool head_in = nr > 0 && k0 > rp[r0];
+    const bool head_long = head_in && rp[r0 + 1] - rp[r0] > kMergeShort;
     for (int j = tid; j < nr; j += kBlock) {
-        if (j == 0 && head_in) head[b] = s_sum[0];
-        else epi(r0 + j, s_sum[j]);
+        if (j == 0 && head_in) {
+            if (head_long) head[b] = s_sum[0];
+        } else {
+            epi(r0 + j, s_sum[j]);
+        }
     }
-    if (tid == kBlock - 1) carry[b] = s_cv[kBlock - 1];
+    // the row the tile stops inside
+    if (r1 < nrows && k1 > rp[r1]) {
+        const int kend = rp[r1 + 1];
+        if (kend - rp[r1] > kMergeShort) {
+            if (tid == kBlock - 1) carry[b] = s_cv[kBlock - 1];
+        } else if (w == 0) {
+            float e = 0.0f;
+            for (int q = k1 + lane; q < kend; q += kWave) e += v[q] * x[ci[q]];
+            for (int off = kWave / 2; off > 0; off >>= 1) e += __shfl_xor(e, off);
+            if (lane == 0) epi(r1, s_cv[kBlock - 1] + e);
+        }
+    }
 }
 
+// Rows of more than kMergeShort entries that cross tile edges: {row, first
+// tile ending inside it, tile it ends in}.  One wavefront per row.
 template <class Epi>
-__global__ __launch_bounds__(kBlock) void k_merge_fix(int ntiles, const int* __restrict__ rp,
-                                                      const int* __restrict__ prow, const float* __restrict__ carry,
+__global__ __launch_bounds__(kBlock) void k_merge_fix(int nlong, const int4* __restrict__ longs,
+                                                      const float* __restrict__ carry,
                                                       const float* __restrict__ head, Epi epi) {
-    const int b = static_cast<int>((blockIdx.x * kBlock + threadIdx.x) / kWave);
+    const int q = static_cast<int>((blockIdx.x * kBlock + threadIdx.x) / kWave);
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    if (b >= ntiles) return;
-    const int r0 = prow[b];
-    if (prow[b + 1] == r0) return;  // no row ends in tile b
-    const int64_t k0 = static_cast<int64_t>(b) * kMergeTile - r0;
-    if (k0 <= rp[r0]) return;  // tile b's first row started in it
-    // tiles bb < b whose end lies inside row r0: prow[bb + 1] == r0, the run
-    // just before b
+    if (q >= nlong) return;
+    const int4 L = longs[q];
     float s = 0.0f;
-    for (int base = b - 1; base >= 0; base -= kWave) {
-        const int bb = base - lane;
-        const bool in = bb >= 0 && prow[bb + 1] == r0;
-        if (in) s += carry[bb];
-        if (!__all(in)) break;
-    }
+    for (int bb = L.y + lane; bb < L.z; bb += kWave) s += carry[bb];
     for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) epi(r0, head[b] + s);
+    if (lane == 0) epi(L.x, s + head[L.z]);
 }
 
 // Correction-LP column data of a refinement round from FP64 A^T y_base:

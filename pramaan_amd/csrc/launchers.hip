@@ -67,9 +67,12 @@ void run_spmv(const int32_t* rp, const int32_t* ci, const float* v, int32_t nnz,
         const size_t nt = static_cast<size_t>(merge_tiles(rows, nnz));
         DevBuf<int> prow(nt + 1);
         DevBuf<float> carry(nt), head(nt);
+        const std::vector<int4> longs = merge_long_rows(rows, rp);
+        DevBuf<int4> d_longs;
+        d_longs.upload(longs, s);
         launch_merge_partition(rows, nnz, d_rp.get(), prow.get(), s);
         launch_merge(rows, nnz, d_rp.get(), d_ci.get(), d_v.get(), d_x.get(), EpiStore<float>{d_y.get()}, prow.get(),
-                     carry.get(), head.get(), s);
+                     d_longs.get(), static_cast<int>(longs.size()), carry.get(), head.get(), s);
         d_y.download(y, rows, s);
         PM_HIP(hipStreamSynchronize(s));
         return;
