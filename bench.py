@@ -23,6 +23,10 @@ windows), timed with HIP events on the batch's stream; algorithmic bytes
 per launch = sum over windows of 8 nnz + 4 (n+1) + 4 m + 28 n (A^T) or
 8 nnz + 4 (m+1) + 4 n + 20 m (A).  spmv_lp1: the same for the single-LP
 engine's fused SpMV on the 500k x 1M LP (12M nonzeros).
+plain_pdhg_gpu: plain cold-start PDHG on the GPU, i.e. the reference
+algorithm restated on the device (PM_ALG_REFERENCE), run to its own end on
+a few sample windows one after another; the FP64 relative KKT error of
+each final point comes from the batch engine's warm-start judge.
 cpu_baseline: the oracle's restatement of the reference PDHG (FP32, its
 own termination test pv, dv <= tol and its default 100,000-iteration
 budget) run to its end on a sample of windows spread over the sequence,
@@ -297,11 +301,56 @@ def bench_rolling(args, P) -> dict:
             "learned_iterations_mean_same_windows": float(it_l[:npw].mean())}
         log(f"previous-window warm start on windows 0..{npw - 1}: iterations mean {st_p.iterations.mean():.0f} "
             f"(cold {it_c[:npw].mean():.0f}, learned {it_l[:npw].mean():.0f}), {st_p.seconds:.2f}s sequential")
+    if args.plain_windows > 0:
+        out["plain_pdhg_gpu"] = plain_pdhg_gpu(args, P, lps, sec_l)
     if not args.no_lp1:
         out["spmv_lp1"] = spmv_lp1(P, args.kernel_reps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_rolling(args, lps)
     return out
+
+
+def plain_pdhg_gpu(args, P, lps, learned_seconds) -> dict:
+    """Plain cold-start PDHG on the GPU: the reference algorithm
+    (PM_ALG_REFERENCE: pdhg_solver.cu's iteration, restarts and pv/dv test,
+    100,000-iteration budget) on the single-LP engine, run to its own end on
+    a sample of windows spread over the sequence, one after another.  The
+    FP64 relative KKT error of each final point is the batch engine's judge
+    of that point as a warm start (pm_batch_warm_start_verdicts)."""
+    k = min(len(lps), args.plain_windows)
+    pick = [int(round(i * (len(lps) - 1) / max(1, k - 1))) for i in range(k)]
+    res, secs = [], []
+    for w in pick:
+        t0 = time.perf_counter()
+        r = P.solve(lps[w], algorithm="reference", tolerance=args.tol, max_iterations=100000)
+        secs.append(time.perf_counter() - t0)
+        res.append(r)
+        log(f"plain PDHG, window {w}: {r.status} after {r.iterations} iterations, {secs[-1]:.2f}s, "
+            f"pv {r.primal_residual:.2e} dv {r.dual_residual:.2e}")
+    sample = [lps[w] for w in pick]
+    with P.Batch(sample, tolerance=args.tol, max_iterations=64, check_frequency=64) as jb:
+        for i, r in enumerate(res):
+            jb.set_warm_start(i, r.x, r.y)
+        jb.run()
+        acc, kkt = jb.warm_start_verdicts()
+    rel = [float(kk) if a == 1 else None for a, kk in zip(acc, kkt)]
+    reached = sum(1 for v in rel if v is not None and v <= args.tol)
+    per_window = float(np.mean(secs))
+    est = per_window * len(lps)
+    log(f"plain PDHG: {reached}/{k} sample windows at rel KKT <= {args.tol:g}; {per_window:.2f}s per window "
+        f"-> {est:.0f}s for {len(lps)} windows in sequence")
+    return {
+        "sample_windows": pick, "seconds_per_window": per_window,
+        "status": [r.status for r in res], "iterations": [int(r.iterations) for r in res],
+        "reference_pv": [float(r.primal_residual) for r in res], "reference_dv": [float(r.dual_residual) for r in res],
+        "rel_kkt_final": rel, "reached_rel_kkt_tolerance": reached,
+        "seconds_all_windows_estimate": est,
+        "learned_speedup_lower_bound": est / learned_seconds if reached < k else None,
+        "note": ("the reference algorithm stops on its own absolute pv/dv test or at 100,000 iterations; "
+                 "rel_kkt_final is the FP64 relative KKT error of its last point (None: worse than the zero "
+                 "start); where no sample window reaches the tolerance, the estimate (per-window seconds x "
+                 "windows) is a lower bound on its time to tolerance"),
+    }
 
 
 def cpu_baseline_rolling(args, lps) -> dict:
@@ -619,6 +668,8 @@ def main() -> int:
     ap.add_argument("--dense-rows", type=int, default=0,
                     help="lp1 workload: rows over every column appended (skewed row lengths)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--plain-windows", type=int, default=4,
+                    help="sample windows of the plain cold-start GPU PDHG comparison (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lp1", action="store_true", help="skip the 500k x 1M SpMV measurement")
     ap.add_argument("--no-graph", action="store_true",
