@@ -390,7 +390,9 @@ def bench_batch(args, P) -> dict:
     log(f"generated {len(lps)} LPs, {sum(lp.m for lp in lps)} rows, {nnz} nonzeros in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
     b = P.Batch(lps, tolerance=args.tol, max_iterations=args.max_iters, check_frequency=64)
-    log(f"resident in {time.perf_counter() - t0:.1f}s; iterate kernels (A, A^T) {b.spmv_variants()}")
+    classes = b.size_classes()
+    log(f"resident in {time.perf_counter() - t0:.1f}s; iterate kernels (A, A^T) {b.spmv_variants()}; "
+        f"size classes {classes}")
     for _ in range(args.warmup):
         b.run()
     times, res, inst = [], None, None
@@ -420,6 +422,7 @@ def bench_batch(args, P) -> dict:
         "instance_seconds_p50": float(np.median(inst)), "instance_seconds_p99": float(np.quantile(inst, 0.99)),
         "iterations_mean": float(it.mean()), "iterations_max": int(it.max()),
         "objective_rel_error_vs_closed_form_max": err,
+        "size_classes": classes,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "kernel_us": ms * 1e3,
                      "alg_bytes_per_launch": by},

@@ -118,9 +118,11 @@ extern "C" {
                                   n >= 2^19 and nnz >= 2^22.              */
 #define PM_SPMV_MERGE       13 /* merge-path CSR (the product split evenly
                                   over rows + nonzeros, whatever the row
-                                  lengths; a second pass adds the parts of
-                                  split rows in order and applies the fused
-                                  update).  Single-LP PDLP engine: both
+                                  lengths; a row of more than 128 entries
+                                  split over tiles is finished by a second
+                                  pass that adds its parts in order and
+                                  applies the fused update).  Single-LP PDLP
+                                  engine: both
                                   products of the iteration; AUTO times it
                                   beside the row kernels on LPs of at least
                                   2^18 nonzeros.  Also pm_launch_spmv.     */
@@ -376,6 +378,23 @@ int pm_batch_time_kernel(pm_batch* b, int32_t which, int32_t reps,
  * pm_batch_create over the whole batch. */
 int pm_batch_spmv_variants(pm_batch* b, int32_t* variant_a,
                            int32_t* variant_at);
+/* Size classes of the iterate launches (no counterpart in the reference;
+ * for callers reporting the batch's kernel choice): the LPs fall in
+ * PM_BATCH_SIZE_CLASSES classes by row count (m < 256, m < 1024, larger).
+ * *launches = 1 when one launch per half iteration covers every LP (then
+ * pm_batch_spmv_variants gives its kernels), PM_BATCH_SIZE_CLASSES when
+ * the autotuner found one launch per class faster; lps[c] = LPs of class
+ * c, variant_a[c] / variant_at[c] = that class's kernels (PM_SPMV_*).
+ * Any output may be NULL. */
+#define PM_BATCH_SIZE_CLASSES 3
+int pm_batch_size_classes(pm_batch* b, int32_t* launches, int32_t* lps,
+                          int32_t* variant_a, int32_t* variant_at);
+/* Overrides the autotuner's choice (no counterpart in the reference; for
+ * tests and tuning): launches = 1 issues one launch per half iteration over
+ * every LP, PM_BATCH_SIZE_CLASSES one per non-empty size class with that
+ * class's kernels (the whole batch's kernels when nothing was tuned per
+ * class).  The next pm_batch_run captures its graphs again. */
+int pm_batch_set_size_classes(pm_batch* b, int32_t launches);
 /* create + run + destroy; x0/y0 may be NULL or arrays of count pointers. */
 int pm_batch_solve(int32_t count, const pm_lp* lps, const double* const* x0,
                    const double* const* y0, const pm_pdhg_options* opts,

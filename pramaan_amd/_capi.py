@@ -104,6 +104,8 @@ SIGNATURES = {
     "pm_batch_set_primal_weight": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
     "pm_batch_scaling": (C.c_int, [C.c_void_p, C.c_int32, _d, _d]),
     "pm_batch_spmv_variants": (C.c_int, [C.c_void_p, _i, _i]),
+    "pm_batch_size_classes": (C.c_int, [C.c_void_p, _i, _i, _i, _i]),
+    "pm_batch_set_size_classes": (C.c_int, [C.c_void_p, C.c_int32]),
     "pm_batch_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _d, _d]),
     "pm_batch_solve": (C.c_int, [C.c_int32, C.POINTER(pm_lp), C.POINTER(_d), C.POINTER(_d),
                                  C.POINTER(pm_pdhg_options), C.POINTER(pm_pdhg_result), C.POINTER(pm_batch_stats)]),
@@ -434,6 +436,22 @@ class Batch:
         dr, dc = np.zeros(max(lp.m, 1)), np.zeros(lp.n)
         _check(lib().pm_batch_scaling(self._h, i, _p(dr, C.c_double), _p(dc, C.c_double)))
         return dr[:lp.m], dc
+
+    def size_classes(self) -> dict:
+        """pm_batch_size_classes: launches per half iteration (1 or one per
+        size class), and per class its LP count and kernels."""
+        n = C.c_int32()
+        lps = (C.c_int32 * 3)()
+        va = (C.c_int32 * 3)()
+        vat = (C.c_int32 * 3)()
+        _check(lib().pm_batch_size_classes(self._h, C.byref(n), lps, va, vat))
+        return {"launches": n.value, "lps": list(lps), "variant_a": [SPMV_NAMES.get(v, v) for v in va],
+                "variant_at": [SPMV_NAMES.get(v, v) for v in vat]}
+
+    def set_size_classes(self, launches: int) -> None:
+        """pm_batch_set_size_classes: 1 launch over every LP, or 3 (one per
+        size class)."""
+        _check(lib().pm_batch_set_size_classes(self._h, launches))
 
     def spmv_variants(self) -> tuple[int, int]:
         a, at = C.c_int32(), C.c_int32()

@@ -168,6 +168,31 @@ inline void build_sliced(int nrows, const int* rp, const int* ci, int64_t nz0, S
     }
 }
 
+// Rows per sliced group of an LP's A: kSliceGroupWide when its groups of
+// kSliceGroup rows span on average more than half of its n columns (random
+// column structure: the window stays near n whatever the group) and n fits
+// the staged kernel's LDS window, kSliceGroup otherwise (banded structure,
+// such as multi-period plans, whose windows grow with the group).
+inline int group_rows_a(int m, int n, const int* rp, const int* ci) {
+    if (m <= kSliceGroup || n > kStagedMaxVec) return kSliceGroup;
+    double width = 0.0;
+    int groups = 0;
+    for (int g0 = 0; g0 < m; g0 += kSliceGroup, ++groups) {
+        const int g1 = std::min(m, g0 + kSliceGroup);
+        int lo = INT_MAX, hi = -1;
+        for (int k = rp[g0]; k < rp[g1]; ++k) {
+            lo = std::min(lo, ci[k]);
+            hi = std::max(hi, ci[k]);
+        }
+        if (hi >= lo) width += hi - lo + 1;
+    }
+    return width > 0.5 * static_cast<double>(n) * groups ? kSliceGroupWide : kSliceGroup;
+}
+
+// Size class of an LP for the per-class iterate launches: fewer than 256
+// rows, fewer than 1,024 (one group of A), or more.
+inline int size_class(int m) { return m < 256 ? 0 : (m < kSliceGroup ? 1 : 2); }
+
 // Stream segments of one CSR: consecutive rows with at most kStreamNnz
 // nonzeros and kBlock rows together; a longer row forms a segment alone.
 inline void stream_segments(int nrows, const int* rp, std::vector<int2>& seg) {
@@ -531,9 +556,11 @@ __global__ __launch_bounds__(kBB) void k_b_init(BatchArrays<VT> A, BatchState* S
 // counts in wcnt[0] (rows of A) and wcnt[1] (rows of A^T).  Also the
 // blocks of the round and check passes (ck_r, ck_c, counts in wcnt[2],
 // wcnt[3], each LP's range in wrange).  One 1024-thread workgroup scans
-// the LPs.
+// the LPs.  bucket: -1 both lists over every LP; -2 the check blocks only;
+// c >= 0 the items of the LPs of size class c only (wr, wc and wcnt then
+// point at that class's regions; wcnt[2], wcnt[3] are left alone).
 template <class VT>
-__global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int count, int ga, int gt,
+__global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int count, int bucket, int ga, int gt,
                                                      const int2* __restrict__ seg_a, const int2* __restrict__ seg_at,
                                                      WorkItem* __restrict__ wr, WorkItem* __restrict__ wc,
                                                      int* __restrict__ wcnt) {
@@ -541,8 +568,8 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
     __shared__ int carry_r, carry_c, carry_kr, carry_kc;
     const int tid = static_cast<int>(threadIdx.x);
     // ga, gt: rows per item of the lanes kernels, 0 for stream segments,
-    // negative for groups of the sliced layout.
-    constexpr int ra = kSliceGroup, rt = kSliceGroupT;
+    // negative for groups of the sliced layout (each LP's own group rows,
+    // BatchDesc::ga / gt).
     if (tid == 0) carry_r = carry_c = carry_kr = carry_kc = 0;
     __syncthreads();
     for (int base = 0; base < count; base += 1024) {
@@ -554,10 +581,14 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
             const BatchDesc& d = A.desc[k];
             m = d.m;
             n = d.n;
-            nr = ga > 0 ? (m + ga - 1) / ga : (ga == 0 ? d.sa_cnt : (m + ra - 1) / ra);
-            nc = gt > 0 ? (n + gt - 1) / gt : (gt == 0 ? d.st_cnt : (n + rt - 1) / rt);
-            br = (m + kCheckRows - 1) / kCheckRows;
-            bc = (n + kCheckRows - 1) / kCheckRows;
+            if (bucket == -1 || d.bucket == bucket) {
+                nr = ga > 0 ? (m + ga - 1) / ga : (ga == 0 ? d.sa_cnt : (m + d.ga - 1) / d.ga);
+                nc = gt > 0 ? (n + gt - 1) / gt : (gt == 0 ? d.st_cnt : (n + d.gt - 1) / d.gt);
+            }
+            if (bucket < 0) {
+                br = (m + kCheckRows - 1) / kCheckRows;
+                bc = (n + kCheckRows - 1) / kCheckRows;
+            }
         }
         sr[tid] = nr;
         sc[tid] = nc;
@@ -577,7 +608,7 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
             fa[tid] += af;
             __syncthreads();
         }
-        if (k < count)
+        if (k < count && bucket < 0)
             A.wrange[k] = make_int4(carry_kr + kr[tid] - br, br, carry_kc + kc[tid] - bc, bc);
         // The LPs of this chunk with anything to write, compacted; each
         // gets `per` threads, which write its items and blocks interleaved
@@ -608,8 +639,8 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
                     first = sg.x;
                     rows = sg.y;
                 } else if (ga < 0) {
-                    first = g * ra;
-                    rows = min(ra, mk - first);
+                    first = g * d.ga;
+                    rows = min(d.ga, mk - first);
                     tag = d.la_off + first / kSliceRows;
                 }
                 wr[offr + g] = WorkItem{rok + first, rows, Pk.k0, Pk.chunk, Pk.use_g, Pk.col_off, Pk.sig, tag};
@@ -621,8 +652,8 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
                     first = sg.x;
                     rows = sg.y;
                 } else if (gt < 0) {
-                    first = g * rt;
-                    rows = min(rt, nk - first);
+                    first = g * d.gt;
+                    rows = min(d.gt, nk - first);
                     tag = d.lt_off + first / kSliceRows;
                 }
                 wc[offc + g] = WorkItem{cok + first, rows, Pk.k0, Pk.chunk, 0, Pk.row_off, Pk.tau, tag};
@@ -638,10 +669,14 @@ __global__ __launch_bounds__(1024) void k_b_worklist(BatchArrays<VT> A, int coun
         __syncthreads();
     }
     if (tid == 0) {
-        wcnt[0] = carry_r;
-        wcnt[1] = carry_c;
-        wcnt[2] = carry_kr;
-        wcnt[3] = carry_kc;
+        if (bucket != -2) {
+            wcnt[0] = carry_r;
+            wcnt[1] = carry_c;
+        }
+        if (bucket < 0) {
+            wcnt[2] = carry_kr;
+            wcnt[3] = carry_kc;
+        }
     }
 }
 
@@ -811,8 +846,9 @@ __device__ __forceinline__ void slice_steps(const float* __restrict__ val, const
     }
 }
 
-// Sliced kernels: a work item is a group of up to kSliceGroup rows of A
-// (kSliceGroupT of A^T, whose shorter rows gain from the larger group) of
+// Sliced kernels: a work item is a group of up to kSliceGroup or
+// kSliceGroupWide rows of A (kSliceGroupT of A^T, whose shorter rows gain
+// from the larger group) of
 // one LP in the sliced layout, walked in passes of 16 slices.  In a pass,
 // wavefront w takes slices w, w + 4, w + 8 and w + 12 and walks them
 // together: lane l reads entry j of its row in slice q at ptr[q] + j * 64
@@ -830,7 +866,7 @@ template <bool PRIMAL, bool STAGED, class VT>
 __global__ __launch_bounds__(kBlock) void k_e_half(BatchArrays<VT> A, const WorkItem* __restrict__ work,
                                                    const int* __restrict__ wcnt, int i) {
     constexpr int kWaves = kBlock / 64, kPer = 4, kPass = kWaves * kPer;
-    constexpr int kG = PRIMAL ? kSliceGroupT : kSliceGroup;
+    constexpr int kG = PRIMAL ? kSliceGroupT : kSliceGroupWide;  // the largest group of either side
     extern __shared__ float xs[];
     __shared__ float sums[kG];
     const int t = static_cast<int>(threadIdx.x);
@@ -1420,8 +1456,11 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         d.lanes_at = lanes_for(static_cast<double>(nnz) / n);
         stream_segments(m, L.row_ptr, seg_a_lp[k]);
         stream_segments(n, cp.data(), seg_at_lp[k]);
-        build_sliced(m, L.row_ptr, L.col_idx, d.nz_off, sl_a_lp[k], kSliceGroup);
-        build_sliced(n, cp.data(), ri.data(), d.nz_off, sl_at_lp[k], kSliceGroupT);
+        d.ga = group_rows_a(m, n, L.row_ptr, L.col_idx);
+        d.bucket = size_class(m);
+        d.gt = kSliceGroupT;
+        build_sliced(m, L.row_ptr, L.col_idx, d.nz_off, sl_a_lp[k], d.ga);
+        build_sliced(n, cp.data(), ri.data(), d.nz_off, sl_at_lp[k], d.gt);
     };
     {
         const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -1545,16 +1584,25 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->lanes_at = lanes_for(static_cast<double>(nzo) / static_cast<double>(co));
     // Work-list capacity: the largest lane count (the smallest groups) or
     // the stream segments, whichever gives more items.
+    // Each LP's items under any kernel fit max(its groups of the largest
+    // lane count, its stream segments); the size classes take consecutive
+    // regions, and the list over every LP uses the whole array.
     constexpr int kMinGroup = kBlock / kMaxLanes;
-    size_t items_r = 0, items_c = 0, segs_r = 0, segs_c = 0;
+    int64_t cap_r[pm_batch::kBuckets] = {0, 0, 0}, cap_c[pm_batch::kBuckets] = {0, 0, 0};
     for (const BatchDesc& d : b->desc) {
-        items_r += (d.m + kMinGroup - 1) / kMinGroup;
-        items_c += (d.n + kMinGroup - 1) / kMinGroup;
-        segs_r += d.sa_cnt;
-        segs_c += d.st_cnt;
+        cap_r[d.bucket] += std::max<int64_t>((d.m + kMinGroup - 1) / kMinGroup, d.sa_cnt);
+        cap_c[d.bucket] += std::max<int64_t>((d.n + kMinGroup - 1) / kMinGroup, d.st_cnt);
+        ++b->bucket_lps[d.bucket];
     }
-    b->work_r.resize(std::max<size_t>(std::max(items_r, segs_r), 1));
-    b->work_c.resize(std::max<size_t>(std::max(items_c, segs_c), 1));
+    int64_t tot_r = 0, tot_c = 0;
+    for (int c = 0; c < pm_batch::kBuckets; ++c) {
+        b->off_r[c] = tot_r;
+        b->off_c[c] = tot_c;
+        tot_r += cap_r[c];
+        tot_c += cap_c[c];
+    }
+    b->work_r.resize(static_cast<size_t>(std::max<int64_t>(tot_r, 1)));
+    b->work_c.resize(static_cast<size_t>(std::max<int64_t>(tot_c, 1)));
     size_t blocks_r = 0, blocks_c = 0;
     for (const BatchDesc& d : b->desc) {
         blocks_r += (d.m + kCheckRows - 1) / kCheckRows;
@@ -1565,11 +1613,15 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
     b->part_r.resize(b->ck_r.size() * kPart);
     b->part_c.resize(b->ck_c.size() * kPart);
     b->wrange.resize(static_cast<size_t>(count));
-    b->wcnt.resize(4);
+    b->wcnt.resize(4 + 2 * pm_batch::kBuckets);
     if (opts.spmv_variant == PM_SPMV_MERGE || opts.spmv_variant == PM_SPMV_COLBLOCK)
         fail(PM_ERR_INVALID_ARG, "spmv_variant PM_SPMV_MERGE and PM_SPMV_COLBLOCK are single-LP layouts");
     const int fixed = variant_kernel(opts.spmv_variant);
     if (fixed != kAutoKernel) b->lanes_a = b->lanes_at = fixed;
+    for (int c = 0; c < pm_batch::kBuckets; ++c) {
+        b->blanes_a[c] = b->lanes_a;
+        b->blanes_at[c] = b->lanes_at;
+    }
     if (fixed == kStagedKernel && std::max(b->win_a, b->win_at) > kStagedMaxVec)
         fail(PM_ERR_INVALID_ARG, "PM_SPMV_STAGED holds each group's column window in LDS: windows must stay within " +
                                      std::to_string(kStagedMaxVec) + " columns");
@@ -1664,11 +1716,15 @@ void launch_resident(pm_batch* b, int mult, K kernel, size_t lds, Args... args) 
 // dual) of iteration i of the chunk, with the kernel for `lanes` (a lanes
 // count or a kernel code), on `mult` resident grids.
 template <class VT>
-void launch_iterate(pm_batch* b, const BatchArrays<VT>& A, int which, int i, int lanes, int mult) {
+void launch_iterate(pm_batch* b, const BatchArrays<VT>& A, int which, int i, int lanes, int mult, int bucket = -1) {
     const bool primal = which == PM_KERNEL_SPMV_AT;
     if (!primal && b->rows_total == 0) return;
     WorkItem* work = primal ? b->work_c.get() : b->work_r.get();
     int* wc = b->wcnt.get();
+    if (bucket >= 0) {
+        work += primal ? b->off_c[bucket] : b->off_r[bucket];
+        wc += 4 + 2 * bucket;
+    }
     if (lanes == kStreamKernel) {
         if (primal) launch_resident(b, mult, k_t_half<true, VT>, 0, A, work, wc, i);
         else launch_resident(b, mult, k_t_half<false, VT>, 0, A, work, wc, i);
@@ -1690,6 +1746,41 @@ void launch_iterate(pm_batch* b, const BatchArrays<VT>& A, int which, int i, int
     PM_LAUNCHED();
 }
 
+// Half `which` of iteration i with the kernels of the launches being issued:
+// one launch over every LP, or one per size class.
+template <class VT>
+void launch_half(pm_batch* b, const BatchArrays<VT>& A, int which, int i) {
+    const bool dual = which == PM_KERNEL_SPMV_A;
+    if (!b->mode_bucketed) {
+        launch_iterate(b, A, which, i, dual ? b->mode_a : b->mode_at, dual ? b->mode_mult_a : b->mode_mult_at);
+        return;
+    }
+    for (int c = 0; c < pm_batch::kBuckets; ++c)
+        if (b->bucket_lps[c] > 0)
+            launch_iterate(b, A, which, i, dual ? b->blanes_a[c] : b->blanes_at[c],
+                           dual ? b->bmult_a[c] : b->bmult_at[c], c);
+}
+
+// Work-list codes of a kernel: rows per item of a lanes kernel, or the code.
+inline int items_code(int lanes) { return lanes > 0 ? kBlock / lanes : lanes; }
+
+// Work list of size class `bucket` (-1: every LP, with the check blocks;
+// -2: the check blocks only) for the given kernels.
+template <class VT>
+void launch_worklist(pm_batch* b, const BatchArrays<VT>& A, int bucket, int lanes_a, int lanes_at) {
+    WorkItem* wr = b->work_r.get();
+    WorkItem* wc = b->work_c.get();
+    int* cnt = b->wcnt.get();
+    if (bucket >= 0) {
+        wr += b->off_r[bucket];
+        wc += b->off_c[bucket];
+        cnt += 4 + 2 * bucket;
+    }
+    hipLaunchKernelGGL(k_b_worklist<VT>, dim3(1), dim3(1024), 0, b->stream, A, b->count, bucket, items_code(lanes_a),
+                       items_code(lanes_at), b->seg_a.get(), b->seg_at.get(), wr, wc, cnt);
+    PM_LAUNCHED();
+}
+
 template <class VT>
 void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
     hipStream_t s = b->stream;
@@ -1703,16 +1794,19 @@ void launch_phase(pm_batch* b, int phase, const VT* oa, const VT* oat) {
             break;
         case 2:
             for (int i = 0; i < b->check_frequency; ++i) {
-                launch_iterate(b, A, PM_KERNEL_SPMV_AT, i, b->mode_at, b->mode_mult_at);
-                launch_iterate(b, A, PM_KERNEL_SPMV_A, i, b->mode_a, b->mode_mult_a);
+                launch_half(b, A, PM_KERNEL_SPMV_AT, i);
+                launch_half(b, A, PM_KERNEL_SPMV_A, i);
             }
             return;
         case 4:
-            hipLaunchKernelGGL(k_b_worklist<VT>, dim3(1), dim3(1024), 0, s, A, b->count,
-                               b->mode_a > 0 ? kBlock / b->mode_a : b->mode_a,
-                               b->mode_at > 0 ? kBlock / b->mode_at : b->mode_at,  // rows per item, or a code
-                               b->seg_a.get(), b->seg_at.get(), b->work_r.get(), b->work_c.get(), b->wcnt.get());
-            break;
+            if (!b->mode_bucketed) {
+                launch_worklist(b, A, -1, b->mode_a, b->mode_at);
+            } else {
+                launch_worklist(b, A, -2, b->mode_a, b->mode_at);
+                for (int c = 0; c < pm_batch::kBuckets; ++c)
+                    if (b->bucket_lps[c] > 0) launch_worklist(b, A, c, b->blanes_a[c], b->blanes_at[c]);
+            }
+            return;
         case 3:
             launch_resident(b, 1, k_b_items<kPassTotals, VT>, 0, A, S);
             PM_LAUNCHED();
@@ -1732,6 +1826,7 @@ void phase(pm_batch* b, int p) {
 
 // Kernels of the launches issued next: the tuned ones, or the tail ones.
 void set_mode(pm_batch* b, bool tail) {
+    b->mode_bucketed = !tail && b->nbuckets > 1;
     b->mode_a = tail ? b->tail_a : b->lanes_a;
     b->mode_at = tail ? b->tail_at : b->lanes_at;
     b->mode_mult_a = tail ? 1 : b->mult_a;
@@ -1851,10 +1946,7 @@ void batch_time_kernel(pm_batch* b, int which, int reps, double* avg_ms, double*
     set_mode(b, false);
     phase(b, 0);  // every LP active, chunk = check_frequency
     phase(b, 4);
-    const bool dual = which == PM_KERNEL_SPMV_A;
-    auto launch = [&](const auto& A) {
-        launch_iterate(b, A, which, 0, dual ? b->lanes_a : b->lanes_at, dual ? b->mult_a : b->mult_at);
-    };
+    auto launch = [&](const auto& A) { launch_half(b, A, which, 0); };
     auto one = [&] {
         if (b->exact) launch(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
         else launch(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
@@ -1879,7 +1971,13 @@ void batch_time_kernel(pm_batch* b, int which, int reps, double* avg_ms, double*
 // Kernel of each half of the iteration (staged, sliced, stream, or 1..32
 // lanes per row; the sliced ones on one or four resident grids): each
 // candidate timed over the whole batch with every LP running (iterates
-// zeroed, small steps), fastest kept.  Runs once at creation.
+// zeroed, small steps), fastest kept.  Then, when the batch holds LPs of
+// more than one size class, each class's fastest kernel is found on the
+// class alone, and one launch per class is kept when the whole sequence of
+// class launches (both halves, every class in turn, as a chunk issues
+// them) takes under 97 % of the single launches' time.  A class timed
+// alone can sit in the last-level cache across repeats, so only the whole
+// sequence decides.  Runs once at creation.
 void batch_autotune(pm_batch* b) {
     hipStream_t s = b->stream;
     for (auto* p : {&b->x, &b->xe, &b->x0, &b->xt, &b->xl, &b->c32, &b->lx32, &b->ux32})
@@ -1892,44 +1990,98 @@ void batch_autotune(pm_batch* b) {
                        static_cast<int32_t>(b->desc[k].col_off), 0};
     }
     b->iter.upload(it, s);
+    b->nbuckets = 1;
     const int cand[] = {kStagedKernel, kSlicedKernel, kStreamKernel, 1, 2, 4, 8, 16, 32};
-    for (int which : {PM_KERNEL_SPMV_AT, PM_KERNEL_SPMV_A}) {
-        if (which == PM_KERNEL_SPMV_A && b->rows_total == 0) continue;
-        int& lanes = which == PM_KERNEL_SPMV_A ? b->lanes_a : b->lanes_at;
-        int& mult = which == PM_KERNEL_SPMV_A ? b->mult_a : b->mult_at;
+    // Mean time of one launch of `which` with kernel l on g grids over the
+    // LPs of size class `bucket` (-1: every LP).
+    auto time_launch = [&](int which, int l, int g, int bucket) {
+        auto run = [&](const auto& A) {
+            launch_worklist(b, A, bucket, l, l);
+            launch_iterate(b, A, which, 0, l, g, bucket);
+            PM_HIP(hipEventRecord(b->ev0, s));
+            for (int r = 0; r < 3; ++r) launch_iterate(b, A, which, 0, l, g, bucket);
+            PM_HIP(hipEventRecord(b->ev1, s));
+        };
+        if (b->exact) run(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
+        else run(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
+        PM_HIP(hipEventSynchronize(b->ev1));
+        float ms = 0.0f;
+        PM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        return static_cast<double>(ms) / 3.0;
+    };
+    auto best_of = [&](int which, int bucket, int& lanes, int& mult) {
         const int widest = which == PM_KERNEL_SPMV_A ? b->win_a : b->win_at;
         double best = INFINITY;
-        int best_l = lanes, best_g = 1;
         for (int l : cand) {
             if ((l == kStagedKernel && widest > kStagedMaxVec) || (l == kSlicedKernel && widest > kSlicedMaxWin))
                 continue;
             for (int g : {1, 4}) {
                 if (g > 1 && l != kStagedKernel && l != kSlicedKernel) continue;
-                lanes = l;
-                mult = g;
-                set_mode(b, false);
-                phase(b, 4);
-                auto launch = [&](const auto& A) { launch_iterate(b, A, which, 0, l, g); };
-                auto one = [&] {
-                    if (b->exact) launch(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
-                    else launch(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
-                };
-                one();
-                PM_HIP(hipEventRecord(b->ev0, s));
-                for (int r = 0; r < 3; ++r) one();
-                PM_HIP(hipEventRecord(b->ev1, s));
-                PM_HIP(hipEventSynchronize(b->ev1));
-                float ms = 0.0f;
-                PM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                const double ms = time_launch(which, l, g, bucket);
                 if (ms < best) {
                     best = ms;
-                    best_l = l;
-                    best_g = g;
+                    lanes = l;
+                    mult = g;
                 }
             }
         }
-        mult = best_g;
-        lanes = best_l;
+        return best;
+    };
+    double whole = 0.0, split = 0.0;  // sums of the best single-launch times, for the record
+    for (int which : {PM_KERNEL_SPMV_AT, PM_KERNEL_SPMV_A}) {
+        if (which == PM_KERNEL_SPMV_A && b->rows_total == 0) continue;
+        const bool dual = which == PM_KERNEL_SPMV_A;
+        whole += best_of(which, -1, dual ? b->lanes_a : b->lanes_at, dual ? b->mult_a : b->mult_at);
+    }
+    int classes = 0;
+    for (int c = 0; c < pm_batch::kBuckets; ++c) {
+        classes += b->bucket_lps[c] > 0;
+        b->blanes_a[c] = b->lanes_a;
+        b->blanes_at[c] = b->lanes_at;
+        b->bmult_a[c] = b->mult_a;
+        b->bmult_at[c] = b->mult_at;
+    }
+    if (classes > 1) {
+        for (int which : {PM_KERNEL_SPMV_AT, PM_KERNEL_SPMV_A}) {
+            if (which == PM_KERNEL_SPMV_A && b->rows_total == 0) continue;
+            const bool dual = which == PM_KERNEL_SPMV_A;
+            for (int c = 0; c < pm_batch::kBuckets; ++c)
+                if (b->bucket_lps[c] > 0)
+                    split += best_of(which, c, dual ? b->blanes_a[c] : b->blanes_at[c],
+                                     dual ? b->bmult_a[c] : b->bmult_at[c]);
+        }
+        (void)split;
+        // both halves as a chunk issues them, one launch over every LP
+        // against one launch per class
+        auto time_halves = [&](int nb) {
+            b->nbuckets = nb;
+            set_mode(b, false);
+            auto run = [&](const auto& A) {
+                if (nb > 1) {
+                    launch_worklist(b, A, -2, b->lanes_a, b->lanes_at);
+                    for (int c = 0; c < pm_batch::kBuckets; ++c)
+                        if (b->bucket_lps[c] > 0) launch_worklist(b, A, c, b->blanes_a[c], b->blanes_at[c]);
+                } else {
+                    launch_worklist(b, A, -1, b->lanes_a, b->lanes_at);
+                }
+                launch_half(b, A, PM_KERNEL_SPMV_AT, 0);
+                launch_half(b, A, PM_KERNEL_SPMV_A, 0);
+                PM_HIP(hipEventRecord(b->ev0, s));
+                for (int r = 0; r < 3; ++r) {
+                    launch_half(b, A, PM_KERNEL_SPMV_AT, 0);
+                    launch_half(b, A, PM_KERNEL_SPMV_A, 0);
+                }
+                PM_HIP(hipEventRecord(b->ev1, s));
+            };
+            if (b->exact) run(batch_arrays<float>(b, b->oa32.get(), b->oat32.get()));
+            else run(batch_arrays<double>(b, b->oa64.get(), b->oat64.get()));
+            PM_HIP(hipEventSynchronize(b->ev1));
+            float ms = 0.0f;
+            PM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            return static_cast<double>(ms) / 3.0;
+        };
+        const double t_one = time_halves(1), t_classes = time_halves(pm_batch::kBuckets);
+        b->nbuckets = t_classes < 0.97 * t_one ? pm_batch::kBuckets : 1;
     }
     set_mode(b, false);
 }
@@ -1947,6 +2099,19 @@ int pm_batch_spmv_variants(pm_batch* b, int32_t* variant_a, int32_t* variant_at)
     });
 }
 
+int pm_batch_size_classes(pm_batch* b, int32_t* launches, int32_t* lps, int32_t* variant_a, int32_t* variant_at) {
+    static_assert(pm_batch::kBuckets == PM_BATCH_SIZE_CLASSES, "size classes of the C-ABI");
+    return pm::guarded([&] {
+        if (!b) pm::fail(PM_ERR_INVALID_ARG, "batch is NULL");
+        if (launches) *launches = b->nbuckets;
+        for (int c = 0; c < pm_batch::kBuckets; ++c) {
+            if (lps) lps[c] = b->bucket_lps[c];
+            if (variant_a) variant_a[c] = pm::lanes_variant(b->nbuckets > 1 ? b->blanes_a[c] : b->lanes_a);
+            if (variant_at) variant_at[c] = pm::lanes_variant(b->nbuckets > 1 ? b->blanes_at[c] : b->lanes_at);
+        }
+    });
+}
+
 int pm_batch_warm_start_verdicts(pm_batch* b, int32_t* accepted, double* start_rel_kkt) {
     return pm::guarded([&] {
         if (!b) pm::fail(PM_ERR_INVALID_ARG, "batch is NULL");
@@ -1954,6 +2119,25 @@ int pm_batch_warm_start_verdicts(pm_batch* b, int32_t* accepted, double* start_r
             if (accepted) accepted[k] = b->desc[k].warm_accepted;
             if (start_rel_kkt) start_rel_kkt[k] = b->desc[k].start_kkt;
         }
+    });
+}
+
+int pm_batch_set_size_classes(pm_batch* b, int32_t launches) {
+    return pm::guarded([&] {
+        if (!b) pm::fail(PM_ERR_INVALID_ARG, "batch is NULL");
+        if (launches != 1 && launches != PM_BATCH_SIZE_CLASSES)
+            pm::fail(PM_ERR_INVALID_ARG, "launches must be 1 or PM_BATCH_SIZE_CLASSES");
+        PM_HIP(hipSetDevice(b->device));
+        PM_HIP(hipStreamSynchronize(b->stream));
+        b->nbuckets = launches;
+        pm::set_mode(b, false);
+        // the captured graphs issue the previous launches: capture again
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->exec_tail) (void)hipGraphExecDestroy(b->exec_tail);
+        if (b->graph_tail) (void)hipGraphDestroy(b->graph_tail);
+        b->exec = b->exec_tail = nullptr;
+        b->graph = b->graph_tail = nullptr;
     });
 }
 

@@ -29,9 +29,11 @@ struct BatchDesc {
     // (rows of A) and seg_at (rows of A^T) from these offsets
     int32_t sa_off, sa_cnt, st_off, st_cnt;
     // slices of the sliced iterate kernels (64 rows each, rows sorted by
-    // length inside every group of kSliceGroup / kSliceGroupT rows), from
+    // length inside every group of ga rows of A / gt rows of A^T), from
     // these offsets in the sliced arrays of A and A^T
     int32_t la_off, lt_off;
+    int32_t ga, gt;
+    int32_t bucket;  // size class of the iterate launches (size_class)
     // results
     int32_t status, iterations, restarts, refinements, polishes;
     double rel_kkt, rel_p, rel_d, rel_g, pobj, dobj, omega;
@@ -56,11 +58,14 @@ constexpr int kStreamNnz = 1024;
 // stream kernel, and the two kernels over the sliced layout, gathering
 // from global memory (sliced) or from a copy in LDS (staged).
 constexpr int kStreamKernel = 0, kSlicedKernel = -1, kStagedKernel = -2;
-// Sliced layout: groups of kSliceGroup consecutive rows of A (kSliceGroupT
-// of A^T), one group per work item; inside a group the rows are sorted by
-// length and cut into slices of kSliceRows (one row per lane of a
-// wavefront).
-constexpr int kSliceRows = 64, kSliceGroup = 1024, kSliceGroupT = 2048;
+// Sliced layout: groups of consecutive rows, one group per work item;
+// inside a group the rows are sorted by length and cut into slices of
+// kSliceRows (one row per lane of a wavefront).  A takes groups of
+// kSliceGroup rows, or kSliceGroupWide for an LP whose groups of
+// kSliceGroup rows already reach across more than half its columns (then a
+// group's column window hardly grows with the group, and the larger group
+// halves the window staged per nonzero); A^T takes kSliceGroupT.
+constexpr int kSliceRows = 64, kSliceGroup = 1024, kSliceGroupWide = 2048, kSliceGroupT = 2048;
 // Largest column window (floats) the staged kernel holds in LDS, and the
 // largest any sliced kernel takes (its indexes are 16-bit).
 constexpr int kStagedMaxVec = 15 * 1024, kSlicedMaxWin = 65536;
@@ -155,6 +160,20 @@ struct pm_batch {
     // autotuner finds it faster: items of uneven cost then balance as the
     // hardware starts the queued workgroups); the tail kernels use 1.
     int mult_a = 1, mult_at = 1, mode_mult_a = 1, mode_mult_at = 1;
+    // Size classes of the iterate launches.  With nbuckets > 1 the LPs of
+    // class c (BatchDesc::bucket) get their own work lists (items from
+    // off_r[c] / off_c[c] in work_r / work_c, counts in wcnt[4 + 2c] and
+    // wcnt[5 + 2c]), their own launch per half iteration and their own
+    // kernel (blanes_*, bmult_*): items of one class cost about the same,
+    // so a persistent grid's static assignment stays balanced.  1: one
+    // launch over every LP.  The tail kernels always run over every LP.
+    static constexpr int kBuckets = 3;
+    int nbuckets = 1;
+    bool mode_bucketed = false;
+    int bucket_lps[kBuckets] = {0, 0, 0};
+    int64_t off_r[kBuckets] = {0, 0, 0}, off_c[kBuckets] = {0, 0, 0};
+    int blanes_a[kBuckets] = {8, 8, 8}, blanes_at[kBuckets] = {8, 8, 8};
+    int bmult_a[kBuckets] = {1, 1, 1}, bmult_at[kBuckets] = {1, 1, 1};
     int chunks_per_graph = 4;
     bool use_graph = true;
     pm::DevBuf<char> state;

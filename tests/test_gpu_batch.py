@@ -63,9 +63,10 @@ def test_batch_direct_launches_match_graph(capi):
 def test_batch_sliced_kernels_match_one_lane_per_row(capi):
     """The sliced kernels (global and LDS gathers) sum every row in CSR
     order with FMA, as the one-lane-per-row kernel does, so whole solves
-    give the same bits; the batch mixes full groups (1024 rows of A, 2048
-    of A^T, walked in two passes of 16 slices) with partial groups and
-    slices, an empty row and a 1500-entry row."""
+    give the same bits; the batch mixes full groups (2048 rows of A for the
+    2300-row LP, whose random columns make its groups wide; 2048 of A^T;
+    walked in passes of 16 slices) with partial groups and slices, an
+    empty row and a 1500-entry row."""
     lps = small_batch(capi, count=5, seed0=1700)
     lps.append(capi.generate(None, seed=1705, m=2300, n=5750, k_min=4, k_max=16))
     nd = 1500
@@ -81,6 +82,25 @@ def test_batch_sliced_kernels_match_one_lane_per_row(capi):
             assert a.status == c.status == "OPTIMAL"
             assert a.iterations == c.iterations and a.objective == c.objective, variant
             assert np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y), variant
+
+
+def test_batch_size_class_launches_match_one_launch(capi):
+    """One iterate launch per size class (m < 256, m < 1024, larger) and one
+    launch over every LP run the same kernel on the same work items, so a
+    batch with LPs of all three classes gives the same bits either way."""
+    lps = small_batch(capi, count=10, seed0=2100)
+    lps.append(capi.generate(None, seed=2111, m=1500, n=3750, k_min=4, k_max=16))
+    out = {}
+    for launches in (1, 3):
+        with capi.Batch(lps, tolerance=1e-7, max_iterations=60000, check_frequency=64, spmv_variant="staged") as b:
+            b.set_size_classes(launches)
+            info = b.size_classes()
+            out[launches], _, _ = b.run(want_vectors=True)
+        assert info["launches"] == launches
+    assert all(k > 0 for k in info["lps"]), info
+    for a, c in zip(out[1], out[3]):
+        assert a.status == c.status == "OPTIMAL"
+        assert a.iterations == c.iterations and np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y)
 
 
 def test_batch_device_scaling_matches_restatement(capi, ruiz_restated):
