@@ -502,31 +502,43 @@ bool build_sell(pm_ctx* c, int nrows, const int* rp, const int* ci, const float*
 }
 
 // Times a sliced ELL product y = S x (mean of 5 after 2 warm-ups).
-float time_sell(pm_ctx* c, const SellMat& S, const float* x, float* y) {
-    EpiStore<float> epi{y};
-    for (int r = 0; r < 2; ++r) launch_sell_mat(S, x, epi, c->stream);
+// Mean time of 5 launches of `go` after warm-ups.  A candidate whose first
+// timed launch already takes more than 4x `bound` (the time to beat) is
+// dropped with that figure: it cannot win, and on skewed rows a losing
+// layout can take milliseconds a launch.
+template <class F>
+float time_launches(pm_ctx* c, const F& go, float bound) {
+    float ms = 0.0f;
+    go();
     PM_HIP(hipEventRecord(c->ev0, c->stream));
-    for (int r = 0; r < 5; ++r) launch_sell_mat(S, x, epi, c->stream);
+    go();
     PM_HIP(hipEventRecord(c->ev1, c->stream));
     PM_HIP(hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
+    PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (ms > 4.0f * bound) return ms;
+    PM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < 5; ++r) go();
+    PM_HIP(hipEventRecord(c->ev1, c->stream));
+    PM_HIP(hipEventSynchronize(c->ev1));
     PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     return ms / 5.0f;
 }
 
-float time_merge(pm_ctx* c, int nrows, const int* rp, const int* ci, const float* v, const float* x,
-                 const int* part, const int4* longs, int nlong, float* y) {
+float time_sell(pm_ctx* c, const SellMat& S, const float* x, float* y, float bound = 1e30f) {
     EpiStore<float> epi{y};
-    for (int r = 0; r < 2; ++r)
-        launch_merge(nrows, c->nnz, rp, ci, v, x, epi, part, longs, nlong, c->mcarry.get(), c->mhead.get(), c->stream);
-    PM_HIP(hipEventRecord(c->ev0, c->stream));
-    for (int r = 0; r < 5; ++r)
-        launch_merge(nrows, c->nnz, rp, ci, v, x, epi, part, longs, nlong, c->mcarry.get(), c->mhead.get(), c->stream);
-    PM_HIP(hipEventRecord(c->ev1, c->stream));
-    PM_HIP(hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
-    PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    return ms / 5.0f;
+    return time_launches(c, [&] { launch_sell_mat(S, x, epi, c->stream); }, bound);
+}
+
+float time_merge(pm_ctx* c, int nrows, const int* rp, const int* ci, const float* v, const float* x,
+                 const int* part, const int4* longs, int nlong, float* y, float bound = 1e30f) {
+    EpiStore<float> epi{y};
+    return time_launches(
+        c,
+        [&] {
+            launch_merge(nrows, c->nnz, rp, ci, v, x, epi, part, longs, nlong, c->mcarry.get(), c->mhead.get(),
+                         c->stream);
+        },
+        bound);
 }
 
 // Builds the column-blocked copy of the scaled A with nblk blocks (the
@@ -566,30 +578,16 @@ void build_blocks(pm_ctx* c, const pm_lp* lp, int nblk) {
 }
 
 // Times the column-blocked A x (mean of 5 after 2 warm-ups).
-float time_blocked(pm_ctx* c, int nblk, int var, const float* x, float* y) {
+float time_blocked(pm_ctx* c, int nblk, int var, const float* x, float* y, float bound = 1e30f) {
     EpiStore<float> epi{y};
-    for (int r = 0; r < 2; ++r) launch_blocked(c, nblk, var, x, epi, c->stream);
-    PM_HIP(hipEventRecord(c->ev0, c->stream));
-    for (int r = 0; r < 5; ++r) launch_blocked(c, nblk, var, x, epi, c->stream);
-    PM_HIP(hipEventRecord(c->ev1, c->stream));
-    PM_HIP(hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
-    PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    return ms / 5.0f;
+    return time_launches(c, [&] { launch_blocked(c, nblk, var, x, epi, c->stream); }, bound);
 }
 
-// Times one SpMV variant (mean of 5 launches after 2 warm-ups).
+// Times one row-kernel SpMV variant.
 float time_variant(pm_ctx* c, int variant, int nrows, const int* rp, const int* ci, const float* v,
-                   const float* x, float* y) {
+                   const float* x, float* y, float bound = 1e30f) {
     EpiStore<float> epi{y};
-    for (int r = 0; r < 2; ++r) launch_rows<float>(variant, nrows, rp, ci, v, x, epi, c->stream);
-    PM_HIP(hipEventRecord(c->ev0, c->stream));
-    for (int r = 0; r < 5; ++r) launch_rows<float>(variant, nrows, rp, ci, v, x, epi, c->stream);
-    PM_HIP(hipEventRecord(c->ev1, c->stream));
-    PM_HIP(hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
-    PM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    return ms / 5.0f;
+    return time_launches(c, [&] { launch_rows<float>(variant, nrows, rp, ci, v, x, epi, c->stream); }, bound);
 }
 
 int autotune(pm_ctx* c, int nrows, int ncols, int nnz, const int* rp, const int* ci, const float* v,
@@ -611,7 +609,7 @@ int autotune(pm_ctx* c, int nrows, int ncols, int nnz, const int* rp, const int*
     for (int v_ : cands) {
         const int lanes = variant_lanes(v_);
         if (lanes > 2 * avg + 2 && lanes > 2) continue;  // mostly idle lanes
-        const float ms = time_variant(c, v_, nrows, rp, ci, v, x, y);
+        const float ms = time_variant(c, v_, nrows, rp, ci, v, x, y, best_ms);
         if (ms < best_ms) { best_ms = ms; best = v_; }
     }
     std::lock_guard<std::mutex> g(g_tune_mu);
@@ -856,12 +854,12 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
             if (build_sell(c, n, cp.data(), ri.data(), c->vat.get(), c->sell_at) && !want_sell) {
                 const float tr = time_variant(c, c->var_at, n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(),
                                               c->tmp_n.get());
-                if (time_sell(c, c->sell_at, c->y.get(), c->tmp_n.get()) >= tr) c->sell_at.reset();
+                if (time_sell(c, c->sell_at, c->y.get(), c->tmp_n.get(), tr) >= tr) c->sell_at.reset();
             }
             if (build_sell(c, m, lp->row_ptr, lp->col_idx, c->va.get(), c->sell_a) && !want_sell) {
                 const float tr = time_variant(c, c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), xs,
                                               c->tmp_m.get());
-                const float ts = time_sell(c, c->sell_a, xs, c->tmp_m.get());
+                const float ts = time_sell(c, c->sell_a, xs, c->tmp_m.get(), tr);
                 t_a = std::min(tr, ts);
                 if (ts >= tr) c->sell_a.reset();
             }
@@ -884,22 +882,22 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
             c->nlong_at = static_cast<int>(lat.size());
             c->mlong_a.upload(la, s);
             c->mlong_at.upload(lat, s);
-            const float tm_at = time_merge(c, n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(),
-                                           c->mpart_at.get(), c->mlong_at.get(), c->nlong_at, c->tmp_n.get());
             const float tc_at = want_merge ? 1e30f
                                 : c->sell_at.slices > 0 ? time_sell(c, c->sell_at, c->y.get(), c->tmp_n.get())
                                                         : time_variant(c, c->var_at, n, c->cp.get(), c->ri.get(),
                                                                        c->vat.get(), c->y.get(), c->tmp_n.get());
+            const float tm_at = time_merge(c, n, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(),
+                                           c->mpart_at.get(), c->mlong_at.get(), c->nlong_at, c->tmp_n.get(), tc_at);
             if (tm_at < tc_at) {
                 c->merge_at = true;
                 c->sell_at.reset();
             }
-            const float tm_a = time_merge(c, m, c->rp.get(), c->ci.get(), c->va.get(), xs, c->mpart_a.get(),
-                                          c->mlong_a.get(), c->nlong_a, c->tmp_m.get());
             const float tc_a = want_merge ? 1e30f
                                : t_a >= 0.0f ? t_a
                                              : time_variant(c, c->var_a, m, c->rp.get(), c->ci.get(), c->va.get(), xs,
                                                             c->tmp_m.get());
+            const float tm_a = time_merge(c, m, c->rp.get(), c->ci.get(), c->va.get(), xs, c->mpart_a.get(),
+                                          c->mlong_a.get(), c->nlong_a, c->tmp_m.get(), tc_a);
             if (tm_a < tc_a) {
                 c->merge_a = true;
                 c->sell_a.reset();
@@ -931,7 +929,7 @@ void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
             for (int nb : counts) {
                 build_blocks(c, lp, nb);
                 for (int v_ : {PM_SPMV_VECTOR2, PM_SPMV_VECTOR4, PM_SPMV_VECTOR8}) {
-                    const float ms = time_blocked(c, nb, v_, xs, c->tmp_m.get());
+                    const float ms = time_blocked(c, nb, v_, xs, c->tmp_m.get(), best);
                     if (ms < best) { best = ms; best_nb = nb; best_var = v_; }
                 }
             }

@@ -36,6 +36,37 @@ __global__ __launch_bounds__(kBlock) void k_row_norm(int nrows, const int* __res
     out[i] = a > 0.0 ? 1.0 / sqrt(a) : 1.0;
 }
 
+// The same infinity norms with the rows of more than kLongRow entries
+// walked by the whole wavefront (a maximum is exact in any order, so the
+// factors are those of the sequential loop): one lane per row otherwise.
+constexpr int kLongRow = 256;
+
+__global__ __launch_bounds__(kBlock) void k_row_norm_max(int nrows, const int* __restrict__ rp,
+                                                         const double* __restrict__ v, double* __restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    int b = 0, e = 0;
+    if (i < nrows) {
+        b = rp[i];
+        e = rp[i + 1];
+    }
+    const bool lng = e - b > kLongRow;
+    double a = 0.0;
+    if (!lng)
+        for (int k = b; k < e; ++k) a = fmax(a, fabs(v[k]));
+    uint64_t mask = __ballot(lng);  // the wavefront's long rows, one at a time
+    while (mask) {
+        const int src = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        mask &= mask - 1;
+        const int bb = __shfl(b, src), ee = __shfl(e, src);
+        double mx = 0.0;
+        for (int k = bb + lane; k < ee; k += 64) mx = fmax(mx, fabs(v[k]));
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        if (lane == src) a = mx;
+    }
+    if (i < nrows) out[i] = a > 0.0 ? 1.0 / sqrt(a) : 1.0;
+}
+
 // v[k] *= rs[row] * cs[col] over one CSR order, L lanes per row striding
 // over its entries (each entry is scaled on its own, so the lanes change
 // nothing but the access pattern); acc[row] *= rs[row].
@@ -101,9 +132,14 @@ inline void ruiz_pc_device(hipStream_t s, int passes, int m, int n, int64_t nnz,
     if (m > 0) hipLaunchKernelGGL(k_fill_f64, grid_for(m), b, 0, s, static_cast<int64_t>(m), dr, 1.0);
     if (n > 0) hipLaunchKernelGGL(k_fill_f64, grid_for(n), b, 0, s, static_cast<int64_t>(n), dc, 1.0);
     for (int pass = 0; pass <= passes; ++pass) {
-        const bool sum = pass == passes;  // the last pass is Pock-Chambolle
-        if (m > 0) hipLaunchKernelGGL(k_row_norm, grid_for(m), b, 0, s, m, rp, vs, sum, rn);
-        if (n > 0) hipLaunchKernelGGL(k_row_norm, grid_for(n), b, 0, s, n, cp, vst, sum, cn);
+        const bool sum = pass == passes;  // the last pass is Pock-Chambolle (sums, in CSR / CSC order)
+        if (sum) {
+            if (m > 0) hipLaunchKernelGGL(k_row_norm, grid_for(m), b, 0, s, m, rp, vs, true, rn);
+            if (n > 0) hipLaunchKernelGGL(k_row_norm, grid_for(n), b, 0, s, n, cp, vst, true, cn);
+        } else {
+            if (m > 0) hipLaunchKernelGGL(k_row_norm_max, grid_for(m), b, 0, s, m, rp, vs, rn);
+            if (n > 0) hipLaunchKernelGGL(k_row_norm_max, grid_for(n), b, 0, s, n, cp, vst, cn);
+        }
         if (m > 0 && n > 0) {
             row_scale(s, la, m, rp, ci, vs, rn, cn, dr);
             row_scale(s, lt, n, cp, ri, vst, cn, rn, dc);

@@ -107,14 +107,15 @@ def test_batch_device_scaling_matches_restatement(capi, ruiz_restated):
     """The batch scales its block-diagonal concatenation on the device; each
     LP's factors equal the plain-loop restatement of that LP alone, bit for
     bit (an LP without rows, an LP with an empty row and column, mixed
-    sizes)."""
+    sizes, rows of 500 entries reduced by a whole wavefront)."""
     from pramaan_amd.mps import read_mps
     lps = [capi.generate(None, seed=31, m=120, n=300, eq_frac=0.6, noise=0.05),
            read_mps(GOLDEN / "afiro.mps"),
            capi.LP.from_dense(np.zeros((0, 3)), [1.0, -1.0, 2.0], [], [], np.zeros(3), np.ones(3)),
            capi.LP.from_dense([[2.0, 0.0, -4.0], [0.0, 0.0, 0.0], [0.5, 0.0, 8.0]], [1.0, 0.0, 1.0],
                               [1.0, -1.0, 0.0], [4.0, 1.0, 9.0], np.zeros(3), np.full(3, 5.0)),
-           capi.generate(None, seed=32, m=700, n=1750, k_min=4, k_max=16)]
+           capi.generate(None, seed=32, m=700, n=1750, k_min=4, k_max=16),
+           capi.with_dense_rows(capi.generate(None, seed=33, m=200, n=500, k_min=4, k_max=16), 2)]
     with capi.Batch(lps, tolerance=1e-6, check_frequency=64) as b:
         for i, lp in enumerate(lps):
             dr, dc = b.scaling(i)

@@ -141,13 +141,17 @@ def test_pdlp_warm_start_from_optimum(capi):
     assert warm.iterations <= cold.iterations
 
 
-@pytest.mark.parametrize("which", ["generated", "afiro"])
+@pytest.mark.parametrize("which", ["generated", "afiro", "dense_rows"])
 def test_device_scaling_matches_restatement(capi, which, ruiz_restated):
     """The device Ruiz + Pock-Chambolle factors equal a plain-loop
-    restatement bit for bit (row and column norms in the same order)."""
+    restatement bit for bit (row and column norms in the same order; rows
+    of more than 256 entries, here 3 rows of 750, are reduced by a whole
+    wavefront in the maximum passes)."""
     if which == "afiro":
         from pramaan_amd.mps import read_mps
         lp = read_mps(GOLDEN / "afiro.mps")
+    elif which == "dense_rows":
+        lp = capi.with_dense_rows(capi.generate(None, seed=21, m=300, n=750, eq_frac=0.6, noise=0.05), 3)
     else:
         lp = capi.generate(None, seed=21, m=300, n=750, eq_frac=0.6, noise=0.05)
     ctx = capi.Context(0)
