@@ -189,6 +189,16 @@ inline int group_rows_a(int m, int n, const int* rp, const int* ci) {
     return width > 0.5 * static_cast<double>(n) * groups ? kSliceGroupWide : kSliceGroup;
 }
 
+// Rows per group when `rows` rows are cut into groups of at most `group`
+// rows of equal size (a multiple of the slice height): the same number of
+// groups as plain cuts of `group` rows, without a small last one.
+inline int balanced_group(int rows, int group) {
+    if (rows <= 0) return group;
+    const int count = (rows + group - 1) / group;
+    const int g = (rows + count - 1) / count;
+    return (g + kSliceRows - 1) / kSliceRows * kSliceRows;
+}
+
 // Size class of an LP for the per-class iterate launches: fewer than 256
 // rows, fewer than 1,024 (one group of A), or more.
 inline int size_class(int m) { return m < 256 ? 0 : (m < kSliceGroup ? 1 : 2); }
@@ -1456,9 +1466,9 @@ void batch_create(pm_batch* b, int count, const pm_lp* lps, const pm_pdhg_option
         d.lanes_at = lanes_for(static_cast<double>(nnz) / n);
         stream_segments(m, L.row_ptr, seg_a_lp[k]);
         stream_segments(n, cp.data(), seg_at_lp[k]);
-        d.ga = group_rows_a(m, n, L.row_ptr, L.col_idx);
+        d.ga = balanced_group(m, group_rows_a(m, n, L.row_ptr, L.col_idx));
         d.bucket = size_class(m);
-        d.gt = kSliceGroupT;
+        d.gt = balanced_group(n, kSliceGroupT);
         build_sliced(m, L.row_ptr, L.col_idx, d.nz_off, sl_a_lp[k], d.ga);
         build_sliced(n, cp.data(), ri.data(), d.nz_off, sl_at_lp[k], d.gt);
     };

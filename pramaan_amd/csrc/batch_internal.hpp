@@ -64,7 +64,9 @@ constexpr int kStreamKernel = 0, kSlicedKernel = -1, kStagedKernel = -2;
 // kSliceGroup rows, or kSliceGroupWide for an LP whose groups of
 // kSliceGroup rows already reach across more than half its columns (then a
 // group's column window hardly grows with the group, and the larger group
-// halves the window staged per nonzero); A^T takes kSliceGroupT.
+// halves the window staged per nonzero); A^T takes kSliceGroupT.  Each
+// LP's groups are then balanced: the same count, equal sizes rounded up to
+// kSliceRows (balanced_group), no small last group.
 constexpr int kSliceRows = 64, kSliceGroup = 1024, kSliceGroupWide = 2048, kSliceGroupT = 2048;
 // Largest column window (floats) the staged kernel holds in LDS, and the
 // largest any sliced kernel takes (its indexes are 16-bit).
