@@ -190,13 +190,14 @@ inline int group_rows_a(int m, int n, const int* rp, const int* ci) {
 }
 
 // Rows per group when `rows` rows are cut into groups of at most `group`
-// rows of equal size (a multiple of the slice height): the same number of
-// groups as plain cuts of `group` rows, without a small last one.
+// rows of about equal size (a multiple of the slice height): the fewest
+// groups whose last one keeps at least half a group.
 inline int balanced_group(int rows, int group) {
-    if (rows <= 0) return group;
-    const int count = (rows + group - 1) / group;
-    const int g = (rows + count - 1) / count;
-    return (g + kSliceRows - 1) / kSliceRows * kSliceRows;
+    if (rows <= group) return group;
+    for (int count = (rows + group - 1) / group;; ++count) {
+        const int g = ((rows + count - 1) / count + kSliceRows - 1) / kSliceRows * kSliceRows;
+        if (g <= kSliceRows || 2 * (rows - (count - 1) * g) >= g) return g;
+    }
 }
 
 // Size class of an LP for the per-class iterate launches: fewer than 256
