@@ -1,4 +1,5 @@
-"""Free-format MPS reader producing the engine's LP (ModelIR layout).
+"""Free-format MPS reader producing the engine's LP (ModelIR layout), and
+the matching writer.
 
 Follows the reference's parse_mps (src/io/mps_parser.cpp:75-490): ROWS
 N/L/G/E (the first N row is the objective, later N rows become free rows),
@@ -129,3 +130,62 @@ def read_mps(path: str | Path) -> LP:
     ci = np.array([j for (_i, j) in keys], np.int32)
     vals = np.array([trip[k] for k in keys], float)
     return LP(c, rp, ci, vals, rl, ru, xl, xu, 0, offset)
+
+
+def write_mps(lp: LP, path: str | Path, name: str = "LP") -> None:
+    """Writes lp as free-format MPS that read_mps and the reference's
+    parse_mps (src/io/mps_parser.cpp:75-490) read back: rows E (equal
+    bounds), L, G or N (free); bounds LO/UP/FX/FR/MI where they differ from
+    the default 0 <= x < +inf; a maximisation is written as the minimisation
+    of -c (the format has no sense section there).  A row with two different
+    finite bounds would need RANGES, which that parser rejects: MpsError."""
+    big = INF * 0.1
+    sign = -1.0 if lp.sense == 1 else 1.0
+    out = [f"NAME {name}", "ROWS", " N OBJ"]
+    kinds = []
+    for i in range(lp.m):
+        lo, hi = lp.row_lower[i], lp.row_upper[i]
+        has_lo, has_hi = lo > -big, hi < big
+        if has_lo and has_hi and lo != hi:
+            raise MpsError(f"row {i} has two different finite bounds (RANGES)")
+        kind = "E" if has_lo and has_hi else ("L" if has_hi else ("G" if has_lo else "N"))
+        kinds.append(kind)
+        out.append(f" {kind} R{i}")
+    out.append("COLUMNS")
+    cp = np.zeros(lp.n + 1, np.int64)
+    np.add.at(cp, lp.col_idx.astype(np.int64) + 1, 1)
+    cp = np.cumsum(cp)
+    order = np.argsort(lp.col_idx, kind="stable")
+    rows_of = np.repeat(np.arange(lp.m), np.diff(lp.row_ptr))
+    for j in range(lp.n):
+        if lp.c[j] != 0.0:
+            out.append(f" X{j} OBJ {float(sign * lp.c[j])!r}")
+        for q in order[cp[j]:cp[j + 1]]:
+            out.append(f" X{j} R{rows_of[q]} {float(lp.values[q])!r}")
+        if lp.c[j] == 0.0 and cp[j] == cp[j + 1]:
+            out.append(f" X{j} OBJ 0.0")
+    out.append("RHS")
+    for i, kind in enumerate(kinds):
+        v = lp.row_upper[i] if kind == "L" else (lp.row_lower[i] if kind in "EG" else 0.0)
+        if kind != "N" and v != 0.0:
+            out.append(f" RHS R{i} {float(v)!r}")
+    if sign * lp.offset != 0.0:
+        out.append(f" RHS OBJ {float(-sign * lp.offset)!r}")
+    out.append("BOUNDS")
+    for j in range(lp.n):
+        lo, hi = lp.var_lower[j], lp.var_upper[j]
+        has_lo, has_hi = lo > -big, hi < big
+        if has_lo and has_hi and lo == hi:
+            out.append(f" FX BND X{j} {float(lo)!r}")
+            continue
+        if not has_lo and not has_hi:
+            out.append(f" FR BND X{j}")
+            continue
+        if not has_lo:
+            out.append(f" MI BND X{j}")
+        elif lo != 0.0:
+            out.append(f" LO BND X{j} {float(lo)!r}")
+        if has_hi:
+            out.append(f" UP BND X{j} {float(hi)!r}")
+    out.append("ENDATA")
+    Path(path).write_text("\n".join(out) + "\n")
