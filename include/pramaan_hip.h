@@ -294,6 +294,11 @@ int pm_ctx_destroy(pm_ctx* ctx);
 int pm_ctx_load(pm_ctx* ctx, const pm_lp* lp, const pm_pdhg_options* opts);
 /* Solves the loaded LP from a cold start (or opts->x0/y0 given at load). */
 int pm_ctx_solve(pm_ctx* ctx, pm_pdhg_result* res);
+/* PM_ALG_PDLP on an LP below 2^18 nonzeros replays each full chunk of
+ * check_frequency iterations as one captured HIP graph (default 1).  use_graph = 0 issues the same launches
+ * directly, with identical results; for profilers that do not follow graph
+ * launches.  No counterpart in the reference. */
+int pm_ctx_set_graph(pm_ctx* ctx, int32_t use_graph);
 
 /* Kernel ids for pm_ctx_time_kernel. */
 #define PM_KERNEL_SPMV_A      0 /* A x-bar with the fused dual update       */

@@ -92,6 +92,7 @@ SIGNATURES = {
     "pm_ctx_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _d, _d]),
     "pm_ctx_scaling": (C.c_int, [C.c_void_p, _d, _d]),
     "pm_ctx_spmv_variants": (C.c_int, [C.c_void_p, _i, _i]),
+    "pm_ctx_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "pm_batch_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(pm_lp), C.POINTER(pm_pdhg_options),
                                   C.POINTER(C.c_void_p)]),
     "pm_batch_set_warm_start": (C.c_int, [C.c_void_p, C.c_int32, _d, _d]),
@@ -340,6 +341,11 @@ class Context:
             r.x, r.y = _p(x, C.c_double), _p(y, C.c_double)
         _check(lib().pm_ctx_solve(self._h, C.byref(r)))
         return _result(r, x, y, None)
+
+    def set_graph(self, use_graph: bool) -> None:
+        """pm_ctx_set_graph: replay full chunks as a captured graph (default)
+        or issue the same launches directly."""
+        _check(lib().pm_ctx_set_graph(self._h, 1 if use_graph else 0))
 
     def time_kernel(self, which: int, reps: int) -> tuple[float, float]:
         ms, b = C.c_double(), C.c_double()

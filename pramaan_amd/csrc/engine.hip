@@ -267,6 +267,13 @@ struct pm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Captured chunk of PDLP iterations (hal_chunk): the launches of
+    // chunk_len iterations whose epilogues read their step parameters from
+    // chunk_dp (written from chunk_hp before each replay).
+    hipGraph_t chunk_graph = nullptr;
+    hipGraphExec_t chunk_exec = nullptr;
+    int chunk_len = 0;
+    bool use_graph = true;
     pm_pdhg_options opts{};
     bool loaded = false;
 
@@ -291,6 +298,8 @@ struct pm_ctx {
 
     // FP32 working vectors.
     DevBuf<float> c32, lr32, ur32, lx32, ux32, sig32, tau32, g32;
+    DevBuf<float> chunk_dp;
+    PinnedBuf<float> chunk_hp;
     DevBuf<float> x, y, xe, x0, y0, xt, yt, xl, yl, tmp_n, tmp_m;
     // FP64 vectors.
     DevBuf<double> xtot, ytot, xbase, ybase, act64;
@@ -411,7 +420,8 @@ void launch_sell_mat(const SellMat& S, const float* x, const Epi& epi, hipStream
 }
 
 // The dual half of a PDLP iteration: A x-bar with the fused dual update.
-void launch_dual_half(pm_ctx* c, const EpiDualHal& ed, hipStream_t s) {
+template <class Epi>
+void launch_dual_half(pm_ctx* c, const Epi& ed, hipStream_t s) {
     if (c->nblk > 0) launch_blocked(c, c->nblk, c->blk_var, c->xe.get(), ed, s);
     else if (c->merge_a)
         launch_merge(c->m, c->nnz, c->rp.get(), c->ci.get(), c->va.get(), c->xe.get(), ed, c->mpart_a.get(),
@@ -421,7 +431,8 @@ void launch_dual_half(pm_ctx* c, const EpiDualHal& ed, hipStream_t s) {
 }
 
 // The primal half of a PDLP iteration: A^T y with the fused primal update.
-void launch_primal_half(pm_ctx* c, const EpiPrimalHal& ep, hipStream_t s) {
+template <class Epi>
+void launch_primal_half(pm_ctx* c, const Epi& ep, hipStream_t s) {
     if (c->merge_at)
         launch_merge(c->n, c->nnz, c->cp.get(), c->ri.get(), c->vat.get(), c->y.get(), ep, c->mpart_at.get(),
                      c->mlong_at.get(), c->nlong_at, c->mcarry.get(), c->mhead.get(), s);
@@ -661,12 +672,16 @@ void check_options(const pm_pdhg_options& opts) {
         fail(PM_ERR_INVALID_ARG, "PM_ALG_REFERENCE starts from x = clamp(0, l, u), y = 0; warm starts need PM_ALG_PDLP");
 }
 
+void drop_chunk_graph(pm_ctx* c);
+
 void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
     check_lp(lp);
     pm_pdhg_options opts;
     pm_pdhg_options_default(&opts);
     if (o) opts = *o;
     check_options(opts);
+    PM_HIP(hipStreamSynchronize(c->stream));
+    drop_chunk_graph(c);  // the layouts and buffers it names are rebuilt
     c->opts = opts;
     c->loaded = false;
     const int m = lp->m, n = lp->n, nnz = lp->nnz;
@@ -1253,6 +1268,67 @@ void hal_iteration(pm_ctx* c, int k, double omega, bool keep_t, bool use_g) {
     }
 }
 
+void drop_chunk_graph(pm_ctx* c) {
+    if (c->chunk_exec) (void)hipGraphExecDestroy(c->chunk_exec);
+    if (c->chunk_graph) (void)hipGraphDestroy(c->chunk_graph);
+    c->chunk_exec = nullptr;
+    c->chunk_graph = nullptr;
+    c->chunk_len = 0;
+}
+
+// `chunk` iterations from in-round index k0 (the last keeps T(z)).  On an
+// LP below 2^18 nonzeros a full chunk of K iterations replays a captured
+// graph whose epilogues
+// (EpiPrimalHalDev, EpiDualHalDev) read tau, sigma, the dual shift's
+// presence and each position's Halpern weight from chunk_dp, filled here
+// with the values hal_iteration would pass, so both paths give the same
+// bits; a shorter chunk launches directly.  The buffers the graph names
+// keep their addresses until the next load, which drops it.
+void hal_chunk(pm_ctx* c, int k0, double omega, bool use_g, int chunk, int K) {
+    // Graphs pay where launches cost as much as the kernels: LPs below the
+    // autotuner's 2^18 nonzeros (larger products hide the launches).
+    if (!c->use_graph || chunk != K || c->nnz >= (1 << 18)) {
+        for (int i = 0; i < chunk; ++i) hal_iteration(c, k0 + i, omega, i == chunk - 1, use_g);
+        return;
+    }
+    hipStream_t s = c->stream;
+    if (c->chunk_dp.size() < static_cast<size_t>(K) + 3) {
+        c->chunk_dp.resize(static_cast<size_t>(K) + 3);
+        c->chunk_hp.resize(static_cast<size_t>(K) + 3);
+    }
+    // the previous chunk's copy has completed: every check synchronises
+    float* hp = c->chunk_hp.get();
+    hp[0] = static_cast<float>(c->eta / omega);
+    hp[1] = static_cast<float>(c->eta * omega);
+    hp[2] = use_g ? 1.0f : 0.0f;
+    for (int i = 0; i < K; ++i) {
+        const int k = k0 + i;
+        hp[3 + i] = static_cast<float>((k + 1.0) / (k + 2.0));
+    }
+    PM_HIP(hipMemcpyAsync(c->chunk_dp.get(), hp, sizeof(float) * (static_cast<size_t>(K) + 3),
+                          hipMemcpyHostToDevice, s));
+    if (!c->chunk_exec || c->chunk_len != K) {
+        drop_chunk_graph(c);
+        PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < K; ++i) {
+            const bool keep = i == K - 1;
+            launch_primal_half(c,
+                               EpiPrimalHalDev{c->chunk_dp.get(), i, c->c32.get(), c->lx32.get(), c->ux32.get(),
+                                               c->x0.get(), c->x.get(), c->xe.get(), keep ? c->xt.get() : nullptr},
+                               s);
+            if (c->m > 0)
+                launch_dual_half(c,
+                                 EpiDualHalDev{c->chunk_dp.get(), i, c->lr32.get(), c->ur32.get(), c->g32.get(),
+                                               c->y0.get(), c->y.get(), keep ? c->yt.get() : nullptr},
+                                 s);
+        }
+        PM_HIP(hipStreamEndCapture(s, &c->chunk_graph));
+        PM_HIP(hipGraphInstantiate(&c->chunk_exec, c->chunk_graph, nullptr, nullptr, 0));
+        c->chunk_len = K;
+    }
+    PM_HIP(hipGraphLaunch(c->chunk_exec, s));
+}
+
 // One step of plain PDHG with PDLP's adaptive step size (PM_STEP_ADAPTIVE):
 // a step of size eta (primal eta / omega, dual eta omega) is kept when eta
 // <= eta_max = ||dz||^2_omega / (2 |dy^T A dx|), the largest step the
@@ -1554,11 +1630,11 @@ void pdlp_solve(pm_ctx* c, pm_pdhg_result* res) {
         const int chunk = ctl.chunk(K);
         if (chunk <= 0) break;
         int extra = 0;  // rejected adaptive steps, each an iteration's work
-        for (int i = 0; i < chunk; ++i) {
-            if (c->adaptive_step)
+        if (c->adaptive_step) {
+            for (int i = 0; i < chunk; ++i)
                 extra += adaptive_iteration(c, ctl.omega, i == chunk - 1, ctl.use_g, ctl.total_it + extra + i) - 1;
-            else
-                hal_iteration(c, ctl.k + i, ctl.omega, i == chunk - 1, ctl.use_g);
+        } else {
+            hal_chunk(c, ctl.k, ctl.omega, ctl.use_g, chunk, K);
         }
         ctl.advance(chunk);
         ctl.total_it += extra;
@@ -1748,11 +1824,19 @@ int pm_ctx_destroy(pm_ctx* c) {
         if (!c) return;
         (void)hipSetDevice(c->device);
         if (c->stream) (void)hipStreamSynchronize(c->stream);
+        pm::drop_chunk_graph(c);
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
         hipStream_t s = c->stream;
         delete c;
         if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int pm_ctx_set_graph(pm_ctx* c, int32_t use_graph) {
+    return pm::guarded([&] {
+        if (!c) pm::fail(PM_ERR_INVALID_ARG, "ctx is NULL");
+        c->use_graph = use_graph != 0;
     });
 }
 

@@ -353,6 +353,41 @@ struct EpiDualHal {
     __device__ void operator()(int i, float ax) const { apply(i, ax, load(i)); }
 };
 
+// The same updates with the step sizes, the dual shift's presence and the
+// Halpern weight read from device memory when the kernel runs: dp[0] tau,
+// dp[1] sigma, dp[2] 1 with a dual shift (else 0), dp[3 + pos] the weight
+// of position pos of a chunk.  One captured chunk of launches then serves
+// every chunk of a solve; the arithmetic is that of EpiPrimalHal /
+// EpiDualHal with rho = 1.
+struct EpiPrimalHalDev {
+    const float* dp;
+    int pos;
+    const float* c;
+    const float* lo;
+    const float* hi;
+    const float* x0;
+    float* x;
+    float* xe;
+    float* xt_out;
+    __device__ void operator()(int j, float aty) const {
+        EpiPrimalHal{dp[0], dp[3 + pos], 1.0f, c, lo, hi, x0, x, xe, xt_out}(j, aty);
+    }
+};
+
+struct EpiDualHalDev {
+    const float* dp;
+    int pos;
+    const float* lo;
+    const float* hi;
+    const float* g;
+    const float* y0;
+    float* y;
+    float* yt_out;
+    __device__ void operator()(int i, float ax) const {
+        EpiDualHal{dp[1], dp[3 + pos], 1.0f, lo, hi, dp[2] != 0.0f ? g : nullptr, y0, y, yt_out}(i, ax);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // FP64 checks
 // ---------------------------------------------------------------------------

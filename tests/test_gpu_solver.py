@@ -160,3 +160,36 @@ def test_device_scaling_matches_restatement(capi, which, ruiz_restated):
     ctx.close()
     wr, wc = ruiz_restated(lp)
     assert np.array_equal(dr, wr) and np.array_equal(dc, wc), (np.abs(dr - wr).max(), np.abs(dc - wc).max())
+
+
+@pytest.mark.parametrize("case", ["config0", "window", "afiro", "dense_rows", "merge"])
+def test_graph_chunks_match_direct_launches(capi, case):
+    """Full chunks of check_frequency iterations replayed as one captured
+    graph (epilogues reading tau, sigma, the dual shift's presence and the
+    Halpern weights from device memory) give the same bits as the same
+    launches issued directly: refinement rounds, restarts and (on the
+    window) feasibility polishing included."""
+    kw = dict(algorithm="pdlp", termination="relative_kkt", tolerance=1e-6, max_iterations=200000,
+              check_frequency=64)
+    if case == "config0":
+        lp, kw["tolerance"] = capi.generate(0, seed=1), 1e-8
+    elif case == "window":
+        lp = capi.RollingHorizon().window(166)
+    elif case == "afiro":
+        from pramaan_amd.mps import read_mps
+        lp, kw["tolerance"] = read_mps(GOLDEN / "afiro.mps"), 1e-8
+    elif case == "dense_rows":
+        lp, kw["max_iterations"] = capi.with_dense_rows(capi.generate(2, seed=1), 16), 1000
+    else:
+        lp, kw["spmv_variant"] = capi.generate(0, seed=2), "merge"
+    out = {}
+    for use_graph in (True, False):
+        with capi.Context(0) as ctx:
+            ctx.set_graph(use_graph)
+            ctx.load(lp, **kw)
+            out[use_graph] = ctx.solve()
+    a, b = out[True], out[False]
+    print(f"{case}: {a.status} it={a.iterations} graph {a.device_seconds:.4f}s direct {b.device_seconds:.4f}s",
+          flush=True)
+    assert a.status == b.status and a.iterations == b.iterations and a.restarts == b.restarts
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
