@@ -273,6 +273,11 @@ struct pm_ctx {
     hipGraph_t chunk_graph = nullptr;
     hipGraphExec_t chunk_exec = nullptr;
     int chunk_len = 0;
+    // The same for the reference algorithm's iterations (ref_chunk), whose
+    // launches take no per-iteration parameters.
+    hipGraph_t ref_graph = nullptr;
+    hipGraphExec_t ref_exec = nullptr;
+    int ref_len = 0;
     bool use_graph = true;
     pm_pdhg_options opts{};
     bool loaded = false;
@@ -672,7 +677,19 @@ void check_options(const pm_pdhg_options& opts) {
         fail(PM_ERR_INVALID_ARG, "PM_ALG_REFERENCE starts from x = clamp(0, l, u), y = 0; warm starts need PM_ALG_PDLP");
 }
 
-void drop_chunk_graph(pm_ctx* c);
+// Longest chunk captured as a graph (two or more launches per iteration).
+constexpr int kGraphMaxChunk = 512;
+
+// Drops the captured iteration graphs (hal_chunk, ref_chunk).
+void drop_chunk_graph(pm_ctx* c) {
+    if (c->chunk_exec) (void)hipGraphExecDestroy(c->chunk_exec);
+    if (c->chunk_graph) (void)hipGraphDestroy(c->chunk_graph);
+    if (c->ref_exec) (void)hipGraphExecDestroy(c->ref_exec);
+    if (c->ref_graph) (void)hipGraphDestroy(c->ref_graph);
+    c->chunk_exec = c->ref_exec = nullptr;
+    c->chunk_graph = c->ref_graph = nullptr;
+    c->chunk_len = c->ref_len = 0;
+}
 
 void ctx_load(pm_ctx* c, const pm_lp* lp, const pm_pdhg_options* o) {
     check_lp(lp);
@@ -1036,6 +1053,28 @@ void ref_iteration(pm_ctx* c) {
                        s);
 }
 
+// `len` reference iterations; on an LP below 2^18 nonzeros a run of K
+// (check_frequency) replays one captured graph of the same launches.
+void ref_chunk(pm_ctx* c, int len, int K) {
+    if (!c->use_graph || len != K || K > kGraphMaxChunk || c->nnz >= (1 << 18)) {
+        for (int i = 0; i < len; ++i) ref_iteration(c);
+        return;
+    }
+    hipStream_t s = c->stream;
+    if (!c->ref_exec || c->ref_len != K) {
+        if (c->ref_exec) (void)hipGraphExecDestroy(c->ref_exec);
+        if (c->ref_graph) (void)hipGraphDestroy(c->ref_graph);
+        c->ref_exec = nullptr;
+        c->ref_graph = nullptr;
+        PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < K; ++i) ref_iteration(c);
+        PM_HIP(hipStreamEndCapture(s, &c->ref_graph));
+        PM_HIP(hipGraphInstantiate(&c->ref_exec, c->ref_graph, nullptr, nullptr, 0));
+        c->ref_len = K;
+    }
+    PM_HIP(hipGraphLaunch(c->ref_exec, s));
+}
+
 void finish_result(pm_ctx* c, pm_pdhg_result* res, const double* xd_dev, const double* yd_dev,
                    std::vector<double>& xh, std::vector<double>& yh);
 
@@ -1055,18 +1094,22 @@ void ref_solve(pm_ctx* c, pm_pdhg_result* res) {
     const int kRestP = 8, kMaxR = 10;
     res->status = PM_SOLVE_ITERATION_LIMIT;
     const auto t0 = std::chrono::steady_clock::now();
-    for (int iter = 0; iter < o.max_iterations; ++iter) {
-        ref_iteration(c);
-        if ((iter + 1) % o.check_frequency == 0) {
+    const int K = o.check_frequency;
+    for (int done = 0; done < o.max_iterations;) {
+        // iterations up to the next check (every K-th iteration), or the end
+        const int next = std::min(o.max_iterations, (done / K + 1) * K);
+        ref_chunk(c, next - done, K);
+        done = next;
+        if (done % K == 0) {
             double pv = 0, dv = 0;
             ref_residuals(c, o.tolerance, pv, dv);
             res->primal_residual = pv;
             res->dual_residual = dv;
             if (o.verbose)
-                fprintf(stderr, "[pramaan] iter %d pv %.3e dv %.3e restarts %d\n", iter + 1, pv, dv, restarts);
+                fprintf(stderr, "[pramaan] iter %d pv %.3e dv %.3e restarts %d\n", done, pv, dv, restarts);
             if (pv <= o.tolerance && dv <= o.tolerance) {
                 res->status = PM_SOLVE_OPTIMAL;
-                res->iterations = iter + 1;
+                res->iterations = done;
                 break;
             }
             const double combined = pv + dv;
@@ -1268,13 +1311,7 @@ void hal_iteration(pm_ctx* c, int k, double omega, bool keep_t, bool use_g) {
     }
 }
 
-void drop_chunk_graph(pm_ctx* c) {
-    if (c->chunk_exec) (void)hipGraphExecDestroy(c->chunk_exec);
-    if (c->chunk_graph) (void)hipGraphDestroy(c->chunk_graph);
-    c->chunk_exec = nullptr;
-    c->chunk_graph = nullptr;
-    c->chunk_len = 0;
-}
+
 
 // `chunk` iterations from in-round index k0 (the last keeps T(z)).  On an
 // LP below 2^18 nonzeros a full chunk of K iterations replays a captured
@@ -1287,7 +1324,7 @@ void drop_chunk_graph(pm_ctx* c) {
 void hal_chunk(pm_ctx* c, int k0, double omega, bool use_g, int chunk, int K) {
     // Graphs pay where launches cost as much as the kernels: LPs below the
     // autotuner's 2^18 nonzeros (larger products hide the launches).
-    if (!c->use_graph || chunk != K || c->nnz >= (1 << 18)) {
+    if (!c->use_graph || chunk != K || K > kGraphMaxChunk || c->nnz >= (1 << 18)) {
         for (int i = 0; i < chunk; ++i) hal_iteration(c, k0 + i, omega, i == chunk - 1, use_g);
         return;
     }
@@ -1432,7 +1469,7 @@ bool polish_phase(pm_ctx* c, Control& ctl, Polish& pol, int phase, double scale,
     for (;;) {
         const int chunk = ctl.chunk(K);
         if (chunk <= 0) return false;
-        for (int i = 0; i < chunk; ++i) hal_iteration(c, pol.k + i, ctl.omega, i == chunk - 1, use_g);
+        hal_chunk(c, pol.k, ctl.omega, use_g, chunk, K);
         ctl.total_it += chunk;
         total_point(c, scale, scale);
         const Kkt kk = eval_kkt(c, c->xtot.get(), c->ytot.get());
