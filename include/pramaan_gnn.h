@@ -23,6 +23,7 @@
 #ifndef PRAMAAN_GNN_H
 #define PRAMAAN_GNN_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -63,13 +64,23 @@ int32_t pm_gnn_num_params(void);
 /* Creates a network with deterministic scaled-uniform initial weights. */
 int pm_gnn_create(int32_t device, uint64_t seed, pm_gnn** out);
 int pm_gnn_destroy(pm_gnn* g);
+/* Page-locked host memory (no counterpart in the reference; for callers
+ * that build graphs for pm_gnn_set_graph and want their copies to run at
+ * full link speed).  pm_host_free(NULL) does nothing. */
+int pm_host_alloc(size_t bytes, void** out);
+int pm_host_free(void* p);
 int pm_gnn_get_params(pm_gnn* g, float* out);
 int pm_gnn_set_params(pm_gnn* g, const float* in);  /* also resets Adam */
 
 /* The graph of one batch of LPs (block-diagonal if several): Ar = A with
  * rows scaled to unit max-norm, in CSR (rp[m+1], ci, ar); Ac = A^T with
  * rows (= columns of A) scaled to unit max-norm, in CSR (cp[n+1], ri, ac);
- * row features xr[m][8], column features xc[n][8].  Copied to the device. */
+ * row features xr[m][PM_GNN_FEATURES], column features
+ * xc[n][PM_GNN_FEATURES].  Copied to the device, where the indices are
+ * checked: in range, strictly increasing within each row of Ar and each
+ * row of Ac, and Ac holding exactly the entries of Ar.  A graph failing
+ * them is refused (PM_ERR_INVALID_ARG) and the network's graph cleared.
+ * Arrays in page-locked memory (pm_host_alloc) copy at full link speed. */
 int pm_gnn_set_graph(pm_gnn* g, int32_t m, int32_t n, int32_t nnz,
                      const int32_t* rp, const int32_t* ci, const float* ar,
                      const int32_t* cp, const int32_t* ri, const float* ac,

@@ -115,6 +115,8 @@ SIGNATURES = {
     "pm_gen_view": (C.c_int, [C.c_void_p, C.POINTER(pm_lp), C.POINTER(_d), C.POINTER(_d), _d]),
     "pm_gen_free": (None, [C.c_void_p]),
     "pm_gnn_num_params": (C.c_int32, []),
+    "pm_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "pm_host_free": (C.c_int, [C.c_void_p]),
     "pm_gnn_set_segments": (C.c_int, [C.c_void_p, C.c_int32, _i]),
     "pm_gnn_set_omega_targets": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "pm_gnn_forward_omega": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
@@ -798,6 +800,41 @@ def with_dense_rows(lp: LP, k: int) -> LP:
     return LP(lp.c, rp, ci, v, np.concatenate([lp.row_lower, np.full(k, -INF)]),
               np.concatenate([lp.row_upper, np.full(k, cap)]), lp.var_lower, lp.var_upper, lp.sense, lp.offset,
               lp.x_star, y_star, lp.p_star)
+
+
+class HostArena:
+    """Numpy arrays in page-locked host memory (pm_host_alloc), one block per
+    name, grown when a larger array is asked for; an array stays valid
+    until the next request under the same name or close()."""
+
+    def __init__(self):
+        self._blocks: dict[str, tuple[int, int]] = {}
+
+    def array(self, name: str, shape, dtype) -> np.ndarray:
+        dt = np.dtype(dtype)
+        count = int(np.prod(shape, dtype=np.int64))
+        nbytes = max(count * dt.itemsize, 1)
+        blk = self._blocks.get(name)
+        if blk is None or blk[1] < nbytes:
+            if blk is not None:
+                _check(lib().pm_host_free(C.c_void_p(blk[0])))
+            p = C.c_void_p()
+            _check(lib().pm_host_alloc(nbytes, C.byref(p)))
+            blk = (p.value, nbytes)
+            self._blocks[name] = blk
+        buf = (C.c_char * blk[1]).from_address(blk[0])
+        return np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
+
+    def close(self) -> None:
+        for ptr, _ in self._blocks.values():
+            lib().pm_host_free(C.c_void_p(ptr))
+        self._blocks.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def generate(which: int | None = None, seed: int = 1, **params) -> LP:

@@ -493,6 +493,40 @@ using namespace pm;
 namespace pm {
 namespace {
 
+// Aggregation values in the other order, with the graph's checks: per row i
+// (of nrows, CSR rp / ci over ncols), every index in range and strictly
+// increasing, and each entry's value `vals` found at its place in the other
+// orientation (cp / ri, rows ascending within a column, binary search) and
+// written there in `out`: out[q] = vals[k] for the entry at CSR place k and
+// the other orientation's place q.  Any failure sets *bad.
+__global__ __launch_bounds__(256) void k_other_order(int nrows, int ncols, const int* __restrict__ rp,
+                                                     const int* __restrict__ ci, const int* __restrict__ cp,
+                                                     const int* __restrict__ ri, const float* __restrict__ vals,
+                                                     float* __restrict__ out, int* __restrict__ bad) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= nrows) return;
+    int prev = -1;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        const int j = ci[k];
+        if (j <= prev || j >= ncols) {
+            bad[0] = 1;
+            return;
+        }
+        prev = j;
+        int lo = cp[j], hi = cp[j + 1];  // the row i among column j's rows
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ri[mid] < i) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo >= cp[j + 1] || ri[lo] != i) {
+            bad[0] = 1;
+            return;
+        }
+        out[lo] = vals[k];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fill(int n, float* __restrict__ p, float v) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = v;
@@ -857,6 +891,20 @@ int pm_gnn_set_params(pm_gnn* g, const float* in) {
     });
 }
 
+int pm_host_alloc(size_t bytes, void** out) {
+    return pm::guarded([&] {
+        if (!out) pm::fail(PM_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        PM_HIP(hipHostMalloc(out, bytes > 0 ? bytes : 1, hipHostMallocDefault));
+    });
+}
+
+int pm_host_free(void* p) {
+    return pm::guarded([&] {
+        if (p) PM_HIP(hipHostFree(p));
+    });
+}
+
 int pm_gnn_set_graph(pm_gnn* g, int32_t m, int32_t n, int32_t nnz, const int32_t* rp, const int32_t* ci,
                      const float* ar, const int32_t* cp, const int32_t* ri, const float* ac, const float* xr,
                      const float* xc) {
@@ -870,12 +918,7 @@ int pm_gnn_set_graph(pm_gnn* g, int32_t m, int32_t n, int32_t nnz, const int32_t
             if (rp[i] > rp[i + 1]) fail(PM_ERR_INVALID_ARG, "rp decreasing");
         for (int j = 0; j < n; ++j)
             if (cp[j] > cp[j + 1]) fail(PM_ERR_INVALID_ARG, "cp decreasing");
-        for (int k = 0; k < nnz; ++k)
-            if (ci[k] < 0 || ci[k] >= n || ri[k] < 0 || ri[k] >= m) fail(PM_ERR_INVALID_ARG, "index out of range");
         PM_HIP(hipSetDevice(g->device));
-        g->m = m;
-        g->n = n;
-        g->nnz = nnz;
         hipStream_t s = g->s;
         g->rp.upload(rp, static_cast<size_t>(m) + 1, s);
         g->cp.upload(cp, static_cast<size_t>(n) + 1, s);
@@ -883,21 +926,35 @@ int pm_gnn_set_graph(pm_gnn* g, int32_t m, int32_t n, int32_t nnz, const int32_t
         g->ri.upload(ri, nnz, s);
         g->ar.upload(ar, nnz, s);
         g->ac.upload(ac, nnz, s);
-        // Transposed-order copies of the aggregation values for the backward
-        // pass: ar_t (Ar^T, CSR over columns) and ac_t (Ac^T, CSR over rows).
-        std::vector<float> ar_t(nnz), ac_t(nnz);
-        {
-            std::vector<int> cur(cp, cp + n);
-            for (int i = 0; i < m; ++i)
-                for (int k = rp[i]; k < rp[i + 1]; ++k) ar_t[cur[ci[k]]++] = ar[k];
-            std::vector<int> cur2(rp, rp + m);
-            for (int j = 0; j < n; ++j)
-                for (int k = cp[j]; k < cp[j + 1]; ++k) ac_t[cur2[ri[k]]++] = ac[k];
-        }
-        g->ar_t.upload(ar_t, s);
-        g->ac_t.upload(ac_t, s);
         g->xr.upload(xr, static_cast<size_t>(m) * kF, s);
         g->xc.upload(xc, static_cast<size_t>(n) * kF, s);
+        // Transposed-order copies of the aggregation values for the backward
+        // pass, made on the device with the index checks (indices in range,
+        // strictly increasing within each row and column, both orientations
+        // the same entries): ar_t (Ar^T, CSR over columns), ac_t (Ac^T, CSR
+        // over rows).
+        g->ar_t.resize(std::max(nnz, 1));
+        g->ac_t.resize(std::max(nnz, 1));
+        g->flag.resize(1);
+        PM_HIP(hipMemsetAsync(g->flag.get(), 0, sizeof(int), s));
+        if (m > 0 && nnz > 0) {
+            hipLaunchKernelGGL(k_other_order, dim3((m + 255) / 256), dim3(256), 0, s, m, n, g->rp.get(), g->ci.get(),
+                               g->cp.get(), g->ri.get(), g->ar.get(), g->ar_t.get(), g->flag.get());
+            hipLaunchKernelGGL(k_other_order, dim3((n + 255) / 256), dim3(256), 0, s, n, m, g->cp.get(), g->ri.get(),
+                               g->rp.get(), g->ci.get(), g->ac.get(), g->ac_t.get(), g->flag.get());
+            PM_LAUNCHED();
+        }
+        int bad = 0;
+        g->flag.download(&bad, 1, s);
+        PM_HIP(hipStreamSynchronize(s));
+        if (bad) {
+            g->m = g->n = g->nnz = 0;
+            fail(PM_ERR_INVALID_ARG, "graph indices out of range, not increasing within a row or column, or the "
+                                     "two orientations disagree (the graph is cleared)");
+        }
+        g->m = m;
+        g->n = n;
+        g->nnz = nnz;
         pm::gnn_reserve(g, m, n, nnz, true);
         PM_HIP(hipStreamSynchronize(s));
     });

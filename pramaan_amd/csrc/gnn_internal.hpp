@@ -17,6 +17,7 @@ struct pm_gnn {
     pm::DevBuf<short> wt, wn;  // 8 dense weights x 4096 bf16 each, two layouts
     pm::DevBuf<int> rp, ci, cp, ri;
     pm::DevBuf<float> ar, ac, ar_t, ac_t, xr, xc;
+    pm::DevBuf<int> flag;  // graph checks of pm_gnn_set_graph
     // activations Hc0..2, Hr0..2, Zr1..2, Zc1..2 stored as bfloat16 bits
     // (every consumer reads them in bfloat16: the MFMA layers, the message
     // passing, the heads); gradients and everything else in FP32

@@ -209,19 +209,14 @@ def _scales(lp):
     return s, cs
 
 
-def _sample_graph(ts: TrainingSet, k: int):
-    pi, w = ts.index[k]
-    rh = ts.plans[pi]
-    lp = rh.window(w)
-    return W.graph_of(lp, rh.cols_per_period, rh.rows_per_period)
-
-
-def _minibatch(ts, idx, pool):
-    """Block-diagonal graph of the windows idx, their x and y targets, each
-    window's column offset and primal-weight target."""
-    gs = list(pool.map(lambda k: _sample_graph(ts, k), idx))
-    bg = W.batch_graphs(gs)
-    bg["seg"] = np.concatenate([[0], np.cumsum([g["n"] for g in gs])]).astype(np.int32)
+def _minibatch(ts, idx, pool, arena=None):
+    """Block-diagonal graph of the windows idx (built in place, in
+    parallel, in `arena`'s page-locked memory when given), their x and y
+    targets, each window's column offset and primal-weight target."""
+    lps = list(pool.map(lambda k: ts.plans[ts.index[k][0]].window(ts.index[k][1]), idx))
+    rh = ts.plans[ts.index[idx[0]][0]]  # every plan shares the period layout
+    bg = W.batch_graphs_of(lps, rh.cols_per_period, rh.rows_per_period, pool, arena)
+    bg["seg"] = np.concatenate([[0], np.cumsum([lp.n for lp in lps])]).astype(np.int32)
     bg["tw"] = np.array([ts.tw[k] for k in idx], np.float32)
     return bg, np.concatenate([ts.tx[k] for k in idx]), np.concatenate([ts.ty[k] for k in idx])
 
@@ -262,18 +257,21 @@ def train(net, ts, epochs=8, batch=16, lr=2e-3, val_frac=0.05, log=print, seed=0
     t_train = 0.0
     hist = []
     # One thread assembles the next mini-batch (its windows built by the
-    # pool) while the GPU trains on the current one.
+    # pool) while the GPU trains on the current one; the two alternate
+    # between two page-locked arenas (a graph's upload has finished when
+    # pm_gnn_set_graph returns, before its arena is built into again).
     t_wall = 0.0
+    arenas = [P.HostArena(), P.HostArena()]
     with ThreadPoolExecutor(threads) as pool, ThreadPoolExecutor(1) as ahead:
         for ep in range(epochs):
             perm = rng.permutation(tr)
             losses = []
             t_ep = time.perf_counter()
-            fut = ahead.submit(_minibatch, ts, perm[:batch], pool) if len(perm) else None
+            fut = ahead.submit(_minibatch, ts, perm[:batch], pool, arenas[0]) if len(perm) else None
             for s in range(0, len(perm), batch):
                 bg, tx, ty = fut.result()
                 nxt = perm[s + batch:s + 2 * batch]
-                fut = ahead.submit(_minibatch, ts, nxt, pool) if len(nxt) else None
+                fut = ahead.submit(_minibatch, ts, nxt, pool, arenas[(s // batch + 1) % 2]) if len(nxt) else None
                 step_lr = lr * 0.5 * (1.0 + np.cos(np.pi * step / max(1, steps_total)))
                 t0 = time.perf_counter()
                 _load(net, bg)

@@ -67,6 +67,65 @@ def batch_graphs(graphs: list[dict]) -> dict:
     return res
 
 
+def batch_graphs_of(lps: list, period_cols: int | None = None, period_rows: int | None = None,
+                    pool=None, arena=None) -> dict:
+    """batch_graphs([graph_of(lp, ...) for lp in lps]) built in place: every
+    LP's graph is written by pm_gnn_lp_features straight into its slices of
+    the combined arrays, offsets added there, so nothing is concatenated
+    afterwards; `pool` (a thread pool) builds the LPs' slices in parallel;
+    `arena` (a _capi.HostArena) places the arrays in page-locked memory."""
+    from . import _capi as P
+    ms = np.array([lp.m for lp in lps], np.int64)
+    ns = np.array([lp.n for lp in lps], np.int64)
+    zs = np.array([lp.nnz for lp in lps], np.int64)
+    mo = np.concatenate([[0], np.cumsum(ms)])
+    no = np.concatenate([[0], np.cumsum(ns)])
+    zo = np.concatenate([[0], np.cumsum(zs)])
+    M, N, Z = int(mo[-1]), int(no[-1]), int(zo[-1])
+    new = (lambda name, shape, dt: arena.array(name, shape, dt)) if arena is not None else \
+        (lambda name, shape, dt: np.empty(shape, dt))
+    rp = new("rp", (M + 1,), np.int32)
+    cp = new("cp", (N + 1,), np.int32)
+    ci = new("ci", (Z,), np.int32)
+    ri = new("ri", (Z,), np.int32)
+    ar = new("ar", (Z,), np.float32)
+    ac = new("ac", (Z,), np.float32)
+    xr = new("xr", (M, FEAT), np.float32)
+    xc = new("xc", (N, FEAT), np.float32)
+
+    def one(k):
+        lp = lps[k]
+        m, n, nnz = lp.m, lp.n, lp.nnz
+        z0, r0, c0 = int(zo[k]), int(mo[k]), int(no[k])
+        cpk = np.zeros(n + 1, np.int32)
+        sc = np.zeros(2)
+        pad_i = np.zeros(1, np.int32) if nnz == 0 else None
+        pad_f = np.zeros(1, np.float32) if nnz == 0 else None
+        s = lp.struct()
+        P._check(P.lib().pm_gnn_lp_features(
+            P.C.byref(s), period_cols or 0, period_rows or 0, P._p(cpk, P.C.c_int32),
+            P._p(ri[z0:] if nnz else pad_i, P.C.c_int32), P._p(ar[z0:] if nnz else pad_f, P.C.c_float),
+            P._p(ac[z0:] if nnz else pad_f, P.C.c_float), P._p(xr[r0:] if m else np.zeros((1, FEAT), np.float32),
+                                                                P.C.c_float),
+            P._p(xc[c0:], P.C.c_float), P._p(sc, P.C.c_double)))
+        ri[z0:z0 + nnz] += r0
+        ci[z0:z0 + nnz] = lp.col_idx
+        ci[z0:z0 + nnz] += c0
+        rp[r0:r0 + m] = lp.row_ptr[:-1]
+        rp[r0:r0 + m] += z0
+        cp[c0:c0 + n] = cpk[:-1]
+        cp[c0:c0 + n] += z0
+
+    if pool is None:
+        for k in range(len(lps)):
+            one(k)
+    else:
+        list(pool.map(one, range(len(lps))))
+    rp[M] = Z
+    cp[N] = Z
+    return dict(m=M, n=N, rp=rp, ci=ci, ar=ar, cp=cp, ri=ri, ac=ac, xr=xr, xc=xc)
+
+
 def targets(lp: LP, g: dict, x: np.ndarray, y: np.ndarray):
     return x / g["s"], y / g["cs"]
 

@@ -71,3 +71,20 @@ def test_features_reject_bad_lp(capi):
     lp.col_idx[3] = 40
     with pytest.raises(capi.PramaanError):
         W.graph_of(lp)
+
+
+def test_batch_graphs_in_place_match_concatenation(capi):
+    """batch_graphs_of (each LP's graph written straight into its slices of
+    the combined arrays, in parallel) gives the arrays of batch_graphs over
+    graph_of, an LP without rows included."""
+    from concurrent.futures import ThreadPoolExecutor
+    from pramaan_amd import warmstart as W
+    rh = capi.RollingHorizon()
+    lps = [rh.window(w) for w in (0, 1, 250)]
+    lps.insert(1, capi.LP.from_dense(np.zeros((0, 3)), [1.0, -1.0, 2.0], [], [], np.zeros(3), np.ones(3)))
+    want = W.batch_graphs([W.graph_of(lp, rh.cols_per_period, rh.rows_per_period) for lp in lps])
+    with ThreadPoolExecutor(4) as pool:
+        got = W.batch_graphs_of(lps, rh.cols_per_period, rh.rows_per_period, pool)
+    assert (got["m"], got["n"]) == (want["m"], want["n"])
+    for k in ("rp", "ci", "ar", "cp", "ri", "ac", "xr", "xc"):
+        assert np.array_equal(got[k], want[k]), k

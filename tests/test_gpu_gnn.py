@@ -123,3 +123,35 @@ def test_training_reduces_loss(capi):
     for _ in range(200):
         last = net.train_step(tx, ty, lr=3e-3)
     assert last < 0.5 * first
+
+
+def test_bad_graphs_are_rejected(capi):
+    """pm_gnn_set_graph checks the graph on the device while it builds the
+    transposed value orders: an index out of range, a row whose columns do
+    not increase, or a column order that is not the transpose of the row
+    order fails with PM_ERR_INVALID_ARG and clears the graph; a good graph
+    loads again afterwards."""
+    _, g, tx, ty = small_problem(capi, seed=7)
+    net = capi.Gnn(seed=8)
+    net.set_graph(g)
+    good, _ = net.gradient(tx, ty)
+    bad = dict(g)
+    bad["ci"] = g["ci"].copy()
+    bad["ci"][5] = g["n"] + 3
+    swapped = dict(g)                     # two columns of a row out of order
+    swapped["ci"] = g["ci"].copy()
+    r = next(i for i in range(g["m"]) if g["rp"][i + 1] - g["rp"][i] >= 2)
+    a = int(g["rp"][r])
+    swapped["ci"][a], swapped["ci"][a + 1] = swapped["ci"][a + 1], swapped["ci"][a]
+    moved = dict(g)                       # a column entry that the rows do not hold
+    moved["ri"] = g["ri"].copy()
+    c = next(j for j in range(g["n"]) if g["cp"][j + 1] - g["cp"][j] >= 1)
+    q = int(g["cp"][c])
+    moved["ri"][q] = (int(g["ri"][q]) + 1) % g["m"]
+    for case in (bad, swapped, moved):
+        with pytest.raises(capi.PramaanError) as e:
+            net.set_graph(case)
+        assert e.value.code == 1
+    net.set_graph(g)
+    again, _ = net.gradient(tx, ty)
+    assert again == good
