@@ -52,6 +52,7 @@ void check_csr(const int32_t* rp, const int32_t* ci, const float* v, int32_t nnz
 // Runs y[rows] = M x over a host CSR (rows x cols) on the null stream.
 void run_spmv(const int32_t* rp, const int32_t* ci, const float* v, int32_t nnz, const float* x, float* y,
               int32_t rows, int32_t cols, int32_t variant) {
+    const int var = pick_variant(variant, rows, nnz);  // checked before the device is touched
     if (rows == 0) return;
     PM_HIP(hipSetDevice(0));
     hipStream_t s = nullptr;
@@ -62,7 +63,6 @@ void run_spmv(const int32_t* rp, const int32_t* ci, const float* v, int32_t nnz,
     d_v.upload(v, nnz, s);
     d_x.upload(x, cols, s);
     if (cols == 0) d_x.resize(1);
-    const int var = pick_variant(variant, rows, nnz);
     if (var == PM_SPMV_MERGE) {
         const size_t nt = static_cast<size_t>(merge_tiles(rows, nnz));
         DevBuf<int> prow(nt + 1);
