@@ -184,14 +184,27 @@ __global__ __launch_bounds__(256) void k_colsum(int N, int rows_per, const float
     }
 }
 
-// out[e] (+)= sum_b P[b][e], fixed order.
+// out[e] = sum_b P[b][e] in a fixed order: a workgroup takes 16 outputs,
+// and slice q (of 16) sums b = q, q + 16, ...; the 16 slice sums are then
+// added in slice order.  Grid: kSumBlock(width) workgroups.
+constexpr int kSumLanes = 16;
+constexpr int kSumSlices = 16;
 __global__ __launch_bounds__(256) void k_sum_partials(int nb, int width, const float* __restrict__ P,
                                                       float* __restrict__ out) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= width) return;
+    const int el = threadIdx.x % kSumLanes, q = threadIdx.x / kSumLanes;
+    const int e = blockIdx.x * kSumLanes + el;
     float s = 0.0f;
-    for (int b = 0; b < nb; ++b) s += P[static_cast<size_t>(b) * width + e];
-    out[e] = s;
+    if (e < width)
+        for (int b = q; b < nb; b += kSumSlices) s += P[static_cast<size_t>(b) * width + e];
+    __shared__ float red[kSumSlices][kSumLanes];
+    red[q][el] = s;
+    __syncthreads();
+    if (q == 0 && e < width) {
+        float t = 0.0f;
+#pragma unroll
+        for (int r = 0; r < kSumSlices; ++r) t += red[r][el];
+        out[e] = t;
+    }
 }
 
 // Out[i][:] (+)= sum_k val[k] H[idx[k]][:], eight lanes per row (eight
@@ -288,22 +301,31 @@ __global__ __launch_bounds__(256) void k_input(int N, const float* __restrict__ 
 }
 
 // Primal-weight head, one workgroup per segment (LP) k: pool[k][c] = mean
-// of H[j][c] over the segment's columns (four column groups summed in LDS
-// in a fixed order), what[k] = pool[k] . w + b.
+// of H[j][c] over the segment's columns, what[k] = pool[k] . w + b.  Lane
+// t takes features 8 (t % 8) .. 8 (t % 8) + 7 of rows j0 + t / 8, j0 + t / 8
+// + 32, ... with one 16-byte load per row, so a workgroup keeps 32 rows in
+// flight; the 32 row-lane sums are added in LDS in a fixed order.
 __global__ __launch_bounds__(256) void k_pool_head(const int* __restrict__ seg, const short* __restrict__ H,
                                                    const float* __restrict__ w, const float* __restrict__ b,
                                                    float* __restrict__ pool, float* __restrict__ what) {
-    const int k = blockIdx.x, c = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int k = blockIdx.x, fg = threadIdx.x & 7, rl = threadIdx.x >> 3;
     const int j0 = seg[k], j1 = seg[k + 1];
-    float s = 0.0f;
-    for (int j = j0 + q; j < j1; j += 4) s += bf2f(H[static_cast<size_t>(j) * kH + c]);
-    __shared__ float red[4][kH];
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int j = j0 + rl; j < j1; j += 32) {
+        const s16x8 h = *reinterpret_cast<const s16x8*>(H + static_cast<size_t>(j) * kH + 8 * fg);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[q] += bf2f(h[q]);
+    }
+    __shared__ float red[32][kH];
     __shared__ float dot[kH];
-    red[q][c] = s;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) red[rl][8 * fg + q] = s[q];
     __syncthreads();
-    if (q == 0) {
-        const int cnt = j1 - j0;
-        const float mean = cnt > 0 ? (red[0][c] + red[1][c] + red[2][c] + red[3][c]) / static_cast<float>(cnt) : 0.0f;
+    if (threadIdx.x < kH) {
+        const int c = threadIdx.x, cnt = j1 - j0;
+        float t = 0.0f;
+        for (int r = 0; r < 32; ++r) t += red[r][c];
+        const float mean = cnt > 0 ? t / static_cast<float>(cnt) : 0.0f;
         pool[static_cast<size_t>(k) * kH + c] = mean;
         dot[c] = mean * w[c];
     }
@@ -587,7 +609,13 @@ void spmm(pm_gnn* g, int nrows, const int* rp, const int* idx, const float* val,
     PM_LAUNCHED();
 }
 
-constexpr int kRowsPer = 2048;  // rows per workgroup in the gradient reductions
+constexpr int kRowsPer = 512;  // rows per workgroup in the gradient reductions
+
+// out[0 .. width) = sum over nb partial rows of P (k_sum_partials).
+void sum_partials(pm_gnn* g, int nb, int width, const float* P, float* out) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(width, kSumLanes)), dim3(256), 0, g->s, nb, width, P, out);
+    PM_LAUNCHED();
+}
 
 // grad[off .. off+4096) = A^T D summed over all N rows.
 void wgrad(pm_gnn* g, int N, const short* A, const float* D, int off) {
@@ -595,9 +623,7 @@ void wgrad(pm_gnn* g, int N, const short* A, const float* D, int off) {
     const int nb = cdiv(N, kRowsPer);
     hipLaunchKernelGGL(k_wgrad, dim3(nb), dim3(256), 0, g->s, N, kRowsPer, A, D, g->part.get());
     PM_LAUNCHED();
-    hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(kH * kH, 256)), dim3(256), 0, g->s, nb, kH * kH, g->part.get(),
-                       g->grad.get() + off);
-    PM_LAUNCHED();
+    sum_partials(g, nb, kH * kH, g->part.get(), g->grad.get() + off);
 }
 
 // grad[boff .. +64) = column sums of D; with X: grad[xoff .. +512) = X^T D.
@@ -612,13 +638,8 @@ void colsum(pm_gnn* g, int N, const float* D, const float* X, int boff, int xoff
     float* P2 = P + static_cast<size_t>(nb) * kH;
     hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, g->s, N, kRowsPer, D, X, P, X ? P2 : nullptr);
     PM_LAUNCHED();
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g->s, nb, kH, P, g->grad.get() + boff);
-    PM_LAUNCHED();
-    if (X) {
-        hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(kF * kH, 256)), dim3(256), 0, g->s, nb, kF * kH, P2,
-                           g->grad.get() + xoff);
-        PM_LAUNCHED();
-    }
+    sum_partials(g, nb, kH, P, g->grad.get() + boff);
+    if (X) sum_partials(g, nb, kF * kH, P2, g->grad.get() + xoff);
 }
 
 void head_grads(pm_gnn* g, int N, const float* d, const short* Hin, int woff_, int boff) {
@@ -632,10 +653,8 @@ void head_grads(pm_gnn* g, int N, const float* d, const short* Hin, int woff_, i
     float* Pb = P + static_cast<size_t>(nb) * kH;
     hipLaunchKernelGGL(k_head_wgrad, dim3(nb), dim3(256), 0, g->s, N, kRowsPer, d, Hin, P, Pb);
     PM_LAUNCHED();
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g->s, nb, kH, P, g->grad.get() + woff_);
-    PM_LAUNCHED();
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g->s, nb, 1, Pb, g->grad.get() + boff);
-    PM_LAUNCHED();
+    sum_partials(g, nb, kH, P, g->grad.get() + woff_);
+    sum_partials(g, nb, 1, Pb, g->grad.get() + boff);
 }
 
 void relu_bwd(pm_gnn* g, int N, const short* Hout, float* dH) {
@@ -686,9 +705,7 @@ float backward(pm_gnn* g) {
                            part + (n > 0 ? nbx : 0));
     PM_LAUNCHED();
     float* loss_dev = g->grad.get() + kOff.total;  // one spare slot after the gradient
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g->s, (n > 0 ? nbx : 0) + (m > 0 ? nby : 0), 1, part,
-                       loss_dev);
-    PM_LAUNCHED();
+    sum_partials(g, (n > 0 ? nbx : 0) + (m > 0 ? nby : 0), 1, part, loss_dev);
     head_grads(g, n, g->dx.get(), g->Hc[2].get(), kOff.wx, kOff.bx);
     head_grads(g, m, g->dy.get(), g->Hr[2].get(), kOff.wy, kOff.by);
     float* dHc = g->dHc.get();
